@@ -1,0 +1,105 @@
+/*
+ * CPU oracle for the DICe local subset-correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker: only the
+ * test suite, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The product library (libdice_hip.so) never links, loads or calls it.
+ *
+ * It is a from-scratch restatement of the reference algorithm (DICe, NTESS,
+ * BSD-3) written from the reference's mathematics and control flow; every
+ * function cites the reference file:line it follows.  It is templated on the
+ * scalar type so that the same source gives the reference's default build
+ * (scalar_t = float, FP64 Hessian accumulation) and its DICE_USE_DOUBLE build.
+ */
+#ifndef DICE_ORACLE_H
+#define DICE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as dice_params in include/dice_hip.h (checked by the tests). */
+typedef struct oracle_params {
+  int32_t interpolation_method;          /* BILINEAR=0 BICUBIC=1 KEYS_FOURTH=2 */
+  int32_t optimization_method;           /* SIMPLEX=0 GRADIENT_BASED=1 GRADIENT_THEN_SEARCH=2
+                                            SIMPLEX_THEN_GRADIENT_BASED=3 GRADIENT_BASED_THEN_SIMPLEX=4 */
+  int32_t correlation_routine;           /* GENERIC_ROUTINE=0 TRACKING_ROUTINE=1 */
+  int32_t initialization_method;         /* USE_FIELD_VALUES=0 USE_NEIGHBOR_VALUES=1 */
+  int32_t enable_translation;
+  int32_t enable_rotation;
+  int32_t enable_normal_strain;
+  int32_t enable_shear_strain;
+  int32_t max_solver_iterations_fast;
+  int32_t max_solver_iterations_robust;
+  double fast_solver_tolerance;
+  double robust_solver_tolerance;
+  double robust_delta_disp;
+  double robust_delta_theta;
+  double skip_solve_gamma_threshold;
+  double initial_gamma_threshold;
+  double final_gamma_threshold;
+  double disp_jump_tol;
+  double theta_jump_tol;
+  double levenberg_marquardt_regularization_factor;
+  double momentum_factor;
+  int32_t compute_def_gradients;
+  int32_t normalize_gamma_with_active_pixels;
+  int32_t compute_uncertainty_fields;
+  int32_t gradient_method;               /* FINITE_DIFFERENCE=0 CONVOLUTION_5_POINT=1 */
+} oracle_params;
+
+/* Field slots of the per-subset SoA double store: fields[slot*n + i]. */
+enum oracle_field {
+  OF_COORD_X = 0, OF_COORD_Y, OF_DISP_X, OF_DISP_Y, OF_ROT_Z, OF_NS_XX, OF_NS_YY, OF_SS_XY,
+  OF_SIGMA, OF_GAMMA, OF_BETA, OF_MATCH, OF_NOISE, OF_CONTRAST, OF_ACTIVE_PIXELS,
+  OF_STATUS, OF_ITERATIONS, OF_CONDITION, OF_NEIGHBOR_ID,
+  OF_FIELD_1, OF_FIELD_2, OF_FIELD_3, OF_FIELD_4, OF_FIELD_5, OF_FIELD_6, OF_FIELD_7,
+  OF_COUNT
+};
+
+void oracle_default_params(oracle_params *p);          /* dice_default_params */
+void oracle_tracking_default_params(oracle_params *p); /* tracking_default_params */
+
+/* Image-level pieces (S = float for *_f, double for *_d). */
+void oracle_fd_gradients_f(int w, int h, const float *img, float *gx, float *gy);
+void oracle_fd_gradients_d(int w, int h, const double *img, double *gx, double *gy);
+void oracle_smooth_gradients_f(int w, int h, float *gx, float *gy);
+void oracle_interp_f(int method, int w, int h, const float *img, int n, const float *x, const float *y, float *out);
+void oracle_interp_d(int method, int w, int h, const double *img, int n, const double *x, const double *y, double *out);
+void oracle_apply_transform_d(int w, int h, const double *in, double *out, int cx, int cy,
+                              double u, double v, double theta);
+void oracle_apply_transform_f(int w, int h, const float *in, float *out, int cx, int cy,
+                              float u, float v, float theta);
+/* Un-warped subset gamma (Subset::initialize without a map, then Subset::gamma). */
+double oracle_subset_gamma_f(int w, int h, const float *ref, const float *def, int cx, int cy, int size);
+/* Polygon::get_owned_pixels: writes up to cap (x,y) pairs in (y,x) order; returns the count. */
+int oracle_polygon_pixels(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y);
+
+/*
+ * One Schema::execute_correlation (GENERIC_ROUTINE) over n subsets.
+ *   storage: 0 = float32 pixels, 1 = uint16 pixels, 2 = float64 pixels (double oracle only).
+ *   square subsets when conf_offsets == NULL (size = subset_size), otherwise conformal pixel lists
+ *   (pixels of subset i are conf_x/conf_y/conf_active[conf_offsets[i] .. conf_offsets[i+1])).
+ *   fields: OF_COUNT*n doubles, read (coordinates, initial guess, sigma) and written.
+ *   first_frame: 1 when frame_id == first_frame_id (enables the field-of-view check).
+ *   threads: OpenMP threads (>=1) over an even contiguous split of the subsets.
+ * Returns 0, or -1 on a bad argument.
+ */
+int oracle_correlate_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int n, int subset_size, const int *conf_offsets, const int *conf_x,
+                       const int *conf_y, const unsigned char *conf_active, double *fields,
+                       int first_frame, int threads);
+int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int n, int subset_size, const int *conf_offsets, const int *conf_x,
+                       const int *conf_y, const unsigned char *conf_active, double *fields,
+                       int first_frame, int threads);
+
+/* 6x6 LU with partial pivoting + inverse (GETRF/GETRI semantics); returns INFO. */
+int oracle_lu_inverse(int n, double *a_rowmajor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  Nothing in the product imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+# Field slots of the per-subset SoA store (enum oracle_field).
+FIELDS = ["COORD_X", "COORD_Y", "DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "SS_XY",
+          "SIGMA", "GAMMA", "BETA", "MATCH", "NOISE", "CONTRAST", "ACTIVE_PIXELS",
+          "STATUS", "ITERATIONS", "CONDITION", "NEIGHBOR_ID",
+          "FIELD_1", "FIELD_2", "FIELD_3", "FIELD_4", "FIELD_5", "FIELD_6", "FIELD_7"]
+F = {name: i for i, name in enumerate(FIELDS)}
+NUM_FIELDS = len(FIELDS)
+
+
+class Params(ctypes.Structure):
+    """Same layout as oracle_params / dice_params."""
+    _fields_ = [
+        ("interpolation_method", ctypes.c_int32),
+        ("optimization_method", ctypes.c_int32),
+        ("correlation_routine", ctypes.c_int32),
+        ("initialization_method", ctypes.c_int32),
+        ("enable_translation", ctypes.c_int32),
+        ("enable_rotation", ctypes.c_int32),
+        ("enable_normal_strain", ctypes.c_int32),
+        ("enable_shear_strain", ctypes.c_int32),
+        ("max_solver_iterations_fast", ctypes.c_int32),
+        ("max_solver_iterations_robust", ctypes.c_int32),
+        ("fast_solver_tolerance", ctypes.c_double),
+        ("robust_solver_tolerance", ctypes.c_double),
+        ("robust_delta_disp", ctypes.c_double),
+        ("robust_delta_theta", ctypes.c_double),
+        ("skip_solve_gamma_threshold", ctypes.c_double),
+        ("initial_gamma_threshold", ctypes.c_double),
+        ("final_gamma_threshold", ctypes.c_double),
+        ("disp_jump_tol", ctypes.c_double),
+        ("theta_jump_tol", ctypes.c_double),
+        ("levenberg_marquardt_regularization_factor", ctypes.c_double),
+        ("momentum_factor", ctypes.c_double),
+        ("compute_def_gradients", ctypes.c_int32),
+        ("normalize_gamma_with_active_pixels", ctypes.c_int32),
+        ("compute_uncertainty_fields", ctypes.c_int32),
+        ("gradient_method", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        _lib.oracle_correlate_f.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i]
+        _lib.oracle_correlate_d.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i]
+        _lib.oracle_subset_gamma_f.restype = ctypes.c_double
+        _lib.oracle_subset_gamma_f.argtypes = [i, i, P, P, i, i, i]
+        _lib.oracle_apply_transform_d.argtypes = [i, i, P, P, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        _lib.oracle_apply_transform_f.argtypes = [i, i, P, P, i, i, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        _lib.oracle_fd_gradients_f.argtypes = [i, i, P, P, P]
+        _lib.oracle_fd_gradients_d.argtypes = [i, i, P, P, P]
+        _lib.oracle_interp_f.argtypes = [i, i, i, P, i, P, P, P]
+        _lib.oracle_interp_d.argtypes = [i, i, i, P, i, P, P, P]
+        _lib.oracle_polygon_pixels.argtypes = [i, P, P, i, P, P]
+        _lib.oracle_lu_inverse.argtypes = [i, P]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params():
+    p = Params()
+    lib().oracle_default_params(ctypes.byref(p))
+    return p
+
+
+def tracking_default_params():
+    p = Params()
+    lib().oracle_tracking_default_params(ctypes.byref(p))
+    return p
+
+
+def new_fields(cx, cy):
+    """Field store (NUM_FIELDS x n, double) with coordinates set, as a fresh Schema holds it."""
+    n = len(cx)
+    fields = np.zeros((NUM_FIELDS, n), dtype=np.float64)
+    fields[F["COORD_X"]] = cx
+    fields[F["COORD_Y"]] = cy
+    return fields
+
+
+def correlate(params, ref, defm, fields, subset_size=0, conformal=None, first_frame=True,
+              precision="double", threads=1):
+    """One execute_correlation over the subsets in `fields` (modified in place).
+
+    ref/defm: 2-D float32, float64 or uint16 arrays.  conformal: None or a tuple
+    (offsets int32[n+1], x int32[], y int32[], active uint8[]).
+    """
+    h, w = ref.shape
+    if ref.dtype == np.uint16:
+        storage = 1
+    elif ref.dtype == np.float64:
+        storage = 2
+    else:
+        storage = 0
+        ref = ref.astype(np.float32)
+        defm = defm.astype(np.float32)
+    ref = np.ascontiguousarray(ref)
+    defm = np.ascontiguousarray(defm.astype(ref.dtype))
+    assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape[0] == NUM_FIELDS
+    n = fields.shape[1]
+    co = cx = cy = ca = None
+    if conformal is not None:
+        co, cx, cy, ca = [np.ascontiguousarray(a) for a in conformal]
+        co = co.astype(np.int32); cx = cx.astype(np.int32); cy = cy.astype(np.int32); ca = ca.astype(np.uint8)
+    fn = lib().oracle_correlate_d if precision == "double" else lib().oracle_correlate_f
+    rc = fn(ctypes.byref(params), w, h, storage, _p(ref), _p(defm), n, subset_size, _p(co), _p(cx), _p(cy),
+            _p(ca), _p(fields), 1 if first_frame else 0, threads)
+    if rc != 0:
+        raise ValueError("oracle_correlate rejected its arguments")
+    return fields
+
+
+def fd_gradients(img):
+    img = np.ascontiguousarray(img)
+    h, w = img.shape
+    gx = np.zeros_like(img)
+    gy = np.zeros_like(img)
+    fn = lib().oracle_fd_gradients_d if img.dtype == np.float64 else lib().oracle_fd_gradients_f
+    fn(w, h, _p(img), _p(gx), _p(gy))
+    return gx, gy
+
+
+def interp(method, img, x, y):
+    """method: 0 bilinear_all, 2 keys_fourth_all, 3 keys_fourth (single)."""
+    img = np.ascontiguousarray(img)
+    h, w = img.shape
+    x = np.ascontiguousarray(x, dtype=img.dtype)
+    y = np.ascontiguousarray(y, dtype=img.dtype)
+    out = np.zeros(x.shape, dtype=img.dtype)
+    fn = lib().oracle_interp_d if img.dtype == np.float64 else lib().oracle_interp_f
+    fn(method, w, h, _p(img), x.size, _p(x), _p(y), _p(out))
+    return out
+
+
+def apply_transform(img, cx, cy, u, v, theta):
+    img = np.ascontiguousarray(img)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    if img.dtype == np.float64:
+        lib().oracle_apply_transform_d(w, h, _p(img), _p(out), cx, cy, u, v, theta)
+    else:
+        lib().oracle_apply_transform_f(w, h, _p(img), _p(out), cx, cy, u, v, theta)
+    return out
+
+
+def subset_gamma(ref, defm, cx, cy, size):
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    defm = np.ascontiguousarray(defm, dtype=np.float32)
+    h, w = ref.shape
+    return lib().oracle_subset_gamma_f(w, h, _p(ref), _p(defm), cx, cy, size)
+
+
+def polygon_pixels(vx, vy):
+    vx = np.ascontiguousarray(vx, dtype=np.int32)
+    vy = np.ascontiguousarray(vy, dtype=np.int32)
+    cap = (int(vx.max() - vx.min()) + 1) * (int(vy.max() - vy.min()) + 1)
+    ox = np.zeros(cap, np.int32)
+    oy = np.zeros(cap, np.int32)
+    cnt = lib().oracle_polygon_pixels(len(vx), _p(vx), _p(vy), cap, _p(ox), _p(oy))
+    return ox[:cnt], oy[:cnt]
+
+
+def lu_inverse(a):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    info = lib().oracle_lu_inverse(a.shape[0], _p(a))
+    return a, info
