@@ -1,0 +1,744 @@
+// Host side of the C ABI declared in include/dice_hip.h.
+//
+// A context owns the device copies of the reference and deformed images, their
+// gradients, the subset set, a device field store, the work buffers of the
+// correlation passes, one HIP stream and (for multi-frame sequences) a
+// hipGraph.  All entry points return a code and never throw.
+#include "dice_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "dice_kernels.h"
+
+using dice_hip::DImage;
+using dice_hip::DSubsets;
+using dice_hip::KParams;
+using dice_hip::Launch;
+
+namespace {
+
+// dynamic LDS per workgroup of the batched kernel (default launch limit)
+constexpr size_t kMaxLdsBytes = 64 * 1024;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct GraphKey {
+  int mode, n_frames, n;
+  const void *frames0;
+  double *fields, *history;
+  bool operator==(const GraphKey &o) const {
+    return mode == o.mode && n_frames == o.n_frames && n == o.n && frames0 == o.frames0 && fields == o.fields &&
+           history == o.history;
+  }
+};
+
+}  // namespace
+
+struct dice_hip_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  dice_params params{};
+  bool have_params = false;
+  int num_cus = 256;
+  // reference image
+  int rw = 0, rh = 0, rstorage = 0;
+  DevBuf ref_pix, ref_gx, ref_gy;
+  // deformed image
+  int dw = 0, dh = 0, dstorage = 0;
+  DevBuf def_pix;
+  const void *def_ptr = nullptr;
+  DevBuf def_gx, def_gy, tmp_x, tmp_y;
+  bool def_has_grads = false;
+  // subsets
+  int n = 0, S = 0, fov = 0;
+  DevBuf cx, cy, off, px, py, act, bbox, nact;
+  // fields + work
+  DevBuf fields, work;
+  // launch shape of the batched mode
+  Launch batched{1, 4, 0, 4096};
+  std::map<std::tuple<int, int, int, int, int>, Launch> tuned;
+  // timing of correlation launches
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t events_used = 0;
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  GraphKey graph_key{};
+  std::vector<const void *> graph_frames;
+
+  int fail(int code, const char *what, hipError_t e = hipSuccess) {
+    err = what;
+    if (e != hipSuccess) {
+      err += ": ";
+      err += hipGetErrorString(e);
+    }
+    return code;
+  }
+};
+
+namespace {
+
+#define HIP_TRY(ctx, expr, what)                                    \
+  do {                                                              \
+    hipError_t _e = (expr);                                         \
+    if (_e != hipSuccess) return (ctx)->fail(DICE_HIP_EHIP, what, _e); \
+  } while (0)
+
+bool params_valid(const dice_params *p, std::string &why) {
+  if (p->interpolation_method != 0 && p->interpolation_method != 2) {
+    why = "interpolation_method must be BILINEAR (0) or KEYS_FOURTH (2)";
+    return false;
+  }
+  if (p->optimization_method < 0 || p->optimization_method > 4 || p->optimization_method == 2) {
+    why = "optimization_method must be SIMPLEX, GRADIENT_BASED, SIMPLEX_THEN_GRADIENT_BASED or "
+          "GRADIENT_BASED_THEN_SIMPLEX";
+    return false;
+  }
+  if (p->initialization_method != 0) {
+    why = "only USE_FIELD_VALUES initialization is supported";
+    return false;
+  }
+  if (!p->enable_translation) {
+    why = "enable_translation is required by the affine shape function";
+    return false;
+  }
+  if (p->max_solver_iterations_fast < 0 || p->max_solver_iterations_robust < 1) {
+    why = "iteration limits out of range";
+    return false;
+  }
+  if (p->gradient_method != 0 && p->gradient_method != 1) {
+    why = "gradient_method must be FINITE_DIFFERENCE or CONVOLUTION_5_POINT";
+    return false;
+  }
+  return true;
+}
+
+KParams make_kparams(const dice_params &p, int first_frame) {
+  KParams k{};
+  k.interp = p.interpolation_method;
+  k.opt = p.optimization_method;
+  k.en_rot = p.enable_rotation != 0;
+  k.en_normal = p.enable_normal_strain != 0;
+  k.en_shear = p.enable_shear_strain != 0;
+  k.max_fast = p.max_solver_iterations_fast;
+  k.max_robust = p.max_solver_iterations_robust;
+  k.fast_tol = (float)p.fast_solver_tolerance;
+  k.robust_tol = p.robust_solver_tolerance;
+  k.delta_disp = (float)p.robust_delta_disp;
+  k.delta_theta = (float)p.robust_delta_theta;
+  k.skip_thr = (float)p.skip_solve_gamma_threshold;
+  k.init_gamma_thr = p.initial_gamma_threshold;
+  k.final_gamma_thr = p.final_gamma_threshold;
+  k.disp_jump_tol = p.disp_jump_tol;
+  k.theta_jump_tol = p.theta_jump_tol;
+  k.lm_alpha = p.levenberg_marquardt_regularization_factor > 0.0 ? (float)p.levenberg_marquardt_regularization_factor : 0.0f;
+  k.momentum = (float)p.momentum_factor;
+  k.def_grads = p.compute_def_gradients != 0;
+  k.normalize_gamma = p.normalize_gamma_with_active_pixels != 0;
+  k.uncertainty = p.compute_uncertainty_fields != 0;
+  k.first_frame = first_frame != 0;
+  return k;
+}
+
+size_t pix_bytes(int storage) { return storage == DICE_STORAGE_U16 ? 2 : 4; }
+
+int check_ready(dice_hip_context *ctx) {
+  if (!ctx->have_params) return ctx->fail(DICE_HIP_ESTATE, "parameters not set");
+  if (!ctx->ref_pix.p) return ctx->fail(DICE_HIP_ESTATE, "reference image not set");
+  if (!ctx->def_ptr) return ctx->fail(DICE_HIP_ESTATE, "deformed image not set");
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_ESTATE, "subsets not set");
+  if (ctx->rw != ctx->dw || ctx->rh != ctx->dh || ctx->rstorage != ctx->dstorage)
+    return ctx->fail(DICE_HIP_EINVAL, "reference and deformed images differ in size or storage");
+  if (ctx->params.compute_def_gradients && !ctx->def_has_grads)
+    return ctx->fail(DICE_HIP_ESTATE, "compute_def_gradients set after the deformed image; set the image again");
+  return DICE_HIP_OK;
+}
+
+void fill_views(dice_hip_context *ctx, DImage &ref, DImage &def, DSubsets &ss) {
+  ref = DImage{ctx->ref_pix.p, ctx->ref_gx.as<float>(), ctx->ref_gy.as<float>(), ctx->rw, ctx->rh, 0, 0};
+  def = DImage{ctx->def_ptr, ctx->def_has_grads ? ctx->def_gx.as<float>() : nullptr,
+               ctx->def_has_grads ? ctx->def_gy.as<float>() : nullptr, ctx->dw, ctx->dh, 0, 0};
+  ss = DSubsets{ctx->n,
+                ctx->S,
+                ctx->fov,
+                ctx->cx.as<int>(),
+                ctx->cy.as<int>(),
+                ctx->S > 0 ? nullptr : ctx->off.as<int>(),
+                ctx->S > 0 ? nullptr : ctx->px.as<int>(),
+                ctx->S > 0 ? nullptr : ctx->py.as<int>(),
+                ctx->S > 0 ? nullptr : ctx->act.as<uint8_t>(),
+                ctx->S > 0 ? nullptr : ctx->bbox.as<int4>(),
+                ctx->S > 0 ? nullptr : ctx->nact.as<int>()};
+}
+
+Launch current_launch(dice_hip_context *ctx, int mode) {
+  if (mode == DICE_MODE_PLAIN) return Launch{0, 0, 0, 0};
+  Launch L = ctx->batched;
+  L.mode = 1;
+  if (L.groups <= 0) {
+    // persistent grid: enough workgroups to fill every CU at the occupancy the
+    // kernel allows, capped by the work available
+    int per_cu = 2;
+    L.groups = ctx->num_cus * per_cu;
+  }
+  const int max_groups = (ctx->n + L.waves_per_group - 1) / L.waves_per_group;
+  if (L.groups > max_groups) L.groups = max_groups > 0 ? max_groups : 1;
+  return L;
+}
+
+hipEvent_t *next_event_pair(dice_hip_context *ctx) {
+  if (ctx->events_used >= ctx->events.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+    ctx->events.emplace_back(a, b);
+  }
+  return &ctx->events[ctx->events_used++].first;
+}
+
+int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int first_frame, bool capturing) {
+  DImage ref, def;
+  DSubsets ss;
+  fill_views(ctx, ref, def, ss);
+  const KParams kp = make_kparams(ctx->params, first_frame);
+  const Launch L = current_launch(ctx, mode);
+  // per-launch timing events on the context stream (eager launches only; a captured
+  // graph is timed as a whole by the caller)
+  const bool timed = ctx->timing && !capturing;
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (timed) {
+    hipEvent_t *p = next_event_pair(ctx);
+    if (!p) return ctx->fail(DICE_HIP_EHIP, "hipEventCreate failed");
+    ev = ctx->events[ctx->events_used - 1];
+    HIP_TRY(ctx, hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
+  }
+  HIP_TRY(ctx, dice_hip::launch_correlate(L, ctx->rstorage, kp, ref, def, ss, d_fields, ctx->work.as<int>(), ctx->stream),
+          "correlation launch");
+  if (timed) HIP_TRY(ctx, hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
+  return DICE_HIP_OK;
+}
+
+void drop_graph(dice_hip_context *ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  ctx->graph_exec = nullptr;
+  ctx->graph_key = GraphKey{};
+}
+
+int upload_image(dice_hip_context *ctx, DevBuf &buf, int w, int h, int storage, const void *host) {
+  const size_t bytes = (size_t)w * h * pix_bytes(storage);
+  HIP_TRY(ctx, buf.ensure(bytes), "hipMalloc image");
+  HIP_TRY(ctx, hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy image");
+  return DICE_HIP_OK;
+}
+
+int gradients(dice_hip_context *ctx, const void *pix, int w, int h, int storage, DevBuf &gx, DevBuf &gy) {
+  const size_t n = (size_t)w * h;
+  HIP_TRY(ctx, gx.ensure(n * 4), "hipMalloc gradients");
+  HIP_TRY(ctx, gy.ensure(n * 4), "hipMalloc gradients");
+  const int smooth = ctx->params.gradient_method == 1;
+  if (smooth) {
+    HIP_TRY(ctx, ctx->tmp_x.ensure(n * 4), "hipMalloc gradients");
+    HIP_TRY(ctx, ctx->tmp_y.ensure(n * 4), "hipMalloc gradients");
+  }
+  HIP_TRY(ctx, dice_hip::launch_gradients(storage, pix, w, h, gx.as<float>(), gy.as<float>(), smooth,
+                                           ctx->tmp_x.as<float>(), ctx->tmp_y.as<float>(), ctx->stream),
+          "gradient kernel");
+  return DICE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dice_hip_create(dice_hip_context **out, int device) {
+  if (!out) return DICE_HIP_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DICE_HIP_EHIP;
+  if (device < 0 || device >= count) return DICE_HIP_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return DICE_HIP_EHIP;
+  dice_hip_context *ctx = new dice_hip_context();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DICE_HIP_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  dice_hip_default_params(&ctx->params);
+  ctx->have_params = true;
+  *out = ctx;
+  return DICE_HIP_OK;
+}
+
+int dice_hip_destroy(dice_hip_context *ctx) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  for (auto &e : ctx->events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
+                    &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
+                    &ctx->fields, &ctx->work})
+    b->release();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DICE_HIP_OK;
+}
+
+const char *dice_hip_last_error(const dice_hip_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void dice_hip_default_params(dice_params *p) {
+  // dice_default_params, DICe_ParameterUtilities.cpp:360-420
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->interpolation_method = 2;
+  p->optimization_method = 4;
+  p->correlation_routine = 0;
+  p->initialization_method = 0;
+  p->enable_translation = 1;
+  p->max_solver_iterations_fast = 25;
+  p->max_solver_iterations_robust = 1000;
+  p->fast_solver_tolerance = 1.0e-4;
+  p->robust_solver_tolerance = 1.0e-6;
+  p->robust_delta_disp = 1.0;
+  p->robust_delta_theta = 0.1;
+  p->skip_solve_gamma_threshold = 1.0e-10;
+  p->initial_gamma_threshold = -1.0;
+  p->final_gamma_threshold = -1.0;
+  p->disp_jump_tol = 10000.0;
+  p->theta_jump_tol = 100.0;
+  p->levenberg_marquardt_regularization_factor = -1.0;
+  p->momentum_factor = 0.0;
+  p->compute_uncertainty_fields = 1;
+}
+
+void dice_hip_tracking_default_params(dice_params *p) {
+  // tracking_default_params, DICe_ParameterUtilities.cpp:297-358
+  if (!p) return;
+  dice_hip_default_params(p);
+  p->correlation_routine = 1;
+  p->optimization_method = 0;
+  p->enable_rotation = 1;
+  p->disp_jump_tol = 8.0;
+  p->theta_jump_tol = 0.1;
+}
+
+int dice_hip_set_params(dice_hip_context *ctx, const dice_params *p) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!p) return ctx->fail(DICE_HIP_EINVAL, "null parameters");
+  std::string why;
+  if (!params_valid(p, why)) return ctx->fail(DICE_HIP_EINVAL, why.c_str());
+  const bool regrad = ctx->params.gradient_method != p->gradient_method;
+  ctx->params = *p;
+  ctx->have_params = true;
+  drop_graph(ctx);
+  (void)hipSetDevice(ctx->device);
+  if (regrad && ctx->ref_pix.p) {
+    int rc = gradients(ctx, ctx->ref_pix.p, ctx->rw, ctx->rh, ctx->rstorage, ctx->ref_gx, ctx->ref_gy);
+    if (rc) return rc;
+  }
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_ref_image(dice_hip_context *ctx, int width, int height, int storage, const void *host_pixels) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (width < 8 || height < 8 || !host_pixels || (storage != DICE_STORAGE_F32 && storage != DICE_STORAGE_U16))
+    return ctx->fail(DICE_HIP_EINVAL, "bad reference image");
+  (void)hipSetDevice(ctx->device);
+  drop_graph(ctx);
+  int rc = upload_image(ctx, ctx->ref_pix, width, height, storage, host_pixels);
+  if (rc) return rc;
+  ctx->rw = width;
+  ctx->rh = height;
+  ctx->rstorage = storage;
+  // reference gradients are always computed (compute_ref_gradients, DICe_Schema.cpp:628,759-761)
+  rc = gradients(ctx, ctx->ref_pix.p, width, height, storage, ctx->ref_gx, ctx->ref_gy);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "set_ref_image");
+  return DICE_HIP_OK;
+}
+
+static int set_def_common(dice_hip_context *ctx, int width, int height, int storage, const void *dptr) {
+  ctx->dw = width;
+  ctx->dh = height;
+  ctx->dstorage = storage;
+  ctx->def_ptr = dptr;
+  ctx->def_has_grads = false;
+  if (ctx->params.compute_def_gradients) {
+    int rc = gradients(ctx, dptr, width, height, storage, ctx->def_gx, ctx->def_gy);
+    if (rc) return rc;
+    ctx->def_has_grads = true;
+  }
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_def_image(dice_hip_context *ctx, int width, int height, int storage, const void *host_pixels) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (width < 8 || height < 8 || !host_pixels || (storage != DICE_STORAGE_F32 && storage != DICE_STORAGE_U16))
+    return ctx->fail(DICE_HIP_EINVAL, "bad deformed image");
+  (void)hipSetDevice(ctx->device);
+  int rc = upload_image(ctx, ctx->def_pix, width, height, storage, host_pixels);
+  if (rc) return rc;
+  rc = set_def_common(ctx, width, height, storage, ctx->def_pix.p);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "set_def_image");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_def_image_device(dice_hip_context *ctx, int width, int height, int storage, const void *device_pixels) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (width < 8 || height < 8 || !device_pixels || (storage != DICE_STORAGE_F32 && storage != DICE_STORAGE_U16))
+    return ctx->fail(DICE_HIP_EINVAL, "bad deformed image");
+  (void)hipSetDevice(ctx->device);
+  return set_def_common(ctx, width, height, storage, device_pixels);
+}
+
+int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, const double *coord_y, int subset_size,
+                         const int32_t *conf_offsets, const int32_t *conf_x, const int32_t *conf_y,
+                         const uint8_t *conf_active) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (n < 1 || !coord_x || !coord_y) return ctx->fail(DICE_HIP_EINVAL, "bad subset coordinates");
+  if (!conf_offsets && subset_size < 1) return ctx->fail(DICE_HIP_EINVAL, "subset_size must be >= 1");
+  (void)hipSetDevice(ctx->device);
+  drop_graph(ctx);
+  std::vector<int> hx(n), hy(n);
+  for (int i = 0; i < n; ++i) {
+    // Objective ctor: static_cast<int_t> of the coordinate fields (DICe_Objective.h:88-92)
+    hx[i] = (int)coord_x[i];
+    hy[i] = (int)coord_y[i];
+  }
+  HIP_TRY(ctx, ctx->cx.ensure(n * sizeof(int)), "hipMalloc subsets");
+  HIP_TRY(ctx, ctx->cy.ensure(n * sizeof(int)), "hipMalloc subsets");
+  HIP_TRY(ctx, hipMemcpy(ctx->cx.p, hx.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy subsets");
+  HIP_TRY(ctx, hipMemcpy(ctx->cy.p, hy.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy subsets");
+  ctx->fov = subset_size > 0 ? subset_size : 0;
+  if (!conf_offsets) {
+    ctx->S = subset_size;
+  } else {
+    if (!conf_x || !conf_y) return ctx->fail(DICE_HIP_EINVAL, "conformal subsets need pixel lists");
+    const int total = conf_offsets[n];
+    if (conf_offsets[0] != 0 || total < 1) return ctx->fail(DICE_HIP_EINVAL, "bad conformal offsets");
+    std::vector<int4> bb(n);
+    std::vector<int> na(n);
+    for (int i = 0; i < n; ++i) {
+      const int b = conf_offsets[i], e = conf_offsets[i + 1];
+      if (e <= b) return ctx->fail(DICE_HIP_EINVAL, "empty conformal subset");
+      int4 box{1 << 30, 1 << 30, -(1 << 30), -(1 << 30)};
+      int cnt = 0;
+      for (int k = b; k < e; ++k) {
+        const int x = conf_x[k] < 0 ? 0 : conf_x[k], y = conf_y[k] < 0 ? 0 : conf_y[k];
+        box.x = std::min(box.x, x);
+        box.y = std::min(box.y, y);
+        box.z = std::max(box.z, x);
+        box.w = std::max(box.w, y);
+        cnt += conf_active ? (conf_active[k] != 0) : 1;
+      }
+      bb[i] = box;
+      na[i] = cnt;
+    }
+    HIP_TRY(ctx, ctx->off.ensure((n + 1) * sizeof(int)), "hipMalloc subsets");
+    HIP_TRY(ctx, ctx->px.ensure(total * sizeof(int)), "hipMalloc subsets");
+    HIP_TRY(ctx, ctx->py.ensure(total * sizeof(int)), "hipMalloc subsets");
+    HIP_TRY(ctx, ctx->act.ensure(total), "hipMalloc subsets");
+    HIP_TRY(ctx, ctx->bbox.ensure(n * sizeof(int4)), "hipMalloc subsets");
+    HIP_TRY(ctx, ctx->nact.ensure(n * sizeof(int)), "hipMalloc subsets");
+    std::vector<uint8_t> a(total, 1);
+    if (conf_active) std::memcpy(a.data(), conf_active, total);
+    HIP_TRY(ctx, hipMemcpy(ctx->off.p, conf_offsets, (n + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, hipMemcpy(ctx->px.p, conf_x, total * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, hipMemcpy(ctx->py.p, conf_y, total * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, hipMemcpy(ctx->act.p, a.data(), total, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, hipMemcpy(ctx->bbox.p, bb.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, hipMemcpy(ctx->nact.p, na.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    ctx->S = 0;
+  }
+  ctx->n = n;
+  HIP_TRY(ctx, ctx->work.ensure((size_t)(n + 4) * sizeof(int)), "hipMalloc work");
+  HIP_TRY(ctx, ctx->fields.ensure((size_t)DICE_F_COUNT * n * sizeof(double)), "hipMalloc fields");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
+  if (!device_fields) return ctx->fail(DICE_HIP_EINVAL, "null fields");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  return enqueue_correlate(ctx, mode, device_fields, first_frame, false);
+}
+
+int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int first_frame) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!fields) return ctx->fail(DICE_HIP_EINVAL, "null fields");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t bytes = (size_t)DICE_F_COUNT * ctx->n * sizeof(double);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->fields.p, fields, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy fields");
+  rc = dice_hip_correlate_device(ctx, mode, ctx->fields.as<double>(), first_frame);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(fields, ctx->fields.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy fields");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "correlate");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_correlate_sequence(dice_hip_context *ctx, int mode, int n_frames, const void *const *device_frames,
+                                double *device_fields, double *device_history, int use_graph) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (n_frames < 1 || !device_frames || !device_fields) return ctx->fail(DICE_HIP_EINVAL, "bad sequence arguments");
+  if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
+  (void)hipSetDevice(ctx->device);
+  if (ctx->params.compute_def_gradients)
+    return ctx->fail(DICE_HIP_EINVAL, "sequences need compute_def_gradients off (gradients are per frame)");
+  if (!ctx->def_ptr) {
+    int rc = set_def_common(ctx, ctx->rw, ctx->rh, ctx->rstorage, device_frames[0]);
+    if (rc) return rc;
+  }
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  const size_t per_frame = (size_t)DICE_F_COUNT * ctx->n;
+  auto enqueue_all = [&](bool capturing) -> int {
+    for (int f = 0; f < n_frames; ++f) {
+      ctx->def_ptr = device_frames[f];
+      int r = enqueue_correlate(ctx, mode, device_fields, f == 0 ? 1 : 0, capturing);
+      if (r) return r;
+      if (device_history) {
+        HIP_TRY(ctx, dice_hip::launch_copy_doubles(device_history + f * per_frame, device_fields, per_frame, ctx->stream),
+                "history copy");
+      }
+    }
+    return DICE_HIP_OK;
+  };
+  if (!use_graph) return enqueue_all(false);
+  GraphKey key{mode, n_frames, ctx->n, device_frames[0], device_fields, device_history};
+  std::vector<const void *> frames(device_frames, device_frames + n_frames);
+  if (!(ctx->graph_exec && key == ctx->graph_key && frames == ctx->graph_frames)) {
+    drop_graph(ctx);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    int r = enqueue_all(true);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    HIP_TRY(ctx, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ctx, e, "hipGraphInstantiate");
+    ctx->graph_key = key;
+    ctx->graph_frames = frames;
+  }
+  HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "hipGraphLaunch");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_synchronize(dice_hip_context *ctx) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return DICE_HIP_OK;
+}
+
+void *dice_hip_stream(dice_hip_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int dice_hip_set_kernel_timing(dice_hip_context *ctx, int enable) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  ctx->timing = enable != 0;
+  ctx->events_used = 0;
+  drop_graph(ctx);
+  return DICE_HIP_OK;
+}
+
+int dice_hip_kernel_timing(dice_hip_context *ctx, double *avg_ms) {
+  if (!ctx || !avg_ms) return DICE_HIP_EINVAL;
+  (void)hipStreamSynchronize(ctx->stream);
+  double sum = 0.0;
+  int cnt = 0;
+  for (size_t i = 0; i < ctx->events_used && i < ctx->events.size(); ++i) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->events[i].first, ctx->events[i].second) == hipSuccess) {
+      sum += ms;
+      ++cnt;
+    }
+  }
+  *avg_ms = cnt ? sum / cnt : 0.0;
+  ctx->events_used = 0;
+  return cnt;
+}
+
+int dice_hip_get_launch_shape(dice_hip_context *ctx, int *waves_per_group, int *groups_per_cu, int *lds_floats) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (waves_per_group) *waves_per_group = ctx->batched.waves_per_group;
+  if (groups_per_cu) *groups_per_cu = ctx->batched.groups > 0 ? ctx->batched.groups / ctx->num_cus : 0;
+  if (lds_floats) *lds_floats = ctx->batched.lds_floats_per_wave;
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_launch_shape(dice_hip_context *ctx, int waves_per_group, int groups_per_cu, int lds_floats) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (waves_per_group < 1 || waves_per_group > 4 || groups_per_cu < 0 || groups_per_cu > 64 || lds_floats < 0 ||
+      (size_t)lds_floats * waves_per_group * 4 > kMaxLdsBytes)
+    return ctx->fail(DICE_HIP_EINVAL, "launch shape out of range");
+  ctx->batched = Launch{1, waves_per_group, groups_per_cu * ctx->num_cus, lds_floats};
+  drop_graph(ctx);
+  return DICE_HIP_OK;
+}
+
+int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int first_frame) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!device_fields) return ctx->fail(DICE_HIP_EINVAL, "null fields");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const auto key = std::make_tuple(ctx->n, ctx->S, ctx->params.interpolation_method, ctx->params.optimization_method,
+                                   ctx->rw * 65536 + ctx->rh);
+  auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) {
+    ctx->batched = it->second;
+    drop_graph(ctx);
+    return DICE_HIP_OK;
+  }
+  // scratch copy of the fields: the candidates run on it, the caller's store is untouched
+  const size_t bytes = (size_t)DICE_F_COUNT * ctx->n * sizeof(double);
+  double *scratch = nullptr;
+  HIP_TRY(ctx, hipMalloc(&scratch, bytes), "hipMalloc autotune");
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  Launch best = ctx->batched;
+  float best_ms = 1e30f;
+  // candidate LDS window budgets (floats per wave) cover the warped subset with margin
+  const int side = ctx->S > 0 ? ctx->S : 48;
+  std::vector<int> windows;
+  for (int extra : {8, 16, 28}) {
+    const int w = (side + extra) * (side + extra);
+    if ((size_t)w * 4 <= kMaxLdsBytes) windows.push_back(w);
+  }
+  if (windows.empty()) windows.push_back(0);
+  for (int waves : {1, 2, 4})
+    for (int per_cu : {1, 2, 4, 8})
+      for (int win : windows) {
+        if ((size_t)waves * win * 4 > kMaxLdsBytes) continue;
+        ctx->batched = Launch{1, waves, per_cu * ctx->num_cus, win};
+        if (hipMemcpyAsync(scratch, device_fields, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) continue;
+        (void)hipEventRecord(a, ctx->stream);
+        rc = enqueue_correlate(ctx, DICE_MODE_BATCHED, scratch, first_frame, false);
+        (void)hipEventRecord(b, ctx->stream);
+        if (rc) break;
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->batched;
+        }
+      }
+  ctx->timing = timing;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(scratch);
+  if (rc) return rc;
+  ctx->batched = best;
+  ctx->tuned[key] = best;
+  drop_graph(ctx);
+  return DICE_HIP_OK;
+}
+
+int dice_hip_correlate_points(float *points, int n_points, int subset_size, const float *ref_img, int ref_w, int ref_h,
+                              const float *def_img, int def_w, int def_h, const dice_params *params,
+                              int update_params) {
+  // DICe_api.cpp:51-153: static state across calls, stride-9 points
+  static dice_hip_context *ctx = nullptr;
+  static bool initialized = false;
+  static int n_init = 0;
+  if (ref_w != def_w || ref_h != def_h) return -1;
+  if (subset_size < 1 || n_points < 1 || !points || !ref_img || !def_img) return -1;
+  const int stride = 9;
+  if (!initialized) {
+    if (dice_hip_create(&ctx, 0) != DICE_HIP_OK) return -1;
+    std::vector<double> cx(n_points), cy(n_points);
+    for (int i = 0; i < n_points; ++i) {
+      cx[i] = points[i * stride + 0];
+      cy[i] = points[i * stride + 1];
+    }
+    if (dice_hip_set_subsets(ctx, n_points, cx.data(), cy.data(), subset_size, nullptr, nullptr, nullptr, nullptr))
+      return -1;
+    n_init = n_points;
+  }
+  if (n_points != n_init) return -1;
+  if (!initialized || update_params) {
+    dice_params p;
+    if (params) p = *params;
+    else dice_hip_tracking_default_params(&p);
+    // set_def_image(w, h, array) builds the deformed image with compute_ref_gradients, which is on by
+    // default (DICe_Schema.cpp:339, :628), so the API path always has deformed-image gradients
+    p.compute_def_gradients = 1;
+    if (dice_hip_set_params(ctx, &p) != DICE_HIP_OK) return -1;
+  }
+  if (!initialized) {
+    if (dice_hip_set_ref_image(ctx, ref_w, ref_h, DICE_STORAGE_F32, ref_img) != DICE_HIP_OK) return -1;
+  }
+  const bool first = !initialized;
+  initialized = true;
+  if (dice_hip_set_def_image(ctx, def_w, def_h, DICE_STORAGE_F32, def_img) != DICE_HIP_OK) return -1;
+  std::vector<double> f((size_t)DICE_F_COUNT * n_points, 0.0);
+  auto F = [&](int slot, int i) -> double & { return f[(size_t)slot * n_points + i]; };
+  for (int i = 0; i < n_points; ++i) {
+    F(DICE_F_COORD_X, i) = points[i * stride + 0];
+    F(DICE_F_COORD_Y, i) = points[i * stride + 1];
+    F(DICE_F_DISP_X, i) = points[i * stride + 2];
+    F(DICE_F_DISP_Y, i) = points[i * stride + 3];
+    F(DICE_F_ROT_Z, i) = points[i * stride + 4];
+    F(DICE_F_SIGMA, i) = points[i * stride + 5];
+    F(DICE_F_GAMMA, i) = points[i * stride + 6];
+    F(DICE_F_BETA, i) = points[i * stride + 7];
+    F(DICE_F_STATUS, i) = points[i * stride + 8];
+  }
+  if (dice_hip_correlate(ctx, DICE_MODE_BATCHED, f.data(), first ? 1 : 0) != DICE_HIP_OK) return -1;
+  for (int i = 0; i < n_points; ++i) {
+    points[i * stride + 2] = (float)F(DICE_F_DISP_X, i);
+    points[i * stride + 3] = (float)F(DICE_F_DISP_Y, i);
+    points[i * stride + 4] = (float)F(DICE_F_ROT_Z, i);
+    points[i * stride + 5] = (float)F(DICE_F_SIGMA, i);
+    points[i * stride + 6] = (float)F(DICE_F_GAMMA, i);
+    points[i * stride + 7] = (float)F(DICE_F_BETA, i);
+    points[i * stride + 8] = (float)F(DICE_F_STATUS, i);
+  }
+  return 0;
+}
+
+}  // extern "C"

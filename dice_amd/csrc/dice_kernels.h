@@ -1,0 +1,98 @@
+// Internal interface between the C-ABI host code and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dice_hip {
+
+constexpr int kWave = 64;
+constexpr int kPlainThreads = 256;
+constexpr int kNumFields = 26;  // DICE_F_COUNT
+
+// Status_Flag values, DICe.h:609-677.
+enum Status : int {
+  CORRELATION_SUCCESSFUL = 0,
+  INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL = 1,
+  INITIALIZE_FAILED = 5,
+  CORRELATION_FAILED = 8,
+  MAX_ITERATIONS_REACHED = 11,
+  INITIALIZE_FAILED_BY_EXCEPTION = 12,
+  JUMP_TOLERANCE_EXCEEDED = 16,
+  HESSIAN_SINGULAR = 21,
+  FRAME_FAILED_DUE_TO_HIGH_GAMMA = 23,
+  FRAME_FAILED_DUE_TO_NEGATIVE_SIGMA = 24,
+};
+
+// Field slots (enum dice_field).
+enum Field : int {
+  F_COORD_X = 0, F_COORD_Y, F_DISP_X, F_DISP_Y, F_ROT_Z, F_NS_XX, F_NS_YY, F_SS_XY,
+  F_SIGMA, F_GAMMA, F_BETA, F_MATCH, F_NOISE, F_CONTRAST, F_ACTIVE_PIXELS,
+  F_STATUS, F_ITERATIONS, F_CONDITION, F_NEIGHBOR_ID,
+  F_FIELD_1, F_FIELD_2, F_FIELD_3, F_FIELD_4, F_FIELD_5, F_FIELD_6, F_FIELD_7,
+};
+
+// Solver options resolved for the device (all wave-uniform).
+struct KParams {
+  int interp;            // 0 bilinear, 2 keys
+  int opt;               // optimization_method
+  int en_rot, en_normal, en_shear;
+  int max_fast, max_robust;
+  float fast_tol;
+  double robust_tol;
+  float delta_disp, delta_theta;
+  float skip_thr;
+  double init_gamma_thr, final_gamma_thr;
+  double disp_jump_tol, theta_jump_tol;
+  float lm_alpha;        // > 0 enables the Levenberg-Marquardt diagonal
+  float momentum;        // != 0 enables momentum
+  int def_grads;         // deformed image has gradients: GN uses interpolated def gradients
+  int normalize_gamma;
+  int uncertainty;
+  int first_frame;
+};
+
+// Device image: pixels (float or uint16) + optional float gradients.
+struct DImage {
+  const void *pix;
+  const float *gx, *gy;
+  int w, h, ox, oy;
+};
+
+// Subset set.  square: S > 0, centre (cx, cy).  conformal: S == 0 and a CSR pixel list.
+struct DSubsets {
+  int n;
+  int S;                       // square side (0: conformal)
+  int fov;                     // subset_dim for the first-frame field-of-view test (0: off)
+  const int *cx, *cy;
+  const int *off;              // n + 1 (conformal)
+  const int *px, *py;          // pixel coordinates (conformal)
+  const uint8_t *act;          // per-pixel active flag (conformal, may be null)
+  const int4 *bbox;            // per-subset pixel-coordinate bbox (min_x, min_y, max_x, max_y)
+  const int *n_active_def;     // per-subset count of defined-active pixels (conformal)
+};
+
+struct Launch {
+  int mode;                    // 0 plain, 1 batched
+  int waves_per_group;         // batched
+  int groups;                  // batched: persistent grid size
+  int lds_floats_per_wave;     // batched: LDS window budget
+};
+
+// Enqueue the correlation of all subsets (primary pass, then the fallback pass
+// when the optimization method has one).  storage: 0 f32, 1 u16.  work: device
+// ints, at least n + 4 (work counter, fallback count and list).
+hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,
+                            const DImage &def, const DSubsets &subs, double *fields, int *work,
+                            hipStream_t stream);
+
+// Finite-difference gradients (+ optional 5x5 smoothing) of an image.
+hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,
+                            float *tmpx, float *tmpy, hipStream_t stream);
+
+// Zero a small int buffer with a kernel (graph-safe, no memset node).
+hipError_t launch_zero_ints(int *p, int count, hipStream_t stream);
+
+// Copy a double buffer with a kernel (for the per-frame history inside graphs).
+hipError_t launch_copy_doubles(double *dst, const double *src, size_t count, hipStream_t stream);
+
+}  // namespace dice_hip

@@ -1,0 +1,127 @@
+// Image-level kernels of the subset-correlation library and the dispatch of
+// the correlation passes (the solver kernels live in dice_k_*.hip).
+#include "dice_device.h"
+
+namespace dice_hip {
+
+hipError_t launch_gn_plain(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
+                           const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+hipError_t launch_gn_batched(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
+                             const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+hipError_t launch_simplex_plain(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
+                                const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+hipError_t launch_simplex_batched(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
+                                  const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+
+namespace {
+using dev::ldpix;
+
+// FD gradients (Image_::compute_gradients_finite_difference, DICe_Image.cpp:1064-1094):
+// 4th-order central differences, first-order one-sided within 2 px of the edge.
+template <typename T>
+__global__ void gradients_kernel(const void *pix, int w, int h, float *gx, float *gy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const int o = y * w + x;
+  const float c1 = 1.0f / 12.0f, c2 = -8.0f / 12.0f;
+  float vx, vy;
+  if (x < 2) vx = ldpix<T>(pix, o + 1) - ldpix<T>(pix, o);
+  else if (x >= w - 2) vx = ldpix<T>(pix, o) - ldpix<T>(pix, o - 1);
+  else vx = c1 * ldpix<T>(pix, o - 2) + c2 * ldpix<T>(pix, o - 1) - c2 * ldpix<T>(pix, o + 1) - c1 * ldpix<T>(pix, o + 2);
+  if (y < 2) vy = ldpix<T>(pix, o + w) - ldpix<T>(pix, o);
+  else if (y >= h - 2) vy = ldpix<T>(pix, o) - ldpix<T>(pix, o - w);
+  else vy = c1 * ldpix<T>(pix, o - 2 * w) + c2 * ldpix<T>(pix, o - w) - c2 * ldpix<T>(pix, o + w) - c1 * ldpix<T>(pix, o + 2 * w);
+  gx[o] = vx;
+  gy[o] = vy;
+}
+
+// 5x5 binomial smoothing of the gradients (DICe_Image.cpp:1032-1062), interior only.
+__global__ void smooth_kernel(const float *inx, const float *iny, int w, int h, float *gx, float *gy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const int o = y * w + x;
+  if (x < 2 || x >= w - 2 || y < 2 || y >= h - 2) { gx[o] = inx[o]; gy[o] = iny[o]; return; }
+  const float k1[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  float vx = 0.0f, vy = 0.0f;
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) {
+      const float c = k1[i] * k1[j];
+      vx += c * inx[o + (i - 2) * w + (j - 2)];
+      vy += c * iny[o + (i - 2) * w + (j - 2)];
+    }
+  gx[o] = vx;
+  gy[o] = vy;
+}
+
+__global__ void zero_ints_kernel(int *p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+__global__ void copy_doubles_kernel(double *dst, const double *src, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+}  // namespace
+
+static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const KParams &kp, const DImage &ref,
+                              const DImage &def, const DSubsets &subs, double *fields, int pass, int *fb_list,
+                              int *fb_count, int *counter, hipStream_t stream) {
+  if (L.mode == 1) {
+    hipError_t e = launch_zero_ints(counter, 1, stream);
+    if (e != hipSuccess) return e;
+  }
+  if (simplex)
+    return (L.mode == 0 ? launch_simplex_plain : launch_simplex_batched)(storage, kp.interp, 0, L, kp, ref, def, subs,
+                                                                          fields, pass, fb_list, fb_count, counter,
+                                                                          stream);
+  return (L.mode == 0 ? launch_gn_plain : launch_gn_batched)(storage, kp.interp, kp.def_grads, L, kp, ref, def, subs,
+                                                             fields, pass, fb_list, fb_count, counter, stream);
+}
+
+hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
+                            const DSubsets &subs, double *fields, int *work, hipStream_t stream) {
+  if (subs.n <= 0) return hipSuccess;
+  // work: [0] batched work counter, [1] fallback count, [4 .. 4+n) fallback list
+  int *counter = work;
+  int *fb_count = work + 1;
+  int *fb_list = work + 4;
+  hipError_t e = launch_zero_ints(work, 4, stream);
+  if (e != hipSuccess) return e;
+  const bool simplex_first = kp.opt == 0 || kp.opt == 3;
+  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, 0, fb_list, fb_count, counter, stream);
+  if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
+  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, 1, fb_list, fb_count, counter, stream);
+}
+
+hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,
+                            float *tmpx, float *tmpy, hipStream_t stream) {
+  dim3 block(64, 4), grid((w + 63) / 64, (h + 3) / 4);
+  float *ox = smooth ? tmpx : gx, *oy = smooth ? tmpy : gy;
+  if (storage == 0)
+    hipLaunchKernelGGL(gradients_kernel<float>, grid, block, 0, stream, pix, w, h, ox, oy);
+  else
+    hipLaunchKernelGGL(gradients_kernel<uint16_t>, grid, block, 0, stream, pix, w, h, ox, oy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !smooth) return e;
+  hipLaunchKernelGGL(smooth_kernel, grid, block, 0, stream, tmpx, tmpy, w, h, gx, gy);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int *p, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, p, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_doubles(double *dst, const double *src, size_t count, hipStream_t stream) {
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_doubles_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, dst, src, count);
+  return hipGetLastError();
+}
+
+}  // namespace dice_hip

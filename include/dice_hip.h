@@ -1,0 +1,187 @@
+/*
+ * dice_hip.h -- C ABI of the MI355X (gfx950) subset-correlation library.
+ *
+ * This is the drop-in boundary for DICe's local subset-correlation path.
+ * Each entry point names the reference interface it replaces.  Plain C types
+ * only: pointers, sizes, a POD parameter struct.  No exceptions cross it.
+ *
+ * Conventions
+ *  - Return codes: DICE_HIP_OK (0) or a negative DICE_HIP_E* code; the text of
+ *    the last error is available from dice_hip_last_error().
+ *  - Host arrays are copied; the caller keeps ownership.  *_device entry points
+ *    take device pointers that must stay valid until the work is synchronized.
+ *  - Status values written to the STATUS field are the reference's Status_Flag
+ *    numbers (DICe.h:609-677); enum values of dice_params match DICe.h.
+ *  - One context per host thread; a context is not thread-safe.  All work of a
+ *    context runs in order on its own HIP stream.
+ */
+#ifndef DICE_HIP_H
+#define DICE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DICE_HIP_OK 0
+#define DICE_HIP_EINVAL (-1)      /* bad argument / unsupported option */
+#define DICE_HIP_EHIP (-2)        /* HIP runtime error */
+#define DICE_HIP_ESTATE (-3)      /* call order: e.g. no image or subsets yet */
+
+/* Storage of image pixels (DICe.h:105-112 storage_t). */
+#define DICE_STORAGE_F32 0
+#define DICE_STORAGE_U16 1
+
+/* Execution modes of dice_hip_correlate*. */
+#define DICE_MODE_PLAIN 0    /* one subset per 256-thread workgroup, global-memory gathers */
+#define DICE_MODE_BATCHED 1  /* persistent work-stealing waves, LDS-staged deformed windows */
+
+/* Correlation parameters: the subset of DICe's correlation parameter list
+ * (DICe.h:144-397, defaults DICe_ParameterUtilities.cpp:297-420) that the
+ * local subset-correlation path reads.  Enum-valued members use DICe.h values. */
+typedef struct dice_params {
+  int32_t interpolation_method;          /* BILINEAR=0, KEYS_FOURTH=2 (BICUBIC=1 rejected) */
+  int32_t optimization_method;           /* SIMPLEX=0, GRADIENT_BASED=1, SIMPLEX_THEN_GRADIENT_BASED=3,
+                                            GRADIENT_BASED_THEN_SIMPLEX=4 (GRADIENT_THEN_SEARCH=2 rejected) */
+  int32_t correlation_routine;           /* GENERIC_ROUTINE=0, TRACKING_ROUTINE=1 */
+  int32_t initialization_method;         /* USE_FIELD_VALUES=0 only */
+  int32_t enable_translation;            /* must be 1 */
+  int32_t enable_rotation;
+  int32_t enable_normal_strain;
+  int32_t enable_shear_strain;
+  int32_t max_solver_iterations_fast;
+  int32_t max_solver_iterations_robust;
+  double fast_solver_tolerance;
+  double robust_solver_tolerance;
+  double robust_delta_disp;
+  double robust_delta_theta;
+  double skip_solve_gamma_threshold;
+  double initial_gamma_threshold;        /* -1 = off */
+  double final_gamma_threshold;          /* -1 = off */
+  double disp_jump_tol;
+  double theta_jump_tol;
+  double levenberg_marquardt_regularization_factor; /* <= 0 = off */
+  double momentum_factor;                /* 0 = off */
+  int32_t compute_def_gradients;         /* deformed image carries FD gradients: GN uses them */
+  int32_t normalize_gamma_with_active_pixels;
+  int32_t compute_uncertainty_fields;    /* FIELD_1..7 on converged Gauss-Newton subsets */
+  int32_t gradient_method;               /* FINITE_DIFFERENCE=0, CONVOLUTION_5_POINT=1 */
+} dice_params;
+
+/* Per-subset field store: SoA of doubles, fields[slot * n + i]
+ * (the Schema's MultiField per Field_Spec, DICe_Schema.cpp:1363-1440). */
+enum dice_field {
+  DICE_F_COORD_X = 0, DICE_F_COORD_Y, DICE_F_DISP_X, DICE_F_DISP_Y, DICE_F_ROT_Z,
+  DICE_F_NS_XX, DICE_F_NS_YY, DICE_F_SS_XY, DICE_F_SIGMA, DICE_F_GAMMA, DICE_F_BETA,
+  DICE_F_MATCH, DICE_F_NOISE, DICE_F_CONTRAST, DICE_F_ACTIVE_PIXELS, DICE_F_STATUS,
+  DICE_F_ITERATIONS, DICE_F_CONDITION, DICE_F_NEIGHBOR_ID,
+  DICE_F_FIELD_1, DICE_F_FIELD_2, DICE_F_FIELD_3, DICE_F_FIELD_4, DICE_F_FIELD_5,
+  DICE_F_FIELD_6, DICE_F_FIELD_7,
+  DICE_F_COUNT
+};
+
+typedef struct dice_hip_context dice_hip_context;
+
+/* Context lifetime.  Replaces the state the reference keeps in a static
+ * DICe::Schema (DICe_api.cpp:97-115): device buffers, stream, graphs. */
+int dice_hip_create(dice_hip_context **ctx, int device);
+int dice_hip_destroy(dice_hip_context *ctx);
+const char *dice_hip_last_error(const dice_hip_context *ctx);
+
+/* Default parameter sets: dice_default_params / tracking_default_params
+ * (DICe_ParameterUtilities.cpp:360-420 / :297-358). */
+void dice_hip_default_params(dice_params *p);
+void dice_hip_tracking_default_params(dice_params *p);
+
+/* Schema::set_params (DICe_Schema.cpp:508-883) for the options above. */
+int dice_hip_set_params(dice_hip_context *ctx, const dice_params *p);
+
+/* Schema::set_ref_image(w, h, ArrayRCP) (DICe_Schema.cpp:395-445): uploads the
+ * pixels and computes the reference gradients on the device
+ * (Image_::compute_gradients, DICe_Image.cpp:1013-1094). */
+int dice_hip_set_ref_image(dice_hip_context *ctx, int width, int height, int storage,
+                           const void *host_pixels);
+
+/* Schema::set_def_image(w, h, ArrayRCP) (DICe_Schema.cpp:327-348): uploads the
+ * deformed pixels; gradients only when compute_def_gradients is set. */
+int dice_hip_set_def_image(dice_hip_context *ctx, int width, int height, int storage,
+                           const void *host_pixels);
+
+/* Same, for pixels already in device memory (no copy; the pointer is used by
+ * later correlate calls and must stay valid).  Gradients, if requested, go to
+ * context-owned buffers. */
+int dice_hip_set_def_image_device(dice_hip_context *ctx, int width, int height, int storage,
+                                  const void *device_pixels);
+
+/* The subset set of the Schema (DICe_Schema.cpp:141-150 + Subset ctors,
+ * DICe_Subset.cpp:78-168).  Square subsets of side subset_size centred on
+ * (coord_x[i], coord_y[i]) when conf_offsets is NULL.  Otherwise conformal
+ * subsets: pixels of subset i are conf_x/conf_y[conf_offsets[i] ..
+ * conf_offsets[i+1]) in the reference's (y, x)-sorted order, with per-pixel
+ * active flags (excluded areas = 0; NULL = all active).  Centroids are the
+ * coordinates.  The coordinates are also written to the COORD fields. */
+int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, const double *coord_y,
+                         int subset_size, const int32_t *conf_offsets, const int32_t *conf_x,
+                         const int32_t *conf_y, const uint8_t *conf_active);
+
+/* Schema::execute_correlation (DICe_Schema.cpp:1622-1792, GENERIC_ROUTINE with
+ * generic_correlation_routine :2025-2368 per subset).  fields: host SoA,
+ * DICE_F_COUNT * n doubles; read (displacements, rotation, strains, SIGMA as
+ * the USE_FIELD_VALUES initial guess, DICe_Initializer.cpp:398-434) and
+ * written (record_step / record_failed_step, :1958-2009).  first_frame = 1
+ * enables the field-of-view test that the reference applies when
+ * frame_id == first_frame_id.  Synchronous. */
+int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int first_frame);
+
+/* Same on a device-resident field store (DICE_F_COUNT * n doubles); enqueued on
+ * the context stream, returns without waiting. */
+int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame);
+
+/* A multi-frame sequence: the CLI frame loop of execute_correlation calls
+ * (DICe_Main.cpp:324-374) for n_frames deformed frames already in device
+ * memory, each initialised from the previous frame's fields.  When
+ * use_graph != 0 the whole sequence is captured once into a hipGraph and
+ * replayed.  device_history (optional, may be NULL): n_frames * DICE_F_COUNT * n
+ * doubles receiving each frame's fields.  Asynchronous. */
+int dice_hip_correlate_sequence(dice_hip_context *ctx, int mode, int n_frames, const void *const *device_frames,
+                                double *device_fields, double *device_history, int use_graph);
+
+/* Waits for all work of the context. */
+int dice_hip_synchronize(dice_hip_context *ctx);
+
+/* The context's HIP stream (hipStream_t), for callers that time or order work. */
+void *dice_hip_stream(dice_hip_context *ctx);
+
+/* Average duration (ms) of the correlation launches (primary + fallback pass)
+ * enqueued eagerly since the last call, measured with HIP events recorded on
+ * the context stream around each launch (launches replayed from a graph are
+ * not instrumented); resets the counters.  Returns the number measured. */
+int dice_hip_kernel_timing(dice_hip_context *ctx, double *avg_ms);
+int dice_hip_set_kernel_timing(dice_hip_context *ctx, int enable);
+
+/* Autotuner for DICE_MODE_BATCHED: times candidate launch shapes (waves per
+ * workgroup, workgroups per CU, LDS window budget) on the current images,
+ * subsets and fields (without modifying fields) and keeps the fastest for
+ * this problem signature.  No counterpart in the reference (CPU only). */
+int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int first_frame);
+int dice_hip_get_launch_shape(dice_hip_context *ctx, int *waves_per_group, int *groups_per_cu,
+                              int *lds_floats_per_wave);
+int dice_hip_set_launch_shape(dice_hip_context *ctx, int waves_per_group, int groups_per_cu,
+                              int lds_floats_per_wave);
+
+/* dice_correlate drop-in (DICe_api.h:129-133, DICe_api.cpp:51-153): stride-9
+ * points [x, y, u, v, theta, sigma, gamma, beta, status], float images of equal
+ * size, a static context created on the first call with the given parameters
+ * (NULL = tracking defaults) and kept between calls; the parameters are
+ * applied again only when update_params != 0.  Returns 0, or -1 on a size
+ * mismatch or bad argument, as the reference does. */
+int dice_hip_correlate_points(float *points, int n_points, int subset_size, const float *ref_img, int ref_w,
+                              int ref_h, const float *def_img, int def_w, int def_h, const dice_params *params,
+                              int update_params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
