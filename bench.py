@@ -49,7 +49,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise copy kernel (y = x * 1) over MIB MiB, 5 launches: the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters of a profile")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.arange(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        assert torch.equal(x, y)
+        print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 5}))
+        return 0
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the correlation of one device's subset share is the "
               "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)

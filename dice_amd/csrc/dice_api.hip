@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +28,9 @@ namespace {
 
 // dynamic LDS per workgroup of the batched kernel (default launch limit)
 constexpr size_t kMaxLdsBytes = 64 * 1024;
+// LDS slot per wave: deformed window, reused for the Gauss-Newton reduction
+// (35 x 64 floats + totals); 10 KB lets 4 four-wave workgroups share a CU
+constexpr int kDefaultSlotFloats = 2560;
 
 struct DevBuf {
   void *p = nullptr;
@@ -82,7 +86,7 @@ struct dice_hip_context {
   // fields + work
   DevBuf fields, work;
   // launch shape of the batched mode
-  Launch batched{1, 4, 0, 4096};
+  Launch batched{1, 4, 0, kDefaultSlotFloats};
   std::map<std::tuple<int, int, int, int, int>, Launch> tuned;
   // timing of correlation launches
   bool timing = false;
@@ -205,7 +209,7 @@ Launch current_launch(dice_hip_context *ctx, int mode) {
   if (L.groups <= 0) {
     // persistent grid: enough workgroups to fill every CU at the occupancy the
     // kernel allows, capped by the work available
-    int per_cu = 2;
+    int per_cu = 4;
     L.groups = ctx->num_cus * per_cu;
   }
   const int max_groups = (ctx->n + L.waves_per_group - 1) / L.waves_per_group;
@@ -484,7 +488,7 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     ctx->S = 0;
   }
   ctx->n = n;
-  HIP_TRY(ctx, ctx->work.ensure((size_t)(n + 4) * sizeof(int)), "hipMalloc work");
+  HIP_TRY(ctx, ctx->work.ensure(dice_hip::work_ints(n) * sizeof(int)), "hipMalloc work");
   HIP_TRY(ctx, ctx->fields.ensure((size_t)DICE_F_COUNT * n * sizeof(double)), "hipMalloc fields");
   return DICE_HIP_OK;
 }
@@ -609,6 +613,7 @@ int dice_hip_get_launch_shape(dice_hip_context *ctx, int *waves_per_group, int *
 int dice_hip_set_launch_shape(dice_hip_context *ctx, int waves_per_group, int groups_per_cu, int lds_floats) {
   if (!ctx) return DICE_HIP_EINVAL;
   if (waves_per_group < 1 || waves_per_group > 4 || groups_per_cu < 0 || groups_per_cu > 64 || lds_floats < 0 ||
+      (lds_floats & 3) != 0 || lds_floats < dice_hip::kMinSlotFloats ||
       (size_t)lds_floats * waves_per_group * 4 > kMaxLdsBytes)
     return ctx->fail(DICE_HIP_EINVAL, "launch shape out of range");
   ctx->batched = Launch{1, waves_per_group, groups_per_cu * ctx->num_cus, lds_floats};
@@ -645,10 +650,10 @@ int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int fi
   const int side = ctx->S > 0 ? ctx->S : 48;
   std::vector<int> windows;
   for (int extra : {8, 16, 28}) {
-    const int w = (side + extra) * (side + extra);
+    const int w = std::max(kDefaultSlotFloats, ((side + extra) * (side + extra) + 3) & ~3);
     if ((size_t)w * 4 <= kMaxLdsBytes) windows.push_back(w);
   }
-  if (windows.empty()) windows.push_back(0);
+  if (windows.empty()) windows.push_back(kDefaultSlotFloats);
   for (int waves : {1, 2, 4})
     for (int per_cu : {1, 2, 4, 8})
       for (int win : windows) {

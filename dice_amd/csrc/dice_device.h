@@ -12,44 +12,87 @@
 //     uncertainty FIELD_1..7 (DICe_Objective.cpp:193-299).
 //
 // Two group shapes share the code:
-//   PLAIN   : a 256-thread workgroup per subset, reductions through LDS and
-//             barriers, deformed intensities gathered straight from HBM/L2.
-//   BATCHED : one 64-lane wavefront per subset; waves of a persistent grid pull
-//             subset indices from a global work counter (work stealing), stage
-//             the warped subset's deformed window in an LDS slot once per
-//             iteration with coalesced loads, gather the 6x6 Keys taps from LDS
-//             and reduce with cross-lane shuffles (no barriers).
-//
-// Precision: pixel work in FP32; per-pixel products accumulated in FP64
-// (Hessian, gradient, ZNSSD sums); the 6x6 solve in FP64.
+//   PLAIN   : a 256-thread workgroup per subset.  Deformed intensities are
+//             gathered straight from HBM/L2, every per-pixel product is added
+//             in FP64 (as the reference does) and reductions go through LDS
+//             with barriers.
+//   BATCHED : one 64-lane wavefront per subset.  Waves of a persistent grid
+//             pull subset indices from a global work counter (work stealing).
+//             Before every warp evaluation the wave stages the deformed window
+//             that covers the warped subset (+ interpolation support) into its
+//             LDS slot with coalesced loads and gathers all interpolation taps
+//             from LDS.  Per-pixel products go to FP32 per-lane partial sums
+//             (at most ceil(N/64) pixels each) that are added across lanes in
+//             FP64 through an LDS transpose; the 6x6 system is solved in FP64
+//             with one row per lane.  No barriers: a wave synchronises itself.
+//             A subset whose window does not fit the slot is handed over to the
+//             PLAIN kernel (overflow list) and restarted there.
 #pragma once
 #include "dice_kernels.h"
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace dice_hip {
 namespace dev {
 
-// Pointers into LDS carry their address space so that no flat-address checks
-// are generated for them.
+// Pointers carry their address space so that loads compile to ds_read (LDS)
+// and global_load (HBM) with immediate offsets instead of flat accesses.
 typedef __attribute__((address_space(3))) float lds_float;
 typedef __attribute__((address_space(3))) double lds_double;
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const uint16_t gu16;
+typedef __attribute__((address_space(1))) const int gint;
+typedef __attribute__((address_space(1))) const uint8_t gu8;
 
 // ------------------------------------------------------------ utilities ---
+// Cross-lane movement without LDS addresses: DPP within rows of 16 lanes and
+// v_readlane to combine the four rows.  Every lane of the wave must be active.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within 8
+constexpr int kDppMirror = 0x140;     // lane i <-> 15 - i within 16
+template <int CTRL, typename V>
+__device__ __forceinline__ V dpp(V v) {
+  return __builtin_amdgcn_update_dpp(V(0), v, CTRL, 0xF, 0xF, false);
+}
+// v_readlane / v_readfirstlane move 32 bits: 64-bit values go as two halves
+// (the builtins take int, so a double argument would be converted).
+__device__ __forceinline__ int lane_value(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+// A value that is the same in every lane, made wave-uniform (scalar register).
+__device__ __forceinline__ double uniform(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+// Sum over the wave.  Each step adds a value and its partner's, so all lanes of
+// a row hold bit-identical row sums; the result is uniform.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp<kDppXor1>(v);
+  v += dpp<kDppXor2>(v);
+  v += dpp<kDppHalfMirror>(v);
+  v += dpp<kDppMirror>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
-  return v;
+  v = min(v, dpp<kDppXor1>(v));
+  v = min(v, dpp<kDppXor2>(v));
+  v = min(v, dpp<kDppHalfMirror>(v));
+  v = min(v, dpp<kDppMirror>(v));
+  return min(min(lane_value(v, 0), lane_value(v, 16)), min(lane_value(v, 32), lane_value(v, 48)));
 }
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-  return v;
+  v = max(v, dpp<kDppXor1>(v));
+  v = max(v, dpp<kDppXor2>(v));
+  v = max(v, dpp<kDppHalfMirror>(v));
+  v = max(v, dpp<kDppMirror>(v));
+  return max(max(lane_value(v, 0), lane_value(v, 16)), max(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // One wavefront.
@@ -108,13 +151,20 @@ struct BlockGroup {
   __device__ __forceinline__ int imax(int v) const { return ired(v, false); }
 };
 
-template <typename T> __device__ __forceinline__ float ldpix(const void *p, int i);
-template <> __device__ __forceinline__ float ldpix<float>(const void *p, int i) {
-  return static_cast<const float *>(p)[i];
-}
-template <> __device__ __forceinline__ float ldpix<uint16_t>(const void *p, int i) {
-  return (float)static_cast<const uint16_t *>(p)[i];
-}
+// Pixel storage of the images (storage_t, DICe.h:105-112): float or uint16.
+template <typename T> struct Pix;
+template <> struct Pix<float> {
+  typedef gfloat G;
+  __device__ __forceinline__ static float at(const void *p, int i) { return ((const gfloat *)p)[i]; }
+  __device__ __forceinline__ static const gfloat *ptr(const void *p) { return (const gfloat *)p; }
+};
+template <> struct Pix<uint16_t> {
+  typedef gu16 G;
+  __device__ __forceinline__ static float at(const void *p, int i) { return (float)((const gu16 *)p)[i]; }
+  __device__ __forceinline__ static const gu16 *ptr(const void *p) { return (const gu16 *)p; }
+};
+// Kept for the image-level kernels.
+template <typename T> __device__ __forceinline__ float ldpix(const void *p, int i) { return Pix<T>::at(p, i); }
 
 // Keys 4th-order cubic-convolution weights (Keys 1981; DICe_Image.cpp:114-122,
 // 860-871) of the six taps at offsets -2..+3 from floor(x), written as cubics
@@ -130,12 +180,34 @@ __device__ __forceinline__ void keys_weights(float t, float w[6]) {
   w[5] = (t2 - t3) * (1.0f / 12.0f);
 }
 
+// 6x6 Keys sum from rows of a 2-D array (LDS window or image) at `b`, pitch p.
+// Rows are reached by adding the pitch (no per-row multiply).
+template <typename P>
+__device__ __forceinline__ float keys_sum(const P *b, int p, const float wx[6], const float wy[6]) {
+  float v = 0.0f;
+  const P *row = b;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    float rs = (float)row[0] * wx[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) rs = fmaf((float)row[k], wx[k], rs);
+    v = m == 0 ? rs * wy[0] : fmaf(rs, wy[m], v);
+    row += p;
+  }
+  return v;
+}
+template <typename P>
+__device__ __forceinline__ float bilinear_sum(const P *b, int p, float tx, float ty) {
+  return (float)b[0] * (1.0f - tx) * (1.0f - ty) + (float)b[1] * tx * (1.0f - ty) + (float)b[p + 1] * tx * ty +
+         (float)b[p] * (1.0f - tx) * ty;
+}
+
 // Affine map and its Jacobian rows (Affine_Shape_Function::map / residuals,
 // DICe_LocalShapeFunction.cpp:220-248, 350-396), parameters in the fixed slot
 // order u, v, theta, ex, ey, gxy.
 struct Affine {
   float p[6];
-  float c, s;            // cos, sin theta
+  float c, s;  // cos, sin theta
   float a11, a12, a21, a22;
   __device__ __forceinline__ void prepare() {
     c = cosf(p[2]);
@@ -152,7 +224,7 @@ struct Affine {
     ox = p[0] + (a11 * dx + a12 * dy);
     oy = p[1] + (a21 * dx + a22 * dy);
   }
-  // residual row; (gx, gy) is the gradient to use (rotated ref or interpolated def)
+  // residual row; (Gx, Gy) is the gradient to use (rotated ref or interpolated def)
   __device__ __forceinline__ void residual(float dx, float dy, float Gx, float Gy, float r[6]) const {
     const float Dx = (1.0f + p[3]) * dx + p[5] * dy;
     const float Dy = (1.0f + p[4]) * dy + p[5] * dx;
@@ -252,30 +324,137 @@ __device__ __forceinline__ void lu_update6(double A[6][6], const double q[6], do
   for (int i = 0; i < 6; ++i) du[i] = b[i];
 }
 
+// Same update with the work spread over a wavefront: lane i (< 6) owns row i
+// of [H | -q].  Per column: first-maximal pivot among the remaining rows, row
+// exchange, scaled multipliers and the rank-1 update of the rows below -- the
+// arithmetic of the unblocked LU above element for element -- with the
+// right-hand side carried along (forward substitution), then back
+// substitution.  Pivot column, pivot rows and the solution are read with
+// v_readlane, so they are wave-uniform.  Exactly-zero pivot: du = -(factors) q.
+// hrow / qv hold row `lane` of H and q on lanes 0..5 (anything on other lanes).
+__device__ __forceinline__ void wave_lu_update6(double (&hrow)[6], double qv, double (&du)[6]) {
+  const int lane = (int)__lane_id();
+  const bool mine = lane < 6;
+  double b = -qv;
+  bool singular = false;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    // pivot: first row p >= j with the largest |A[p][j]|
+    double pv = lane_value(hrow[j], j);
+    double best = fabs(pv);
+    int p = j;
+#pragma unroll
+    for (int r = j + 1; r < 6; ++r) {
+      const double a = lane_value(hrow[j], r);
+      if (fabs(a) > best) { best = fabs(a); p = r; pv = a; }
+    }
+    if (pv != 0.0) {
+      if (p != j) {
+        // exchange rows j and p (right-hand side included)
+        double rj[6], rp[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { rj[k] = lane_value(hrow[k], j); rp[k] = lane_value(hrow[k], p); }
+        const double bj = lane_value(b, j), bp = lane_value(b, p);
+        if (lane == j) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) hrow[k] = rp[k];
+          b = bp;
+        } else if (lane == p) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) hrow[k] = rj[k];
+          b = bj;
+        }
+      }
+      const double r = 1.0 / pv;
+      if (mine && lane > j) hrow[j] *= r;
+    } else {
+      singular = true;
+    }
+    // rank-1 update of the rows below j with (the new) row j
+    if (j < 5) {
+      double rowj[6];
+#pragma unroll
+      for (int k = j + 1; k < 6; ++k) rowj[k] = lane_value(hrow[k], j);
+      const double bj = lane_value(b, j);
+      if (mine && lane > j) {
+        const double l = hrow[j];
+#pragma unroll
+        for (int k = j + 1; k < 6; ++k) hrow[k] = fma(-l, rowj[k], hrow[k]);
+        if (pv != 0.0) b = fma(-l, bj, b);
+      }
+    }
+  }
+  if (singular) {
+    // GETRI returned early: du = (stored factors) * (-q), q in its original order
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s = fma(hrow[k], -lane_value(qv, k), s);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) du[i] = lane_value(s, i);
+    return;
+  }
+  // back substitution U x = y
+#pragma unroll
+  for (int k = 5; k >= 0; --k) {
+    const double xk = lane_value(b, k) / lane_value(hrow[k], k);
+    du[k] = xk;
+    if (lane < k) b = fma(-hrow[k], xk, b);
+  }
+}
+
+// 2x2 condition estimate of the (u, v) block in scalar_t, as the reference
+// stores it (DICe_Objective.cpp:558-564); -1 when the determinant is 0.
+__device__ __forceinline__ float cond2x2(double h00, double h01, double h10, double h11) {
+  const float det_h = (float)(h00 * h11 - h10 * h01);
+  const float norm_H = (float)sqrt(h00 * h00 + h01 * h01 + h10 * h10 + h11 * h11);
+  float c = -1.0f;
+  if (det_h != 0.0f) {
+    const float dd = det_h * det_h;
+    const float norm_Hi = (float)sqrt((1.0 / (double)dd) * (h00 * h00 + h01 * h01 + h10 * h10 + h11 * h11));
+    c = norm_H * norm_Hi;
+  }
+  return c;
+}
+
+// Status of a wave that handed its subset over (never stored in a field).
+constexpr int kOverflow = -1000;
+
 // ------------------------------------------------------ subset solver ---
 template <typename T, typename G, int INTERP, bool DEFGRAD>
 struct Solver {
+  static constexpr bool kIsWave = G::kSize == kWave;
   const KParams kp;
   const DImage ref;
   const DImage def;
   G grp;
-  // LDS window (BATCHED only)
+  // LDS slot of this wave (BATCHED only): deformed window / reduction scratch
   lds_float *win;
   int win_cap;
-  bool use_win;
+  // BATCHED fast Gauss-Newton pass (reference gradients): per-subset constants
+  // sum(phi phi^T) at pers[0, 36) and sum(phi) at pers[36, 42) over the active
+  // pixels, phi = (gx, gy, dx gx, dy gx, dx gy, dy gy); last_fast: the last
+  // Gauss-Newton warp kept every active pixel
+  static constexpr bool kFast = kIsWave && !DEFGRAD;
+  lds_double *pers;
+  bool last_fast;
   int wx0, wy0, wpitch;
+  bool overflow;
   // subset
-  int cx, cy, n, S, hw, off;
+  int cx, cy, n, S, hw;
   int bx0, by0, bx1, by1;
-  int n_def_active;       // pixels flagged active in the subset definition
-  const int *px, *py;
-  const uint8_t *pact;
-  float invS;
+  bool ref_in;       // every subset pixel lies inside the reference image
+  int n_def_active;  // pixels flagged active in the subset definition
+  const gint *px;
+  const gint *py;
+  const gu8 *pact;
   float meanF;
-  float en[6];            // 1 for enabled parameters
+  float en[6];  // 1 for enabled parameters
 
   __device__ __forceinline__ Solver(const KParams &k, const DImage &r, const DImage &d, G g, lds_float *w, int cap)
-      : kp(k), ref(r), def(d), grp(g), win(w), win_cap(cap), use_win(false) {
+      : kp(k), ref(r), def(d), grp(g), win(w), win_cap(kFast ? cap - kPersistFloats : cap), wx0(0), wy0(0),
+        wpitch(0), overflow(false) {
+    pers = kFast ? (lds_double *)(w + win_cap) : nullptr;
+    last_fast = false;
     en[0] = en[1] = 1.0f;
     en[2] = k.en_rot ? 1.0f : 0.0f;
     en[3] = en[4] = k.en_normal ? 1.0f : 0.0f;
@@ -286,52 +465,94 @@ struct Solver {
     cx = ss.cx[i];
     cy = ss.cy[i];
     S = ss.S;
+    overflow = false;
     if (S > 0) {
       hw = S / 2;
-      S = 2 * hw + 1;     // square subsets are odd (DICe_Subset.cpp:90-92)
+      S = 2 * hw + 1;  // square subsets are odd (DICe_Subset.cpp:90-92)
       n = S * S;
-      invS = 1.0f / (float)S;
-      bx0 = max(cx - hw, 0); bx1 = max(cx + hw, 0);
-      by0 = max(cy - hw, 0); by1 = max(cy + hw, 0);
+      bx0 = max(cx - hw, 0);
+      bx1 = max(cx + hw, 0);
+      by0 = max(cy - hw, 0);
+      by1 = max(cy + hw, 0);
       n_def_active = n;
       px = py = nullptr;
       pact = nullptr;
-      off = 0;
     } else {
-      off = ss.off[i];
+      const int off = ss.off[i];
       n = ss.off[i + 1] - off;
-      px = ss.px + off;
-      py = ss.py + off;
-      pact = ss.act ? ss.act + off : nullptr;
+      px = (const gint *)ss.px + off;
+      py = (const gint *)ss.py + off;
+      pact = ss.act ? (const gu8 *)ss.act + off : nullptr;
       const int4 b = ss.bbox[i];
-      bx0 = b.x; by0 = b.y; bx1 = b.z; by1 = b.w;
+      bx0 = b.x;
+      by0 = b.y;
+      bx1 = b.z;
+      by1 = b.w;
       n_def_active = ss.n_active_def[i];
+    }
+    ref_in = bx0 - ref.ox >= 0 && by0 - ref.oy >= 0 && bx1 - ref.ox < ref.w && by1 - ref.oy < ref.h;
+  }
+
+  // Visit the group's share of the subset pixels: f(x, y, active).  Negative
+  // coordinates clamp to 0 (DICe_Subset.cpp:65-68, 100-101).  Square subsets
+  // walk (row, col) incrementally in a loop of their own.
+  template <class F>
+  __device__ __forceinline__ void for_pixels(F &&f) const {
+    const int r0 = grp.rank();
+    if (S > 0) {
+      const int drow = G::kSize / S, dcol = G::kSize - drow * S;
+      int row = r0 / S, col = r0 - row * S;
+      const int x0 = cx - hw, y0 = cy - hw;
+#pragma unroll 1
+      for (int i = r0; i < n; i += G::kSize) {
+        f(max(x0 + col, 0), max(y0 + row, 0), true);
+        col += dcol;
+        row += drow;
+        if (col >= S) { col -= S; ++row; }
+      }
+    } else {
+#pragma unroll 1
+      for (int i = r0; i < n; i += G::kSize) f(max(px[i], 0), max(py[i], 0), pact ? (pact[i] != 0) : true);
     }
   }
 
-  // pixel coordinates (negative coordinates clamp to 0, DICe_Subset.cpp:65-68,100-101)
-  __device__ __forceinline__ void pixel(int i, int &x, int &y, bool &a) const {
-    if (S > 0) {
-      const int row = (int)(((float)i + 0.5f) * invS);
-      const int col = i - row * S;
-      x = max(cx - hw + col, 0);
-      y = max(cy - hw + row, 0);
-      a = true;
+  // for_pixels with the body instantiated twice, once for subsets inside the
+  // reference image (tag true: reference reads need no bounds test) and once
+  // for the others, selected by a uniform branch: f(tag, x, y, active).
+  template <class F>
+  __device__ __forceinline__ void for_pixels_ref(F &&f) const {
+    if (ref_in)
+      for_pixels([&](int x, int y, bool a) { f(std::true_type{}, x, y, a); });
+    else
+      for_pixels([&](int x, int y, bool a) { f(std::false_type{}, x, y, a); });
+  }
+
+  // Reference intensity and gradients at local (rx, ry); outside the image the
+  // intensity is pixel 0 (Image_::operator(), DICe_Image.h:186-192) and the
+  // gradients 0.  Subsets inside the image skip the test (uniform branch).
+  template <bool INSIDE = false>
+  __device__ __forceinline__ void ref_pixel(int rx, int ry, float &F, float &gx, float &gy) const {
+    if (INSIDE || ref_in || (rx >= 0 && ry >= 0 && rx < ref.w && ry < ref.h)) {
+      const int o = ry * ref.w + rx;
+      F = Pix<T>::at(ref.pix, o);
+      gx = ((const gfloat *)ref.gx)[o];
+      gy = ((const gfloat *)ref.gy)[o];
     } else {
-      x = max(px[i], 0);
-      y = max(py[i], 0);
-      a = pact ? (pact[i] != 0) : true;
+      F = Pix<T>::at(ref.pix, 0);
+      gx = gy = 0.0f;
     }
+  }
+  template <bool INSIDE = false>
+  __device__ __forceinline__ float ref_value(int rx, int ry) const {
+    if (INSIDE || ref_in || (rx >= 0 && ry >= 0 && rx < ref.w && ry < ref.h))
+      return Pix<T>::at(ref.pix, ry * ref.w + rx);
+    return Pix<T>::at(ref.pix, 0);
   }
 
   // Image_::operator() (DICe_Image.h:186-192): out of bounds reads pixel 0
   __device__ __forceinline__ float img_at(const DImage &im, int x, int y) const {
-    if (x < 0 || y < 0 || x >= im.w || y >= im.h) return ldpix<T>(im.pix, 0);
-    return ldpix<T>(im.pix, y * im.w + x);
-  }
-  __device__ __forceinline__ float grad_at(const float *g, const DImage &im, int x, int y) const {
-    if (x < 0 || y < 0 || x >= im.w || y >= im.h) return 0.0f;
-    return g[y * im.w + x];
+    if (x < 0 || y < 0 || x >= im.w || y >= im.h) return Pix<T>::at(im.pix, 0);
+    return Pix<T>::at(im.pix, y * im.w + x);
   }
 
   // Is the warped pixel kept?  Rounding + 4-px guard of Subset::initialize
@@ -342,146 +563,389 @@ struct Solver {
     return !(rx < def.ox + 4 || rx >= def.ox + def.w - 4 || ry < def.oy + 4 || ry >= def.oy + def.h - 4);
   }
 
-  // Deformed intensity (and gradients) at x' = cx + ddx, y' = cy + ddy.
-  // Only called for kept pixels, so all taps lie inside the image.
-  __device__ __forceinline__ float sample(float ddx, float ddy, bool want_grad, float &gxo, float &gyo) const {
+  // Deformed intensity at x' = cx + ddx, y' = cy + ddy (interior interpolant of
+  // interpolate_keys_fourth_all / interpolate_bilinear_all, DICe_Image.cpp:392-425,
+  // 832-883).  Only called for kept pixels, so all taps lie inside the image
+  // and, in BATCHED mode, inside the staged window.
+  __device__ __forceinline__ float sample(float ddx, float ddy) const {
     const float flx = floorf(ddx), fly = floorf(ddy);
     const float tx = ddx - flx, ty = ddy - fly;
     const int ix = cx - def.ox + (int)flx;
     const int iy = cy - def.oy + (int)fly;
-    if (INTERP == 0) {
-      // interpolate_bilinear_all, DICe_Image.cpp:392-425
-      const float w00 = (1.0f - tx) * (1.0f - ty), w10 = tx * (1.0f - ty);
-      const float w11 = tx * ty, w01 = (1.0f - tx) * ty;
-      float v;
-      if (use_win) {
-        const lds_float *b = win + (iy - wy0) * wpitch + (ix - wx0);
-        v = b[0] * w00 + b[1] * w10 + b[wpitch + 1] * w11 + b[wpitch] * w01;
-      } else {
-        const int o = iy * def.w + ix;
-        v = ldpix<T>(def.pix, o) * w00 + ldpix<T>(def.pix, o + 1) * w10 +
-            ldpix<T>(def.pix, o + def.w + 1) * w11 + ldpix<T>(def.pix, o + def.w) * w01;
-      }
-      if (want_grad) {
-        const int o = iy * def.w + ix;
-        gxo = def.gx[o] * w00 + def.gx[o + 1] * w10 + def.gx[o + def.w + 1] * w11 + def.gx[o + def.w] * w01;
-        gyo = def.gy[o] * w00 + def.gy[o + 1] * w10 + def.gy[o + def.w + 1] * w11 + def.gy[o + def.w] * w01;
-      }
-      return v;
+    if constexpr (kIsWave) {
+      // window origin and pitch are scalar; kept pixels index the window with
+      // non-negative offsets below 2^24
+      if (INTERP == 0) return bilinear_sum(win + (__umul24(iy - wy0, wpitch) + (ix - wx0)), wpitch, tx, ty);
+      float wx[6], wy[6];
+      keys_weights(tx, wx);
+      keys_weights(ty, wy);
+      return keys_sum(win + (__umul24(iy - 2 - wy0, wpitch) + (ix - 2 - wx0)), wpitch, wx, wy);
+    } else {
+      const auto *P = Pix<T>::ptr(def.pix);
+      if (INTERP == 0) return bilinear_sum(P + iy * def.w + ix, def.w, tx, ty);
+      float wx[6], wy[6];
+      keys_weights(tx, wx);
+      keys_weights(ty, wy);
+      return keys_sum(P + (iy - 2) * def.w + ix - 2, def.w, wx, wy);
     }
-    // interpolate_keys_fourth_all, DICe_Image.cpp:832-883 (interior branch)
+  }
+  // Deformed-image gradients at the same position (DEFGRAD: the deformed image
+  // carries gradients and interpolate_*_all returns them too).
+  __device__ __forceinline__ void sample_grad(float ddx, float ddy, float &gxo, float &gyo) const {
+    const float flx = floorf(ddx), fly = floorf(ddy);
+    const float tx = ddx - flx, ty = ddy - fly;
+    const int ix = cx - def.ox + (int)flx;
+    const int iy = cy - def.oy + (int)fly;
+    const gfloat *GX = (const gfloat *)def.gx, *GY = (const gfloat *)def.gy;
+    if (INTERP == 0) {
+      gxo = bilinear_sum(GX + iy * def.w + ix, def.w, tx, ty);
+      gyo = bilinear_sum(GY + iy * def.w + ix, def.w, tx, ty);
+      return;
+    }
     float wx[6], wy[6];
     keys_weights(tx, wx);
     keys_weights(ty, wy);
-    float v = 0.0f;
-    if (use_win) {
-      const lds_float *b = win + (iy - 2 - wy0) * wpitch + (ix - 2 - wx0);
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const lds_float *row = b + m * wpitch;
-        float rs = row[0] * wx[0];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) rs = fmaf(row[k], wx[k], rs);
-        v = fmaf(rs, wy[m], v);
-      }
-    } else {
-      const int o = (iy - 2) * def.w + ix - 2;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const int ro = o + m * def.w;
-        float rs = ldpix<T>(def.pix, ro) * wx[0];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) rs = fmaf(ldpix<T>(def.pix, ro + k), wx[k], rs);
-        v = fmaf(rs, wy[m], v);
-      }
-    }
-    if (want_grad) {
-      const int o = (iy - 2) * def.w + ix - 2;
-      float ax = 0.0f, ay = 0.0f;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const int ro = o + m * def.w;
-        float rx = 0.0f, ry = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          rx = fmaf(def.gx[ro + k], wx[k], rx);
-          ry = fmaf(def.gy[ro + k], wx[k], ry);
-        }
-        ax = fmaf(rx, wy[m], ax);
-        ay = fmaf(ry, wy[m], ay);
-      }
-      gxo = ax;
-      gyo = ay;
-    }
-    return v;
+    gxo = keys_sum(GX + (iy - 2) * def.w + ix - 2, def.w, wx, wy);
+    gyo = keys_sum(GY + (iy - 2) * def.w + ix - 2, def.w, wx, wy);
   }
 
-  // Stage the deformed window covering the warped bbox (+ interpolation
-  // support) into this group's LDS slot.  Falls back to global gathers when
-  // the window does not fit.
-  __device__ __forceinline__ void stage_window(const Affine &A) {
-    use_win = false;
-    if (win_cap <= 0) return;
-    float lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
+  // Stage the deformed-image window [x0, x1] x [y0, y1] (local coordinates,
+  // clamped to the image) into the wave's LDS slot.  Returns false (and marks
+  // the subset as overflowing) when it does not fit.
+  __device__ __forceinline__ bool stage(int x0, int x1, int y0, int y1) {
+    x0 = max(x0, 0);
+    y0 = max(y0, 0);
+    x1 = min(x1, def.w - 1);
+    y1 = min(y1, def.h - 1);
+    const int ww = x1 - x0 + 1, hh = y1 - y0 + 1;
+    if (ww <= 0 || hh <= 0) {
+      wx0 = x0; wy0 = y0; wpitch = 1;
+      return true;  // nothing is kept
+    }
+    if (ww * hh > win_cap) {
+      overflow = true;
+      return false;
+    }
+    wx0 = __builtin_amdgcn_readfirstlane(x0);  // uniform already: keep it scalar
+    wy0 = __builtin_amdgcn_readfirstlane(y0);
+    wpitch = __builtin_amdgcn_readfirstlane(ww);
+    grp.sync();
+    const auto *P = Pix<T>::ptr(def.pix) + y0 * def.w + x0;
+    const float inv = 1.0f / (float)ww;
+    const int tot = ww * hh;
+#pragma unroll 4
+    for (int k = grp.rank(); k < tot; k += G::kSize) {
+      const int r = (int)(((float)k + 0.5f) * inv);
+      const int c = k - r * ww;
+      win[k] = (float)P[r * def.w + c];
+    }
+    grp.sync();
+    return true;
+  }
+
+  // Extremes of the displacement (x' - cx, y' - cy) over the subset bbox at
+  // parameters A: the map is affine, so they are taken at the bbox corners.
+  __device__ __forceinline__ void warp_bounds(const Affine &A, float &lo_x, float &hi_x, float &lo_y,
+                                              float &hi_y) const {
+    lo_x = 1e30f, hi_x = -1e30f, lo_y = 1e30f, hi_y = -1e30f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float dx = (float)((k & 1) ? bx1 : bx0) - (float)cx;
       const float dy = (float)((k & 2) ? by1 : by0) - (float)cy;
       float ox, oy;
       A.disp(dx, dy, ox, oy);
-      lo_x = fminf(lo_x, ox); hi_x = fmaxf(hi_x, ox);
-      lo_y = fminf(lo_y, oy); hi_y = fmaxf(hi_y, oy);
+      lo_x = fminf(lo_x, ox);
+      hi_x = fmaxf(hi_x, ox);
+      lo_y = fminf(lo_y, oy);
+      hi_y = fmaxf(hi_y, oy);
     }
-    const int lead = INTERP == 0 ? 0 : 2, trail = INTERP == 0 ? 1 : 3;
-    int x0 = cx - def.ox + (int)floorf(lo_x) - lead - 1;
-    int x1 = cx - def.ox + (int)floorf(hi_x) + trail + 1;
-    int y0 = cy - def.oy + (int)floorf(lo_y) - lead - 1;
-    int y1 = cy - def.oy + (int)floorf(hi_y) + trail + 1;
-    x0 = max(x0, 0); y0 = max(y0, 0);
-    x1 = min(x1, def.w - 1); y1 = min(y1, def.h - 1);
-    const int ww = x1 - x0 + 1, hh = y1 - y0 + 1;
-    if (ww <= 0 || hh <= 0 || ww * hh > win_cap) return;
-    wx0 = x0; wy0 = y0; wpitch = ww;
+  }
+
+  // Window covering the warped subset bbox (+ interpolation support and a
+  // 1-px rounding margin) at parameters A.
+  __device__ __forceinline__ bool stage_affine(const Affine &A) {
+    if constexpr (!kIsWave) {
+      (void)A;
+      return true;
+    } else {
+      float lo_x, hi_x, lo_y, hi_y;
+      warp_bounds(A, lo_x, hi_x, lo_y, hi_y);
+      const int lead = INTERP == 0 ? 0 : 2, trail = INTERP == 0 ? 1 : 3;
+      return stage(cx - def.ox + (int)floorf(lo_x) - lead - 1, cx - def.ox + (int)floorf(hi_x) + trail + 1,
+                   cy - def.oy + (int)floorf(lo_y) - lead - 1, cy - def.oy + (int)floorf(hi_y) + trail + 1);
+    }
+  }
+
+  // Does every pixel pass the 4-px guard of kept() at parameters A?  The
+  // rounded positions are monotone in the displacement, so the bbox extremes
+  // decide; a 1e-3 px margin covers the rounding of the per-pixel map.
+  __device__ __forceinline__ bool all_kept(const Affine &A) const {
+    float lo_x, hi_x, lo_y, hi_y;
+    warp_bounds(A, lo_x, hi_x, lo_y, hi_y);
+    const int rx0 = cx + (int)floorf(lo_x + 0.5f - 1e-3f), rx1 = cx + (int)floorf(hi_x + 0.5f + 1e-3f);
+    const int ry0 = cy + (int)floorf(lo_y + 0.5f - 1e-3f), ry1 = cy + (int)floorf(hi_y + 0.5f + 1e-3f);
+    return rx0 >= def.ox + 4 && rx1 < def.ox + def.w - 4 && ry0 >= def.oy + 4 && ry1 < def.oy + def.h - 4;
+  }
+
+  // Sum the 35 per-lane FP32 Gauss-Newton partials over the wave into FP64
+  // totals in LDS.  Lane values go to a 35 x 64 table with row stride 65 (the
+  // padding keeps both the column writes and the row reads free of bank
+  // conflicts), lanes < 35 each add one row in FP64 and store the total in the
+  // layout the solve reads: q at [0, 6), H as a full 6 x 6 matrix at [8, 44),
+  // sum r at [44, 50), sum G at 50, count at 51 (kGnTot* offsets).  Uses the
+  // first kMinSlotFloats floats of the wave's LDS slot (no window is live).
+  __device__ __forceinline__ const lds_double *lds_transpose_sum(const float (&v)[kGnSums]) {
+    const int lane = grp.rank();
+    lds_float *red = win;
     grp.sync();
-    const int tot = ww * hh;
-    for (int k = grp.rank(); k < tot; k += G::kSize) {
-      const int r = k / ww;
-      const int c = k - r * ww;
-      win[k] = ldpix<T>(def.pix, (y0 + r) * def.w + x0 + c);
+#pragma unroll
+    for (int k = 0; k < kGnSums; ++k) red[k * kGnRedStride + lane] = v[k];
+    grp.sync();
+    lds_double *tot = (lds_double *)(red + kGnTotOffset);
+    if (lane < kGnSums) {
+      const lds_float *row = red + lane * kGnRedStride;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+      for (int j = 0; j < kWave; j += 4) {
+        s0 += (double)row[j];
+        s1 += (double)row[j + 1];
+        s2 += (double)row[j + 2];
+        s3 += (double)row[j + 3];
+      }
+      const double s = (s0 + s1) + (s2 + s3);
+      // destination(s) of partial `lane`
+      int d0, d1 = -1;
+      if (lane < 6) {
+        d0 = kGnTotQ + lane;
+      } else if (lane < 27) {
+        // packed upper triangle: row k holds 6 - k entries
+        int h = lane - 6, k = 0, len = 6;
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+          if (h >= len) { h -= len; --len; ++k; }
+        const int l = k + h;
+        d0 = kGnTotH + 6 * k + l;
+        if (l != k) d1 = kGnTotH + 6 * l + k;
+      } else {
+        d0 = kGnTotR + (lane - 27);  // sum r [0, 6), then sum G, count
+      }
+      tot[d0] = s;
+      if (d1 >= 0) tot[d1] = s;
     }
     grp.sync();
-    use_win = true;
+    return tot;
+  }
+
+  // Reference intensity and the gradient to use at a kept pixel: rotated
+  // reference gradient (use_ref_grads, DICe_LocalShapeFunction.cpp:375-376) or
+  // the interpolated deformed-image gradient.
+  __device__ __forceinline__ void gn_ref(const Affine &A, int rx, int ry, float ox, float oy, float &Fv, float &Gx,
+                                         float &Gy) const {
+    if constexpr (DEFGRAD) {
+      Fv = ref_value(rx, ry);
+      sample_grad(ox, oy, Gx, Gy);
+    } else {
+      float gx, gy;
+      ref_pixel(rx, ry, Fv, gx, gy);
+      Gx = A.c * gx - A.s * gy;
+      Gy = A.s * gx + A.c * gy;
+    }
+  }
+
+  // One Gauss-Newton pass at parameters A: totals over the kept pixels,
+  // acc = [q with the shift (6), H upper triangle (21), sum r (6), sum G, count].
+  // PLAIN: every product added in FP64, workgroup reduction.
+  __device__ __forceinline__ void accumulate_gn_block(const Affine &A, float shift, double (&acc)[35]) {
+#pragma unroll
+    for (int k = 0; k < 35; ++k) acc[k] = 0.0;
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      float ox, oy;
+      A.disp(dx, dy, ox, oy);
+      if (!kept(ox, oy)) return;
+      const float Gv = sample(ox, oy);
+      float Fv, Gx, Gy;
+      gn_ref(A, x - ref.ox, y - ref.oy, ox, oy, Fv, Gx, Gy);
+      float r[6];
+      A.residual(dx, dy, Gx, Gy, r);
+      double rd[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) rd[k] = (double)(r[k] * en[k]);
+      const double gmf = (double)((Gv - shift) - (Fv - meanF));
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] = fma(gmf, rd[k], acc[k]);
+      int h = 6;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int l = k; l < 6; ++l) { acc[h] = fma(rd[k], rd[l], acc[h]); ++h; }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[27 + k] += rd[k];
+      acc[33] += (double)Gv;
+      acc[34] += 1.0;
+    });
+    grp.sum(acc);
+  }
+
+  // BATCHED: FP32 products and per-lane partial sums, FP64 totals in LDS.
+  __device__ __forceinline__ const lds_double *accumulate_gn_wave(const Affine &A, float shift) {
+    float pa[kGnSums - 1];
+#pragma unroll
+    for (int k = 0; k < kGnSums - 1; ++k) pa[k] = 0.0f;
+    int cntl = 0;
+    for_pixels([&](int x, int y, bool a) {
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      float ox, oy;
+      A.disp(dx, dy, ox, oy);
+      if (!a || !kept(ox, oy)) return;
+      const float Gv = sample(ox, oy);
+      float Fv, Gx, Gy;
+      gn_ref(A, x - ref.ox, y - ref.oy, ox, oy, Fv, Gx, Gy);
+      float r[6];
+      A.residual(dx, dy, Gx, Gy, r);
+#pragma unroll
+      for (int k = 2; k < 6; ++k) r[k] *= en[k];
+      const float gmf = (Gv - shift) - (Fv - meanF);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) pa[k] = fmaf(gmf, r[k], pa[k]);
+      int h = 6;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int l = k; l < 6; ++l) { pa[h] = fmaf(r[k], r[l], pa[h]); ++h; }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) pa[27 + k] += r[k];
+      pa[33] += Gv;
+      ++cntl;
+    });
+    float part[kGnSums];
+#pragma unroll
+    for (int k = 0; k < kGnSums - 1; ++k) part[k] = pa[k];
+    part[kGnSums - 1] = (float)cntl;
+    return lds_transpose_sum(part);
+  }
+
+  // Sum K per-lane FP32 values over the wave in FP64, eight table rows at a
+  // time: lane (lane / 8, lane % 8) adds eight entries of its row (start
+  // rotated by the segment against bank conflicts) and DPP within the eight
+  // lanes finishes the row, identically in all of them; dest(row, total) is
+  // called by one lane per row.  The table uses the first K * kGnRedStride
+  // floats of the slot.
+  template <int K, class D>
+  __device__ __forceinline__ void lds_chunk_sum(const float (&v)[K], D &&dest) {
+    const int lane = grp.rank();
+    lds_float *red = win;
+    grp.sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * kGnRedStride + lane] = v[k];
+    grp.sync();
+    const int seg = lane & 7;
+#pragma unroll
+    for (int c = 0; c < (K + 7) / 8; ++c) {
+      const int r = c * 8 + (lane >> 3);
+      const lds_float *src = red + min(r, K - 1) * kGnRedStride + seg * 8;
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t += (double)src[(j + seg) & 7];
+      t += dpp<kDppXor1>(t);
+      t += dpp<kDppXor2>(t);
+      t += dpp<kDppHalfMirror>(t);
+      if (seg == 0 && r < K) dest(r, t);
+    }
+    grp.sync();
+  }
+
+  // Per-subset constants of the fast pass over the active pixels: sum(phi
+  // phi^T) (a full symmetric 6 x 6) and sum(phi) into pers[].
+  __device__ __forceinline__ void precompute_phi() {
+    float pa[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) pa[k] = 0.0f;
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
+      float F, gx, gy;
+      ref_pixel(x - ref.ox, y - ref.oy, F, gx, gy);
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      const float ph[6] = {gx, gy, dx * gx, dy * gx, dx * gy, dy * gy};
+      int h = 0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int l = k; l < 6; ++l) { pa[h] = fmaf(ph[k], ph[l], pa[h]); ++h; }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) pa[21 + k] += ph[k];
+    });
+    lds_double *P = pers;
+    lds_chunk_sum<27>(pa, [&](int r, double t) {
+      if (r < 21) {
+        int h = r, k = 0, len = 6;  // packed upper triangle: row k holds 6 - k entries
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+          if (h >= len) { h -= len; --len; ++k; }
+        const int l = k + h;
+        P[6 * k + l] = t;
+        P[6 * l + k] = t;
+      } else {
+        P[36 + (r - 21)] = t;
+      }
+    });
+  }
+
+  // Fast pass: sum(gmf phi) (6) and sum(G) over the active pixels, all kept.
+  __device__ __forceinline__ const lds_double *accumulate_gn_fast(const Affine &A, float shift) {
+    float pa[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pa[k] = 0.0f;
+    for_pixels_ref([&](auto inside, int x, int y, bool a) {
+      if (!a) return;
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      float ox, oy;
+      A.disp(dx, dy, ox, oy);
+      const float Gv = sample(ox, oy);
+      float Fv, gx, gy;
+      ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
+      const float gmf = (Gv - shift) - (Fv - meanF);
+      const float g1 = gmf * gx, g2 = gmf * gy;
+      pa[0] += g1;
+      pa[1] += g2;
+      pa[2] = fmaf(g1, dx, pa[2]);
+      pa[3] = fmaf(g1, dy, pa[3]);
+      pa[4] = fmaf(g2, dx, pa[4]);
+      pa[5] = fmaf(g2, dy, pa[5]);
+      pa[6] += Gv;
+    });
+    lds_double *tot = (lds_double *)(win + kGnTotOffset);
+    lds_chunk_sum<7>(pa, [&](int r, double t) { tot[r] = t; });
+    return tot;
   }
 
   // Objective::gamma (DICe_Objective.cpp:58-76) at parameters A: warp, then
   // the ZNSSD of Subset::gamma from FP64 sums; also active count and contrast.
   __device__ __forceinline__ float gamma_eval(const Affine &A, int &active, float &contrast) {
-    stage_window(A);
+    active = 0;
+    contrast = 0.0f;
+    if (!stage_affine(A)) return -1.0f;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
-    #pragma unroll 1
-    for (int i = grp.rank(); i < n; i += G::kSize) {
-      int x, y;
-      bool a;
-      pixel(i, x, y, a);
-      if (!a) continue;
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
-      if (!kept(ox, oy)) continue;
-      float gdum, gdum2;
-      const double d = sample(ox, oy, false, gdum, gdum2);
-      const double r = img_at(ref, x - ref.ox, y - ref.oy);
+      if (!kept(ox, oy)) return;
+      const double d = sample(ox, oy);
+      const double r = ref_value(x - ref.ox, y - ref.oy);
       acc[0] += 1.0;
       acc[1] += d;
       acc[2] += d * d;
       acc[3] += r;
       acc[4] += r * r;
       acc[5] += d * r;
-    }
+    });
     grp.sum(acc);
     const double cnt = acc[0];
     active = (int)cnt;
-    if (cnt <= 0.0) { contrast = 0.0f; return -1.0f; }
+    if (cnt <= 0.0) return -1.0f;
     const double md = acc[1] / cnt, mr = acc[3] / cnt;
     const double sdd = fmax(acc[2] - cnt * md * md, 0.0);
     const double srr = fmax(acc[4] - cnt * mr * mr, 0.0);
@@ -496,106 +960,143 @@ struct Solver {
     return g;
   }
 
-  // Objective_ZNSSD::computeUpdateFast.  Returns the solver status; p_state
-  // receives the parameters of the last warp (for sigma).
-  __device__ __forceinline__ int gauss_newton(Affine &A, int &iters, float p_state[6], double *fields, int nsub, int gid) {
+  // Objective_ZNSSD::computeUpdateFast.  Returns the solver status (or
+  // kOverflow); p_state receives the parameters of the last warp (for sigma).
+  __device__ __forceinline__ int gauss_newton(Affine &A, int &iters, float p_state[6], double *fields, int nsub,
+                                              int gid) {
     const int max_its = kp.max_fast;
     float upd_old[6] = {0, 0, 0, 0, 0, 0};
     float shift = meanF;  // running estimate of meanG (one-pass ZSSD gradient)
     int it = 0;
+#pragma unroll 1
     for (; it <= max_its; ++it) {
       iters = it;
       A.prepare();
 #pragma unroll
       for (int k = 0; k < 6; ++k) p_state[k] = A.p[k];
-      stage_window(A);
-      // acc: q[6], H[21] (upper triangle), sr[6], sG, cnt
-      double acc[35];
+      if (!stage_affine(A)) return kOverflow;
+      float du[6];
+      float meanG;
+      if constexpr (!kIsWave) {
+        double acc[35];
+        accumulate_gn_block(A, shift, acc);
+        const double cnt = acc[34];
+        meanG = cnt > 0.0 ? (float)(acc[33] / cnt) : 0.0f;
+        const double corr = (double)meanG - (double)shift;
+        double q[6], H[6][6];
 #pragma unroll
-      for (int k = 0; k < 35; ++k) acc[k] = 0.0;
-      #pragma unroll 1
-      for (int i = grp.rank(); i < n; i += G::kSize) {
-        int x, y;
-        bool a;
-        pixel(i, x, y, a);
-        if (!a) continue;
-        const float dx = (float)(x - cx), dy = (float)(y - cy);
-        float ox, oy;
-        A.disp(dx, dy, ox, oy);
-        if (!kept(ox, oy)) continue;
-        float Gx, Gy;
-        const float Gv = sample(ox, oy, DEFGRAD, Gx, Gy);
-        const int rx = x - ref.ox, ry = y - ref.oy;
-        const float Fv = img_at(ref, rx, ry);
-        if (!DEFGRAD) {
-          const float gx = grad_at(ref.gx, ref, rx, ry), gy = grad_at(ref.gy, ref, rx, ry);
-          Gx = A.c * gx - A.s * gy;
-          Gy = A.s * gx + A.c * gy;
+        for (int k = 0; k < 6; ++k) q[k] = acc[k] - corr * acc[27 + k];
+        {
+          int h = 6;
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int l = k; l < 6; ++l) { H[k][l] = acc[h]; H[l][k] = acc[h]; ++h; }
         }
-        float r[6];
-        A.residual(dx, dy, Gx, Gy, r);
-        double rd[6];
+        // disabled parameters: identity rows, zero gradient (same solution as the
+        // reduced N x N system of the reference)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) rd[k] = (double)(r[k] * en[k]);
-        const double gmf = (double)((Gv - shift) - (Fv - meanF));
+        for (int k = 2; k < 6; ++k)
+          if (en[k] == 0.0f) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] = fma(gmf, rd[k], acc[k]);
-        int h = 6;
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-#pragma unroll
-          for (int l = k; l < 6; ++l) { acc[h] = fma(rd[k], rd[l], acc[h]); ++h; }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[27 + k] += rd[k];
-        acc[33] += (double)Gv;
-        acc[34] += 1.0;
-      }
-      grp.sum(acc);
-      const double cnt = acc[34];
-      const float meanG = cnt > 0.0 ? (float)(acc[33] / cnt) : 0.0f;
-      const double corr = (double)meanG - (double)shift;
-      double q[6], H[6][6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) q[k] = acc[k] - corr * acc[27 + k];
-      {
-        int h = 6;
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-#pragma unroll
-          for (int l = k; l < 6; ++l) { H[k][l] = acc[h]; H[l][k] = acc[h]; ++h; }
-      }
-      // disabled parameters: identity rows, zero gradient (same solution as the
-      // reduced N x N system of the reference)
-#pragma unroll
-      for (int k = 2; k < 6; ++k)
-        if (en[k] == 0.0f) {
-#pragma unroll
-          for (int l = 0; l < 6; ++l) { H[k][l] = 0.0; H[l][k] = 0.0; }
-          H[k][k] = 1.0;
-          q[k] = 0.0;
-        }
-      if (kp.lm_alpha > 0.0f) { H[0][0] += kp.lm_alpha; H[1][1] += kp.lm_alpha; }
-      // 2x2 condition estimate in scalar_t (float) as the reference stores it
-      const float det_h = (float)(H[0][0] * H[1][1] - H[1][0] * H[0][1]);
-      const float norm_H = (float)sqrt(H[0][0] * H[0][0] + H[0][1] * H[0][1] + H[1][0] * H[1][0] + H[1][1] * H[1][1]);
-      float cond = -1.0f;
-      if (det_h != 0.0f) {
-        const float dd = det_h * det_h;
-        const float norm_Hi = (float)sqrt((1.0 / (double)dd) *
-                                          (H[0][0] * H[0][0] + H[0][1] * H[0][1] + H[1][0] * H[1][0] + H[1][1] * H[1][1]));
-        cond = norm_H * norm_Hi;
-      }
-      if (grp.rank() == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
-      if (cond > 1.0e12f) return HESSIAN_SINGULAR;
-      float old[6], du[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) old[k] = A.p[k];
-      {
+            for (int l = 0; l < 6; ++l) { H[k][l] = 0.0; H[l][k] = 0.0; }
+            H[k][k] = 1.0;
+            q[k] = 0.0;
+          }
+        if (kp.lm_alpha > 0.0f) { H[0][0] += kp.lm_alpha; H[1][1] += kp.lm_alpha; }
+        const float cond = cond2x2(H[0][0], H[0][1], H[1][0], H[1][1]);
+        if (grp.rank() == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
+        if (cond > 1.0e12f) return HESSIAN_SINGULAR;
         double dd[6];
         lu_update6(H, q, dd);
 #pragma unroll
         for (int k = 0; k < 6; ++k) du[k] = (float)dd[k];
+      } else {
+        // lane i < 6 owns row i of the system.  The enable flags are indexed
+        // through selects (a per-lane index would move the array to scratch).
+        const int lane = grp.rank();
+        const int i = lane < 6 ? lane : 0;
+        float en_i = en[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) en_i = (i == k) ? en[k] : en_i;
+        double hrow[6], qv, h00, h01, h11;
+        bool fast = false;
+        if constexpr (kFast) fast = all_kept(A);
+        last_fast = fast;
+        if (fast) {
+          // every active pixel is kept: with reference gradients the residual
+          // row is r = M phi, M(theta, ex, ey, gxy) below, so H = M S M^T with
+          // the per-subset S = sum(phi phi^T), and
+          // q = M (sum(gmf phi) - (meanG - shift) sum(phi))
+          const lds_double *tot = accumulate_gn_fast(A, shift);
+          const double cnt = (double)n_def_active;
+          meanG = cnt > 0.0 ? (float)(uniform(tot[6]) / cnt) : 0.0f;
+          const double corr = (double)meanG - (double)shift;
+          const double c = A.c, sn = A.s, ex1 = 1.0 + (double)A.p[3], ey1 = 1.0 + (double)A.p[4], g = A.p[5];
+          // row i of M (phi = gx, gy, dx gx, dy gx, dx gy, dy gy)
+          double mi[6];
+          mi[0] = i == 0 ? c : (i == 1 ? sn : 0.0);
+          mi[1] = i == 0 ? -sn : (i == 1 ? c : 0.0);
+          mi[2] = i == 2 ? -g : (i == 3 ? 1.0 : 0.0);
+          mi[3] = i == 2 ? -ey1 : (i == 5 ? 1.0 : 0.0);
+          mi[4] = i == 2 ? ex1 : (i == 5 ? 1.0 : 0.0);
+          mi[5] = i == 2 ? g : (i == 4 ? 1.0 : 0.0);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) mi[k] *= (double)en_i;
+          double vv[6];
+          qv = 0.0;
+#pragma unroll
+          for (int l = 0; l < 6; ++l) {
+            qv = fma(mi[l], tot[l] - corr * pers[36 + l], qv);
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t = fma(pers[6 * l + k], mi[k], t);
+            vv[l] = t;
+          }
+          hrow[0] = c * vv[0] - sn * vv[1];
+          hrow[1] = sn * vv[0] + c * vv[1];
+          hrow[2] = (double)en[2] * (ex1 * vv[4] - ey1 * vv[3] + g * (vv[5] - vv[2]));
+          hrow[3] = (double)en[3] * vv[2];
+          hrow[4] = (double)en[4] * vv[5];
+          hrow[5] = (double)en[5] * (vv[3] + vv[4]);
+          h00 = lane_value(hrow[0], 0);
+          h01 = lane_value(hrow[1], 0);
+          h11 = lane_value(hrow[1], 1);
+        } else {
+          // totals land in LDS
+          const lds_double *tot = accumulate_gn_wave(A, shift);
+          const double cnt = uniform(tot[kGnTotR + 7]);
+          meanG = cnt > 0.0 ? (float)(uniform(tot[kGnTotR + 6]) / cnt) : 0.0f;
+          const double corr = (double)meanG - (double)shift;
+          const lds_double *Hi = tot + kGnTotH + 6 * i;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) hrow[k] = Hi[k];
+          qv = tot[kGnTotQ + i] - corr * tot[kGnTotR + i];
+          h00 = uniform(tot[kGnTotH]);
+          h01 = uniform(tot[kGnTotH + 1]);
+          h11 = uniform(tot[kGnTotH + 7]);
+        }
+        // disabled parameters: identity rows and columns, zero gradient
+        const bool row_on = en_i != 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (!row_on || en[k] == 0.0f) hrow[k] = (k == i) ? 1.0 : 0.0;
+        if (!row_on) qv = 0.0;
+        const double lm = kp.lm_alpha > 0.0f ? (double)kp.lm_alpha : 0.0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (k == i) hrow[k] += lm;
+        const float cond = cond2x2(h00 + lm, h01, h01, h11 + lm);
+        if (lane == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
+        if (cond > 1.0e12f) return HESSIAN_SINGULAR;
+        double dd[6];
+        wave_lu_update6(hrow, qv, dd);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) du[k] = (float)dd[k];
       }
+      float old[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) old[k] = A.p[k];
       if (kp.momentum != 0.0f) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) { du[k] += kp.momentum * upd_old[k]; upd_old[k] = du[k]; }
@@ -606,7 +1107,7 @@ struct Solver {
       bool conv = fabsf(A.p[0] - old[0]) < kp.fast_tol && fabsf(A.p[1] - old[1]) < kp.fast_tol;
       if (kp.en_rot) conv = conv && fabsf(A.p[2] - old[2]) < kp.fast_tol;
       if (conv) {
-        if (kp.uncertainty) uncertainty(A, fields, nsub, gid);
+        if (kp.uncertainty && !uncertainty(A, fields, nsub, gid)) return kOverflow;
         break;
       }
       shift = meanG;
@@ -615,20 +1116,46 @@ struct Solver {
     return CORRELATION_SUCCESSFUL;
   }
 
+  // interpolate_keys_fourth (DICe_Image.cpp:890-924) at local coordinates:
+  // bilinear near the border (0 outside [0, w-1.5)), taps from base/pitch.
+  template <typename P>
+  __device__ __forceinline__ float keys_single(const P *base, int pitch, int ox0, int oy0, float lx, float ly) const {
+    const int w = def.w, h = def.h;
+    if (lx <= 2.5f || lx >= w - 3.5f || ly <= 2.5f || ly >= h - 3.5f) {
+      if (lx < 0.0f || lx >= w - 1.5f || ly < 0.0f || ly >= h - 1.5f) return 0.0f;
+      const int x1 = (int)lx, y1 = (int)ly;
+      return bilinear_sum(base + (y1 - oy0) * pitch + (x1 - ox0), pitch, lx - x1, ly - y1);
+    }
+    const int ix = (int)lx, iy = (int)ly;
+    float wx[6], wy[6];
+    keys_weights(lx - ix, wx);
+    keys_weights(ly - iy, wy);
+    return keys_sum(base + (iy - 2 - oy0) * pitch + (ix - 2 - ox0), pitch, wx, wy);
+  }
+
   // computeUncertaintyFields without an image deformer (DICe_Objective.cpp:193-299):
-  // exact displacement 0, so the "exact" residual reads the deformed pixel itself.
-  __device__ __forceinline__ void uncertainty(const Affine &A, double *fields, int nsub, int gid) {
+  // exact displacement 0, so the "exact" residual interpolates the deformed
+  // image at the pixel itself.  At an integer position the Keys weights are
+  // exactly (0, 0, 1, 0, 0, 0) and the bilinear ones (1, 0, 0, 0), so that
+  // value is the pixel, or 0 where interpolate_keys_fourth returns 0 (x or y
+  // outside [0, size - 1.5)); it is read directly.  Returns false if the
+  // BATCHED window for the shifted positions did not fit.
+  __device__ __forceinline__ bool uncertainty(const Affine &A, double *fields, int nsub, int gid) {
     const float u = A.p[0], v = A.p[1];
+    if constexpr (kIsWave) {
+      // window covering the subset bbox shifted by (u, v), Keys support and a
+      // 1-px margin for the rounding of x + u
+      const int fu = (int)floorf(u), fv = (int)floorf(v);
+      if (!stage(bx0 - ref.ox + fu - 3, bx1 - ref.ox + fu + 4, by0 - ref.oy + fv - 3, by1 - ref.oy + fv + 4))
+        return false;
+    }
     double acc[6] = {0, 0, 0, 0, 0, 0};  // sssig, n_uhat_g2, n_err_g2, n_err_jg2, r_tot2, n_uhat2
-    #pragma unroll 1
-    for (int i = grp.rank(); i < n; i += G::kSize) {
-      int x, y;
-      bool a;
-      pixel(i, x, y, a);
+    for_pixels([&](int x, int y, bool a) {
+      (void)a;
       const int rx = x - ref.ox, ry = y - ref.oy;
-      // the subset's gradient arrays: reference gradients (or, in def-gradient mode,
-      // the reference value for pixels this sketch does not re-interpolate)
-      const float gx = grad_at(ref.gx, ref, rx, ry), gy = grad_at(ref.gy, ref, rx, ry);
+      // the subset's gradient arrays: the reference gradients
+      float refv, gx, gy;
+      ref_pixel(rx, ry, refv, gx, gy);
       const float g2 = gx * gx + gy * gy;
       acc[0] += (double)g2;
       const float inv = g2 < 1.0e-4f ? 0.0f : 1.0f / g2;
@@ -637,14 +1164,19 @@ struct Solver {
       acc[1] += (double)(ug * ug * inv);
       acc[2] += (double)(ug * ug * inv);
       acc[3] += (double)(jg * jg * inv);
-      const float refv = img_at(ref, rx, ry);
-      const float sub_r = keys_single((float)rx + u, (float)ry + v) - refv;
-      const float sub_e = keys_single((float)rx, (float)ry) - refv;
+      float sub_r;
+      if constexpr (kIsWave) {
+        sub_r = keys_single(win, wpitch, wx0, wy0, (float)rx + u, (float)ry + v) - refv;
+      } else {
+        sub_r = keys_single(Pix<T>::ptr(def.pix), def.w, 0, 0, (float)rx + u, (float)ry + v) - refv;
+      }
+      const bool inb = rx >= 0 && ry >= 0 && rx <= def.w - 2 && ry <= def.h - 2;
+      const float sub_e = (inb ? Pix<T>::at(def.pix, ry * def.w + rx) : 0.0f) - refv;
       acc[4] += (double)((sub_r - sub_e) * (sub_r - sub_e) * inv);
       acc[5] += (double)(u * u + v * v);
-    }
+    });
     grp.sum(acc);
-    if (grp.rank() != 0) return;
+    if (grp.rank() != 0) return true;
     const double sssig = (1.0 + acc[0]) / (double)(n == 0 ? 1 : n);
     const double cos_hat = acc[5] == 0.0 ? 1.0 : sqrt(acc[1] / acc[5]);
     fields[(size_t)F_FIELD_1 * nsub + gid] = (float)cos_hat;
@@ -654,34 +1186,7 @@ struct Solver {
     fields[(size_t)F_FIELD_5 * nsub + gid] = (float)sqrt(acc[2]);
     fields[(size_t)F_FIELD_6 * nsub + gid] = (float)sqrt(acc[4]);
     fields[(size_t)F_FIELD_7 * nsub + gid] = (float)sssig;
-  }
-
-  // interpolate_keys_fourth (DICe_Image.cpp:890-924) at local coordinates:
-  // bilinear near the border (and 0 outside [0, w-1.5)).
-  __device__ __forceinline__ float keys_single(float lx, float ly) const {
-    const int w = def.w, h = def.h;
-    if (lx <= 2.5f || lx >= w - 3.5f || ly <= 2.5f || ly >= h - 3.5f) {
-      if (lx < 0.0f || lx >= w - 1.5f || ly < 0.0f || ly >= h - 1.5f) return 0.0f;
-      const int x1 = (int)lx, y1 = (int)ly;
-      const float tx = lx - x1, ty = ly - y1;
-      const int o = y1 * w + x1;
-      return ldpix<T>(def.pix, o) * (1.0f - tx) * (1.0f - ty) + ldpix<T>(def.pix, o + 1) * tx * (1.0f - ty) +
-             ldpix<T>(def.pix, o + w + 1) * tx * ty + ldpix<T>(def.pix, o + w) * (1.0f - tx) * ty;
-    }
-    const int ix = (int)lx, iy = (int)ly;
-    float wx[6], wy[6];
-    keys_weights(lx - ix, wx);
-    keys_weights(ly - iy, wy);
-    float v = 0.0f;
-    const int o = (iy - 2) * w + ix - 2;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      float rs = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) rs = fmaf(ldpix<T>(def.pix, o + m * w + k), wx[k], rs);
-      v = fmaf(rs, wy[m], v);
-    }
-    return v;
+    return true;
   }
 
   // Simplex::minimize via Objective::gamma (DICe_Simplex.cpp:60-277).  The
@@ -724,6 +1229,7 @@ struct Solver {
 #pragma unroll
         for (int j = 0; j < 6; ++j) A.p[j] = pts[i][j];
         gv[i] = eval_here(A);
+        if (overflow) return kOverflow;
         if (i == 0 && gv[0] < kp.skip_thr && gv[0] >= 0.0f) { iters = 0; return CORRELATION_SUCCESSFUL; }
 #pragma unroll
         for (int j = 0; j < 6; ++j) A.p[j] = init[j];
@@ -740,6 +1246,7 @@ struct Solver {
     float old_rtol = 0.0f, gamma_new = 0.0f, gamma_old = -1.0f;
     int repeats = 0;
     int iteration = 0;
+#pragma unroll 1
     for (iteration = 0; iteration < kp.max_robust; iteration++) {
       if (iteration >= kp.max_robust - 1) { iters = iteration; return MAX_ITERATIONS_REACHED; }
       int ilo = 0, ihi, inhi;
@@ -770,12 +1277,12 @@ struct Solver {
       // reflect / expand / contract (fac -1, 2, 0.5) with replacement of the high vertex
       float ytry = 0.0f;
       const float facs[3] = {-1.0f, 2.0f, 0.5f};
-      int stage = 0;  // 0 reflect, 1 expand, 2 contract
+      int stage_no = 0;  // 0 reflect, 1 expand, 2 contract
       float ysave = 0.0f;
       bool shrink = false;
 #pragma unroll
       for (int st = 0; st < 3; ++st) {
-        if (st == stage) {
+        if (st == stage_no) {
           const float fac = facs[st];
           const float fac1 = (float)((1.0 - fac) / nd);
           const float fac2 = fac1 - fac;
@@ -798,14 +1305,14 @@ struct Solver {
             g_hi = ytry;
           }
           if (st == 0) {
-            if (ytry <= g_lo) stage = 1;
-            else if (ytry >= g_nhi) stage = 2;
-            else stage = 3;
+            if (ytry <= g_lo) stage_no = 1;
+            else if (ytry >= g_nhi) stage_no = 2;
+            else stage_no = 3;
           } else if (st == 1) {
-            stage = 3;
+            stage_no = 3;
           } else {
             shrink = ytry >= ysave;
-            stage = 3;
+            stage_no = 3;
           }
         }
       }
@@ -839,6 +1346,7 @@ struct Solver {
       }
       // the extra objective evaluation at the current variables (DICe_Simplex.cpp:264)
       gamma_new = eval_here(A);
+      if (overflow) return kOverflow;
     }
     iters = iteration;
     return CORRELATION_SUCCESSFUL;
@@ -847,8 +1355,10 @@ struct Solver {
   // Immerkaer noise estimate (Subset::noise_std_dev, DICe_Subset.cpp:622-703)
   // over the bbox of the kept pixels at p_state, shifted by the final (u, v).
   __device__ __forceinline__ float noise_std_dev(int min_x, int max_x, int min_y, int max_y, float u, float v) {
-    min_x = (int)((float)min_x + u); max_x = (int)((float)max_x + u);
-    min_y = (int)((float)min_y + v); max_y = (int)((float)max_y + v);
+    min_x = (int)((float)min_x + u);
+    max_x = (int)((float)max_x + u);
+    min_y = (int)((float)min_y + v);
+    max_y = (int)((float)max_y + v);
     const int ih = def.h, iw = def.w, ox = def.ox, oy = def.oy;
     if (min_x < ox) min_x = ox;
     if (min_y < oy) min_y = oy;
@@ -861,6 +1371,8 @@ struct Solver {
     double acc[1] = {0.0};
     if (bw > 0 && bh > 0) {
       const int tot = bw * bh;
+      const auto *P = Pix<T>::ptr(def.pix);
+#pragma unroll 1
       for (int k = grp.rank(); k < tot; k += G::kSize) {
         const int r = k / bw;
         const int xx = min_x + (k - r * bw), yy = min_y + r;
@@ -869,11 +1381,12 @@ struct Solver {
           const float p = img_at(def, lx, ly);
           acc[0] += (double)sqrtf(p * p);
         } else {
-          const int o = ly * iw + lx;
-          const void *P = def.pix;
-          const float conv = ldpix<T>(P, o - iw - 1) - 2.0f * ldpix<T>(P, o - iw) + ldpix<T>(P, o - iw + 1) -
-                             2.0f * ldpix<T>(P, o - 1) + 4.0f * ldpix<T>(P, o) - 2.0f * ldpix<T>(P, o + 1) +
-                             ldpix<T>(P, o + iw - 1) - 2.0f * ldpix<T>(P, o + iw) + ldpix<T>(P, o + iw + 1);
+          const auto *c0 = P + (ly - 1) * iw + lx - 1;
+          const auto *c1 = c0 + iw;
+          const auto *c2 = c1 + iw;
+          const float conv = (float)c0[0] - 2.0f * (float)c0[1] + (float)c0[2] - 2.0f * (float)c1[0] +
+                             4.0f * (float)c1[1] - 2.0f * (float)c1[2] + (float)c2[0] - 2.0f * (float)c2[1] +
+                             (float)c2[2];
           acc[0] += (double)fabsf(conv);
         }
       }
@@ -886,42 +1399,53 @@ struct Solver {
   // Objective::sigma (DICe_Objective.cpp:150-184) with the subset state of the
   // last warp (parameters p_state) and the final parameters A.
   __device__ __forceinline__ float sigma(const float p_state[6], const Affine &A, float &noise) {
+    if constexpr (kFast) {
+      if (last_fast && S > 0) {
+        // the last warp kept every pixel of the square subset: the gradient
+        // sums are sum(phi phi^T)[0][0] and [1][1], the kept bbox is the bbox
+        noise = noise_std_dev(bx0, bx1, by0, by1, A.p[0], A.p[1]);
+        if (noise < 0.0f) return -1.0f;
+        const float sgx = (float)uniform(pers[0]), sgy = (float)uniform(pers[7]);
+        const float sg = sgx > sgy ? sgy : sgx;
+        return sg > 0.0f ? (float)sqrt(2.0 * (double)noise * (double)noise / (double)sg) : -1.0f;
+      }
+    }
     Affine Ps;
 #pragma unroll
     for (int k = 0; k < 6; ++k) Ps.p[k] = p_state[k];
     Ps.prepare();
     int mnx = cx, mxx = cx, mny = cy, mxy = cy;
     double acc[3] = {0, 0, 0};  // sum gx^2, sum gy^2, kept count
-    #pragma unroll 1
-    for (int i = grp.rank(); i < n; i += G::kSize) {
-      int x, y;
-      bool a;
-      pixel(i, x, y, a);
-      if (!a) continue;
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       Ps.disp(dx, dy, ox, oy);
-      if (!kept(ox, oy)) continue;
-      mnx = min(mnx, x); mxx = max(mxx, x);
-      mny = min(mny, y); mxy = max(mxy, y);
+      if (!kept(ox, oy)) return;
+      mnx = min(mnx, x);
+      mxx = max(mxx, x);
+      mny = min(mny, y);
+      mxy = max(mxy, y);
       float gx, gy;
-      if (DEFGRAD) {
-        const bool keep_win = use_win;
-        use_win = false;
-        sample(ox, oy, true, gx, gy);
-        use_win = keep_win;
+      if constexpr (DEFGRAD) {
+        sample_grad(ox, oy, gx, gy);
       } else {
-        gx = grad_at(ref.gx, ref, x - ref.ox, y - ref.oy);
-        gy = grad_at(ref.gy, ref, x - ref.ox, y - ref.oy);
+        float fdum;
+        ref_pixel(x - ref.ox, y - ref.oy, fdum, gx, gy);
       }
       acc[0] += (double)(gx * gx);
       acc[1] += (double)(gy * gy);
       acc[2] += 1.0;
-    }
+    });
     grp.sum(acc);
-    mnx = grp.imin(mnx); mxx = grp.imax(mxx);
-    mny = grp.imin(mny); mxy = grp.imax(mxy);
-    if (acc[2] <= 0.0) { noise = -1.0f; return -1.0f; }
+    mnx = grp.imin(mnx);
+    mxx = grp.imax(mxx);
+    mny = grp.imin(mny);
+    mxy = grp.imax(mxy);
+    if (acc[2] <= 0.0) {
+      noise = -1.0f;
+      return -1.0f;
+    }
     noise = noise_std_dev(mnx, mxx, mny, mxy, A.p[0], A.p[1]);
     if (noise < 0.0f) return -1.0f;
     const float sgx = (float)acc[0], sgy = (float)acc[1];
@@ -932,9 +1456,15 @@ struct Solver {
   __device__ __forceinline__ void record_failed(double *fields, int nsub, int gid, int status, int iters) {
     if (grp.rank() != 0) return;
     auto f = [&](int s) -> double & { return fields[(size_t)s * nsub + gid]; };
-    f(F_SIGMA) = -1.0; f(F_MATCH) = -1.0; f(F_GAMMA) = -1.0; f(F_BETA) = -1.0;
-    f(F_NOISE) = -1.0; f(F_CONTRAST) = -1.0; f(F_ACTIVE_PIXELS) = -1.0;
-    f(F_STATUS) = status; f(F_ITERATIONS) = iters;
+    f(F_SIGMA) = -1.0;
+    f(F_MATCH) = -1.0;
+    f(F_GAMMA) = -1.0;
+    f(F_BETA) = -1.0;
+    f(F_NOISE) = -1.0;
+    f(F_CONTRAST) = -1.0;
+    f(F_ACTIVE_PIXELS) = -1.0;
+    f(F_STATUS) = status;
+    f(F_ITERATIONS) = iters;
   }
 
   __device__ __forceinline__ void init_params(Affine &A, const double *fields, int nsub, int gid) {
@@ -947,18 +1477,25 @@ struct Solver {
     A.p[5] = kp.en_shear ? (float)f(F_SS_XY) : 0.0f;
   }
 
+  __device__ __forceinline__ void push(int *list, int *count, int gid) {
+    if (grp.rank() == 0) list[atomicAdd(count, 1)] = gid;
+  }
+
   // generic_correlation_routine (DICe_Schema.cpp:2025-2368) for one subset,
   // split in two passes so that each kernel holds one optimizer:
   //   pass 0: FOV / initial-guess checks, the primary optimizer, and either
   //           the epilogue (sigma, gamma, record) or a fallback request;
   //   pass 1: the fallback optimizer of GRADIENT_BASED_THEN_SIMPLEX /
   //           SIMPLEX_THEN_GRADIENT_BASED for the subsets that asked for it.
-  // SIMPLEX_OPT selects which optimizer this instantiation runs.
+  // SIMPLEX_OPT selects which optimizer this instantiation runs.  A BATCHED
+  // wave whose window overflows its LDS slot pushes the subset to ov_list and
+  // leaves it to the PLAIN kernel (nothing it wrote is an input of the solve).
   template <bool SIMPLEX_OPT>
   __device__ __forceinline__ void run(const DSubsets &ss, int gid, double *fields, int pass, int *fb_list,
-                                      int *fb_count) {
+                                      int *fb_count, int *ov_list, int *ov_count) {
     const int nsub = ss.n;
     set_subset(ss, gid);
+    last_fast = false;
     auto fv = [&](int s) { return fields[(size_t)s * nsub + gid]; };
     if (pass == 0) {
       // field-of-view test on the first frame when a subset size is set (:2033-2049)
@@ -972,7 +1509,10 @@ struct Solver {
         }
       }
       // Field_Value_Initializer (DICe_Initializer.cpp:398-434)
-      if (fv(F_SIGMA) == -1.0) { record_failed(fields, nsub, gid, INITIALIZE_FAILED, -1); return; }
+      if (fv(F_SIGMA) == -1.0) {
+        record_failed(fields, nsub, gid, INITIALIZE_FAILED, -1);
+        return;
+      }
     }
     Affine A;
     init_params(A, fields, nsub, gid);
@@ -981,24 +1521,25 @@ struct Solver {
     if (!SIMPLEX_OPT) {
       // mean of the reference subset over defined-active pixels (DICe_Objective.cpp:464)
       double acc[2] = {0.0, 0.0};
-      #pragma unroll 1
-      for (int i = grp.rank(); i < n; i += G::kSize) {
-        int x, y;
-        bool a;
-        pixel(i, x, y, a);
-        if (!a) continue;
-        acc[0] += (double)img_at(ref, x - ref.ox, y - ref.oy);
+      for_pixels([&](int x, int y, bool a) {
+        if (!a) return;
+        acc[0] += (double)ref_value(x - ref.ox, y - ref.oy);
         acc[1] += 1.0;
-      }
+      });
       grp.sum(acc);
       meanF = acc[1] > 0.0 ? (float)(acc[0] / acc[1]) : 0.0f;
+      if constexpr (kFast) precompute_phi();
     }
     if (pass == 0 && kp.init_gamma_thr != -1.0) {
       int act;
       float con;
       A.prepare();
       const float g0 = gamma_eval(A, act, con);
-      if (g0 > kp.init_gamma_thr || g0 < 0.0f) { record_failed(fields, nsub, gid, INITIALIZE_FAILED, iters); return; }
+      if (overflow) { push(ov_list, ov_count, gid); return; }
+      if (g0 > kp.init_gamma_thr || g0 < 0.0f) {
+        record_failed(fields, nsub, gid, INITIALIZE_FAILED, iters);
+        return;
+      }
     }
     const float prev_u = (float)fv(F_DISP_X), prev_v = (float)fv(F_DISP_Y), prev_t = (float)fv(F_ROT_Z);
     float p_state[6];
@@ -1010,19 +1551,20 @@ struct Solver {
     } else {
       corr = gauss_newton(A, iters, p_state, fields, nsub, gid);
     }
+    if (corr == kOverflow || overflow) { push(ov_list, ov_count, gid); return; }
     const float du = A.p[0] - prev_u, dv = A.p[1] - prev_v, dt = (kp.en_rot ? A.p[2] : 0.0f) - prev_t;
     const bool jump = fabsf(du) > kp.disp_jump_tol || fabsf(dv) > kp.disp_jump_tol || fabsf(dt) > kp.theta_jump_tol;
     if (pass == 0) {
       if (corr != CORRELATION_SUCCESSFUL || jump) {
         if (kp.opt == 3 || kp.opt == 4) {
           // the fallback starts again from the initial guess (:2238, :2266)
-          if (grp.rank() == 0) {
-            const int slot = atomicAdd(fb_count, 1);
-            fb_list[slot] = gid;
-          }
+          push(fb_list, fb_count, gid);
           return;
         }
-        if (corr != CORRELATION_SUCCESSFUL) { record_failed(fields, nsub, gid, corr, iters); return; }
+        if (corr != CORRELATION_SUCCESSFUL) {
+          record_failed(fields, nsub, gid, corr, iters);
+          return;
+        }
         // a successful solve that jumped goes on and fails the later jump test
       }
     } else if (corr != CORRELATION_SUCCESSFUL) {
@@ -1031,16 +1573,23 @@ struct Solver {
     }
     float noise = 0.0f;
     const float sig = sigma(p_state, A, noise);
-    if (sig < 0.0f) { record_failed(fields, nsub, gid, FRAME_FAILED_DUE_TO_NEGATIVE_SIGMA, iters); return; }
+    if (sig < 0.0f) {
+      record_failed(fields, nsub, gid, FRAME_FAILED_DUE_TO_NEGATIVE_SIGMA, iters);
+      return;
+    }
     A.prepare();
     int active;
     float contrast;
     const float gam = gamma_eval(A, active, contrast);
+    if (overflow) { push(ov_list, ov_count, gid); return; }
     if ((kp.final_gamma_thr != -1.0 && gam > kp.final_gamma_thr) || gam < 0.0f) {
       record_failed(fields, nsub, gid, FRAME_FAILED_DUE_TO_HIGH_GAMMA, iters);
       return;
     }
-    if (jump) { record_failed(fields, nsub, gid, JUMP_TOLERANCE_EXCEEDED, iters); return; }
+    if (jump) {
+      record_failed(fields, nsub, gid, JUMP_TOLERANCE_EXCEEDED, iters);
+      return;
+    }
     if (grp.rank() == 0) {
       auto f = [&](int s) -> double & { return fields[(size_t)s * nsub + gid]; };
       f(F_DISP_X) = A.p[0];
@@ -1048,67 +1597,85 @@ struct Solver {
       if (kp.en_rot) f(F_ROT_Z) = A.p[2];
       if (kp.en_normal) { f(F_NS_XX) = A.p[3]; f(F_NS_YY) = A.p[4]; }
       if (kp.en_shear) f(F_SS_XY) = A.p[5];
-      f(F_SIGMA) = sig; f(F_MATCH) = 0.0; f(F_GAMMA) = gam; f(F_BETA) = 0.0;
-      f(F_NOISE) = noise; f(F_CONTRAST) = contrast; f(F_ACTIVE_PIXELS) = active;
-      f(F_STATUS) = init_status; f(F_ITERATIONS) = iters;
+      f(F_SIGMA) = sig;
+      f(F_MATCH) = 0.0;
+      f(F_GAMMA) = gam;
+      f(F_BETA) = 0.0;
+      f(F_NOISE) = noise;
+      f(F_CONTRAST) = contrast;
+      f(F_ACTIVE_PIXELS) = active;
+      f(F_STATUS) = init_status;
+      f(F_ITERATIONS) = iters;
     }
   }
 };
 
-// ---------------------------------------------------------------- kernels ---
+// Work items of a pass: all subsets (list == nullptr) or a device list.
+struct Items {
+  const int *list;
+  const int *count;
+  __device__ __forceinline__ int total(int n) const { return list ? *count : n; }
+  __device__ __forceinline__ int at(int k) const { return list ? list[k] : k; }
+};
 
-// Work items: pass 0 walks all subsets; pass 1 walks the fallback list.
-__device__ __forceinline__ int work_item(int k, int pass, const int *fb_list) { return pass == 0 ? k : fb_list[k]; }
-__device__ __forceinline__ int work_count(int n, int pass, const int *fb_count) { return pass == 0 ? n : *fb_count; }
-
+// PLAIN: workgroup k handles items k, k + gridDim.x, ...
 template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
 __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams kp, DImage ref, DImage def,
-                                                                        DSubsets ss, double *fields, int pass,
-                                                                        int *fb_list, int *fb_count) {
+                                                                        DSubsets ss, double *fields, PassArgs pa) {
   __shared__ double red[4 * kMaxRed];
-  const int k = blockIdx.x;
-  lds_double *red_l = (lds_double *)red;
-  if (k >= work_count(ss.n, pass, fb_count)) return;
-  const int gid = work_item(k, pass, fb_list);
-  BlockGroup g{red_l};
+  const Items items{pa.items, pa.item_count};
+  const int total = items.total(ss.n);
+  BlockGroup g{(lds_double *)red};
   Solver<T, BlockGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)red, 0);
-  s.template run<SIMPLEX_OPT>(ss, gid, fields, pass, fb_list, fb_count);
+#pragma unroll 1
+  for (int k = blockIdx.x; k < total; k += gridDim.x)
+    s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, nullptr, nullptr);
 }
 
+// BATCHED: persistent waves pull items from a global counter until it passes
+// the item count (every wave reaches that exit).  The register budget is set
+// for DICE_BATCHED_WAVES_PER_EU waves per SIMD (4: 128 VGPRs).
+#ifndef DICE_BATCHED_WAVES_PER_EU
+#define DICE_BATCHED_WAVES_PER_EU 4
+#endif
 template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
-__global__ __launch_bounds__(256) void correlate_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
-                                                                double *fields, int pass, int *fb_list,
-                                                                int *fb_count, int *counter, int lds_per_wave) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHED_WAVES_PER_EU))) void correlate_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
+                                                                double *fields, PassArgs pa, int lds_per_wave) {
   extern __shared__ float lds[];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
+  const Items items{pa.items, pa.item_count};
   WaveGroup g{lane};
   Solver<T, WaveGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                           lds_per_wave);
-  const int total = work_count(ss.n, pass, fb_count);
+  const int total = items.total(ss.n);
+#pragma unroll 1
   while (true) {
     int k = 0;
-    if (lane == 0) k = atomicAdd(counter, 1);
+    if (lane == 0) k = atomicAdd(pa.counter, 1);
     k = __shfl(k, 0, kWave);
     if (k >= total) break;
-    s.template run<SIMPLEX_OPT>(ss, work_item(k, pass, fb_list), fields, pass, fb_list, fb_count);
+    s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
+                                pa.ov_count);
   }
 }
 
-// Launch one pass of one kernel variant.
+// Launch one pass of one kernel variant.  MODE 0: PLAIN, MODE 1: BATCHED.
+// A PLAIN pass over a device list launches min(n, list_groups) workgroups.
+constexpr int kListGroups = 2048;
 template <typename T, int INTERP, bool DEFGRAD, bool SX, int MODE>
 hipError_t launch_variant(const Launch &L, const KParams &kp, const DImage &ref, const DImage &def,
-                          const DSubsets &subs, double *fields, int pass, int *fb_list, int *fb_count,
-                          int *counter, hipStream_t stream) {
+                          const DSubsets &subs, double *fields, const PassArgs &pa, hipStream_t stream) {
   if (MODE == 0) {
-    hipLaunchKernelGGL((correlate_plain_kernel<T, INTERP, DEFGRAD, SX>), dim3(subs.n), dim3(kPlainThreads), 0, stream,
-                       kp, ref, def, subs, fields, pass, fb_list, fb_count);
+    const int groups = pa.items ? (subs.n < kListGroups ? subs.n : kListGroups) : subs.n;
+    hipLaunchKernelGGL((correlate_plain_kernel<T, INTERP, DEFGRAD, SX>), dim3(groups), dim3(kPlainThreads), 0,
+                       stream, kp, ref, def, subs, fields, pa);
     return hipGetLastError();
   }
   const size_t shm = (size_t)L.waves_per_group * L.lds_floats_per_wave * sizeof(float);
   hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX>), dim3(L.groups),
-                     dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pass, fb_list,
-                     fb_count, counter, L.lds_floats_per_wave);
+                     dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
+                     L.lds_floats_per_wave);
   return hipGetLastError();
 }
 

@@ -6,8 +6,8 @@ namespace dice_hip {
 
 hipError_t launch_gn_plain(int storage, int interp, int defgrad, const Launch &L, const KParams &kp,
                               const DImage &ref, const DImage &def, const DSubsets &subs, double *fields,
-                              int pass, int *fb_list, int *fb_count, int *counter, hipStream_t stream) {
-#define V(T, I, D) dev::launch_variant<T, I, D, false, 0>(L, kp, ref, def, subs, fields, pass, fb_list, fb_count, counter, stream)
+                              const PassArgs &pa, hipStream_t stream) {
+#define V(T, I, D) dev::launch_variant<T, I, D, false, 0>(L, kp, ref, def, subs, fields, pa, stream)
   if (defgrad) {
     if (storage == 0) return interp == 0 ? V(float, 0, true) : V(float, 2, true);
     return interp == 0 ? V(uint16_t, 0, true) : V(uint16_t, 2, true);

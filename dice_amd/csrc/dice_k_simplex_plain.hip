@@ -6,8 +6,8 @@ namespace dice_hip {
 
 hipError_t launch_simplex_plain(int storage, int interp, int defgrad, const Launch &L, const KParams &kp,
                               const DImage &ref, const DImage &def, const DSubsets &subs, double *fields,
-                              int pass, int *fb_list, int *fb_count, int *counter, hipStream_t stream) {
-#define V(T, I, D) dev::launch_variant<T, I, D, true, 0>(L, kp, ref, def, subs, fields, pass, fb_list, fb_count, counter, stream)
+                              const PassArgs &pa, hipStream_t stream) {
+#define V(T, I, D) dev::launch_variant<T, I, D, true, 0>(L, kp, ref, def, subs, fields, pa, stream)
   (void)defgrad;
   if (storage == 0) return interp == 0 ? V(float, 0, false) : V(float, 2, false);
   return interp == 0 ? V(uint16_t, 0, false) : V(uint16_t, 2, false);

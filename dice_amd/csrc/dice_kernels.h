@@ -71,6 +71,18 @@ struct DSubsets {
   const int *n_active_def;     // per-subset count of defined-active pixels (conformal)
 };
 
+// BATCHED Gauss-Newton reduction: kGnSums per-lane partials (q 6, H upper
+// triangle 21, sum r 6, sum G, count) transposed through a table of row stride
+// kGnRedStride floats, totals (52 doubles) at float offset kGnTotOffset.
+constexpr int kGnSums = 35;
+constexpr int kGnRedStride = kWave + 1;
+constexpr int kGnTotOffset = kGnSums * kGnRedStride + 1;
+constexpr int kGnTotQ = 0, kGnTotH = 8, kGnTotR = 44;
+// Per-subset constants of the BATCHED fast Gauss-Newton pass at the end of a
+// wave's slot: the 6 x 6 matrix sum(phi phi^T) and sum(phi), 42 doubles.
+constexpr int kPersistFloats = 84;
+constexpr int kMinSlotFloats = kGnTotOffset + 2 * 52 + kPersistFloats;  // smallest LDS slot per wave
+
 struct Launch {
   int mode;                    // 0 plain, 1 batched
   int waves_per_group;         // batched
@@ -78,9 +90,25 @@ struct Launch {
   int lds_floats_per_wave;     // batched: LDS window budget
 };
 
+// Device work lists of one correlation pass.  items == nullptr: every subset;
+// otherwise the *item_count subsets listed in items.  Subsets that ask for the
+// fallback optimizer are appended to fb_list (pass 0 only); BATCHED waves
+// append subsets whose deformed window does not fit their LDS slot to ov_list.
+struct PassArgs {
+  int pass;
+  const int *items;
+  const int *item_count;
+  int *fb_list, *fb_count;
+  int *ov_list, *ov_count;
+  int *counter;                // BATCHED work-stealing counter (zeroed before the launch)
+};
+
+// Device ints of the work area of launch_correlate.
+inline size_t work_ints(int n) { return 4 + 3 * (size_t)(n > 0 ? n : 0); }
+
 // Enqueue the correlation of all subsets (primary pass, then the fallback pass
 // when the optimization method has one).  storage: 0 f32, 1 u16.  work: device
-// ints, at least n + 4 (work counter, fallback count and list).
+// ints, at least work_ints(n) (counters, fallback and overflow lists).
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,
                             const DImage &def, const DSubsets &subs, double *fields, int *work,
                             hipStream_t stream);

@@ -4,14 +4,16 @@
 
 namespace dice_hip {
 
+typedef hipError_t (*PassLauncher)(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
+                                   const DSubsets &, double *, const PassArgs &, hipStream_t);
 hipError_t launch_gn_plain(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
-                           const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+                           const DSubsets &, double *, const PassArgs &, hipStream_t);
 hipError_t launch_gn_batched(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
-                             const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+                             const DSubsets &, double *, const PassArgs &, hipStream_t);
 hipError_t launch_simplex_plain(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
-                                const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+                                const DSubsets &, double *, const PassArgs &, hipStream_t);
 hipError_t launch_simplex_batched(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
-                                  const DSubsets &, double *, int, int *, int *, int *, hipStream_t);
+                                  const DSubsets &, double *, const PassArgs &, hipStream_t);
 
 namespace {
 using dev::ldpix;
@@ -67,34 +69,46 @@ __global__ void copy_doubles_kernel(double *dst, const double *src, size_t count
 
 }  // namespace
 
+// One optimizer pass over `items` (nullptr: all subsets).  BATCHED mode runs the
+// persistent kernel, then the PLAIN kernel over the subsets it handed over.
 static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const KParams &kp, const DImage &ref,
-                              const DImage &def, const DSubsets &subs, double *fields, int pass, int *fb_list,
-                              int *fb_count, int *counter, hipStream_t stream) {
-  if (L.mode == 1) {
-    hipError_t e = launch_zero_ints(counter, 1, stream);
-    if (e != hipSuccess) return e;
+                              const DImage &def, const DSubsets &subs, double *fields, PassArgs pa, int *ov_list,
+                              int *ov_count, hipStream_t stream) {
+  const int defgrad = simplex ? 0 : kp.def_grads;
+  PassLauncher plain = simplex ? launch_simplex_plain : launch_gn_plain;
+  if (L.mode == 0) {
+    pa.ov_list = pa.ov_count = nullptr;
+    return plain(storage, kp.interp, defgrad, L, kp, ref, def, subs, fields, pa, stream);
   }
-  if (simplex)
-    return (L.mode == 0 ? launch_simplex_plain : launch_simplex_batched)(storage, kp.interp, 0, L, kp, ref, def, subs,
-                                                                          fields, pass, fb_list, fb_count, counter,
-                                                                          stream);
-  return (L.mode == 0 ? launch_gn_plain : launch_gn_batched)(storage, kp.interp, kp.def_grads, L, kp, ref, def, subs,
-                                                             fields, pass, fb_list, fb_count, counter, stream);
+  PassLauncher batched = simplex ? launch_simplex_batched : launch_gn_batched;
+  hipError_t e = launch_zero_ints(pa.counter, 1, stream);
+  if (e != hipSuccess) return e;
+  pa.ov_list = ov_list;
+  pa.ov_count = ov_count;
+  e = batched(storage, kp.interp, defgrad, L, kp, ref, def, subs, fields, pa, stream);
+  if (e != hipSuccess) return e;
+  PassArgs po = pa;
+  po.items = ov_list;
+  po.item_count = ov_count;
+  po.ov_list = po.ov_count = nullptr;
+  return plain(storage, kp.interp, defgrad, L, kp, ref, def, subs, fields, po, stream);
 }
 
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
                             const DSubsets &subs, double *fields, int *work, hipStream_t stream) {
   if (subs.n <= 0) return hipSuccess;
-  // work: [0] batched work counter, [1] fallback count, [4 .. 4+n) fallback list
-  int *counter = work;
-  int *fb_count = work + 1;
-  int *fb_list = work + 4;
+  // work: [0] work counter, [1] fallback count, [2] / [3] overflow counts of
+  // pass 0 / 1, then the fallback list and the two overflow lists (n each)
+  const size_t n = (size_t)subs.n;
+  int *fb_list = work + 4, *ov0 = work + 4 + n, *ov1 = work + 4 + 2 * n;
   hipError_t e = launch_zero_ints(work, 4, stream);
   if (e != hipSuccess) return e;
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
-  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, 0, fb_list, fb_count, counter, stream);
+  PassArgs pa{0, nullptr, nullptr, fb_list, work + 1, nullptr, nullptr, work};
+  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + 2, stream);
   if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
-  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, 1, fb_list, fb_count, counter, stream);
+  PassArgs pb{1, fb_list, work + 1, nullptr, nullptr, nullptr, nullptr, work};
+  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + 3, stream);
 }
 
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,

@@ -70,7 +70,10 @@ def run_oracle(p, ref, de, cx, cy, subset_size, fields=None, conformal=None, fir
 def compare(fg, fo, u_tol=U_TOL, gamma_rtol=GAMMA_RTOL, status_agree=STATUS_AGREE, label=""):
     sg, so = fg[dh.F["STATUS"]], fo[o.F["STATUS"]]
     agree = float(np.mean(sg == so))
-    assert agree >= status_agree, "%s status agreement %.4f" % (label, agree)
+    if agree < status_agree:
+        pairs, counts = np.unique(np.stack([sg[sg != so], so[sg != so]]), axis=1, return_counts=True)
+        detail = ", ".join("gpu %d / oracle %d: %d" % (a, b, c) for (a, b), c in zip(pairs.T, counts))
+        raise AssertionError("%s status agreement %.4f (%s)" % (label, agree, detail))
     both = np.isin(sg, dh.CONVERGED_STATUSES) & np.isin(so, dh.CONVERGED_STATUSES)
     for name in ("DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "SS_XY"):
         d = np.abs(fg[dh.F[name]] - fo[o.F[name]])[both]
@@ -385,7 +388,10 @@ def test_sequence_graph_matches_eager(ctx):
 
 
 def test_autotune_keeps_results(ctx):
-    """The autotuner only changes the launch shape: results are identical."""
+    """The autotuner only changes the launch shape.  Statuses and iterations are identical and values agree
+    to float rounding: a subset whose warped window outgrows a smaller LDS slot on some iteration is solved
+    by the workgroup kernel (FP64 per-pixel sums) instead of the wave (FP32 lane partials), which moves the
+    last bits of its fields."""
     torch = pytest.importorskip("torch")
     ref, de = synth.config1(seed=2)
     cx, cy = dh.regular_grid(512, 512, 31, 10)
@@ -403,5 +409,8 @@ def test_autotune_keeps_results(ctx):
     shape = ctx.launch_shape()
     assert shape[0] >= 1
     f2 = ctx.correlate(dh.new_fields(cx, cy), mode=dh.MODE_BATCHED)
-    assert np.array_equal(f1, f2)
+    for name in ("STATUS", "ITERATIONS"):
+        assert np.array_equal(f1[dh.F[name]], f2[dh.F[name]]), name
+    rel = np.abs(f1 - f2) / np.maximum(np.abs(f1), 1e-6)
+    assert rel.max() < 1e-5 and np.mean(rel > 0) < 0.01
     assert np.array_equal(d_f.cpu().numpy(), dh.new_fields(cx, cy))
