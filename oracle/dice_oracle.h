@@ -10,6 +10,28 @@
  * function cites the reference file:line it follows.  It is templated on the
  * scalar type so that the same source gives the reference's default build
  * (scalar_t = float, FP64 Hessian accumulation) and its DICE_USE_DOUBLE build.
+ *
+ * What pins each part (tests/test_oracle.py):
+ *   fd_gradients ........ analytic 255 cos(x/4pi) sin(y/4pi) within 1e-3 and the one-sided border
+ *                         differences (DICe_TestImage.cpp:166-232)
+ *   interp .............. baboon apply_transform golden, L2 < 0.1 in f32 and f64 (DICe_TestImage.cpp:236-266),
+ *                         a planted sign flip fails it; bilinear exact on bilinear polynomials, Keys exact on
+ *                         cubics; the border fall-through of keys_fourth_all (DICe_Image.cpp:851-857)
+ *   subset_gamma ........ stripe closed form gamma = 0.4 k (DICe_TestSubsetGamma.cpp:76-117)
+ *   correlate (GN) ...... parallel_subsets gold (49 subsets: u, v, gamma, iterations, status) and
+ *                         single_speckled_shift gold (u, v, sigma) from tests/regression; the 6-parameter
+ *                         known answers of DICe_TestCorrelateParams.cpp:78-140; sub-pixel shifts
+ *                         (DICe_TestInterpShift.cpp:70-128); CONDITION against ||H||_F^2 / |det H| of the u,v
+ *                         block built from the pinned gradients
+ *   correlate (simplex) . DICe_TestCorrelateParams (SIMPLEX, SIMPLEX_THEN_GRADIENT_BASED) and the
+ *                         dice_correlate integer shifts of DICe_TestAPI.cpp:70-150
+ *   sigma / noise ....... SIGMA of the single_speckled_shift gold (2e-6)
+ *   statuses ............ flat subsets, field of view, failed initial guess, max iterations + fallback
+ *   polygon_pixels ...... conformal(_d).rawi golden raster (DICe_TestSubset.cpp:248-309)
+ *   lu_inverse .......... numpy inverse; exactly singular input leaves the LU factors
+ * parity unpinned: the uncertainty fields FIELD_1..7 (computeUncertaintyFields), Nelder-Mead
+ * iteration counts and which stop rule fired, and sigma/noise beyond the seven gold subsets; they
+ * follow the cited lines by review only.
  */
 #ifndef DICE_ORACLE_H
 #define DICE_ORACLE_H

@@ -1,0 +1,215 @@
+// C++ tests of the Schema mirror, in the shape of the reference's component
+// tests: DICe_TestCorrelateParams.cpp:78-140, DICe_TestCorrelateConformal.cpp:75-135,
+// the parallel_subsets regression gold, and the conformal raster golden.
+//
+//   test_schema correlate_params <ref.f32> <def.f32> <w> <h> <backend 0|1>
+//   test_schema correlate_conformal <ref.f32> <def.f32> <w> <h> <backend 0|1>
+//   test_schema parallel_subsets <ref.f32> <def.f32> <w> <h> <gold.txt> <backend 0|1>
+//   test_schema polygon            (prints the owned pixels "y x" of the golden's polygons)
+//   test_schema errors             (error behaviour that needs no GPU)
+//
+// Images are raw row-major float32.  The exit status is the number of failures.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dice_schema.hpp"
+
+using namespace dice_hip;
+
+namespace {
+
+std::vector<float> read_raw(const char *path, int w, int h) {
+  std::vector<float> v((size_t)w * h);
+  FILE *f = std::fopen(path, "rb");
+  if (!f || std::fread(v.data(), sizeof(float), v.size(), f) != v.size()) {
+    std::cerr << "cannot read " << path << "\n";
+    std::exit(100);
+  }
+  std::fclose(f);
+  return v;
+}
+
+const double u_exact = 9.8, v_exact = -7.62, t_exact = 0.262, ex_exact = 0.121, ey_exact = 0.121;
+
+// One pass over the four optimizers x two interpolants, as the reference's tests loop.
+int optimizer_sweep(Schema &schema, ParameterList &params, double errtol, const char *label) {
+  const Optimization_Method opts[] = {SIMPLEX, SIMPLEX_THEN_GRADIENT_BASED, GRADIENT_BASED,
+                                      GRADIENT_BASED_THEN_SIMPLEX};
+  const Interpolation_Method interps[] = {BILINEAR, KEYS_FOURTH};
+  int errors = 0;
+  for (Optimization_Method opt : opts)
+    for (Interpolation_Method interp : interps) {
+      params.set("correlation_routine", GENERIC_ROUTINE);
+      params.set("interpolation_method", interp);
+      params.set("initialization_method", USE_FIELD_VALUES);
+      params.set("optimization_method", opt);
+      schema.set_params(params);
+      schema.local_field_value(0, SUBSET_COORDINATES_X_FS) = 30;
+      schema.local_field_value(0, SUBSET_COORDINATES_Y_FS) = 30;
+      schema.local_field_value(0, SIGMA_FS) = 0.0;
+      schema.local_field_value(0, GAMMA_FS) = 0.0;
+      schema.local_field_value(0, STATUS_FLAG_FS) = 0;
+      schema.local_field_value(0, SUBSET_DISPLACEMENT_X_FS) = u_exact - 0.25;
+      schema.local_field_value(0, SUBSET_DISPLACEMENT_Y_FS) = v_exact - 0.25;
+      schema.local_field_value(0, ROTATION_Z_FS) = t_exact - 0.05;
+      schema.local_field_value(0, NORMAL_STRETCH_XX_FS) = 0.0;
+      schema.local_field_value(0, NORMAL_STRETCH_YY_FS) = 0.0;
+      schema.local_field_value(0, SHEAR_STRETCH_XY_FS) = 0.0;
+      schema.execute_correlation();
+      bool bad = false;
+      bad |= std::abs(schema.local_field_value(0, SUBSET_DISPLACEMENT_X_FS) - u_exact) > errtol;
+      bad |= std::abs(schema.local_field_value(0, SUBSET_DISPLACEMENT_Y_FS) - v_exact) > errtol;
+      bad |= std::abs(schema.local_field_value(0, ROTATION_Z_FS) - t_exact) > errtol;
+      bad |= std::abs(schema.local_field_value(0, NORMAL_STRETCH_XX_FS) - ex_exact) > errtol;
+      bad |= std::abs(schema.local_field_value(0, NORMAL_STRETCH_YY_FS) - ey_exact) > errtol;
+      bad |= std::abs(schema.local_field_value(0, SHEAR_STRETCH_XY_FS)) > errtol;
+      bad |= std::abs(schema.local_field_value(0, STATUS_FLAG_FS) - 1.0) > errtol;  // INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL
+      bad |= std::abs(schema.local_field_value(0, SIGMA_FS) + 1.0) < errtol;      // sigma != -1
+      std::cout << label << (bad ? " FAIL" : " PASS") << " opt " << schema.optimization_method() << " interp "
+                << schema.interpolation_method() << " u " << schema.local_field_value(0, SUBSET_DISPLACEMENT_X_FS)
+                << " v " << schema.local_field_value(0, SUBSET_DISPLACEMENT_Y_FS) << " t "
+                << schema.local_field_value(0, ROTATION_Z_FS) << " ex "
+                << schema.local_field_value(0, NORMAL_STRETCH_XX_FS) << " ey "
+                << schema.local_field_value(0, NORMAL_STRETCH_YY_FS) << " g "
+                << schema.local_field_value(0, SHEAR_STRETCH_XY_FS) << " sigma "
+                << schema.local_field_value(0, SIGMA_FS) << " gamma " << schema.local_field_value(0, GAMMA_FS)
+                << " flag " << schema.local_field_value(0, STATUS_FLAG_FS) << "\n";
+      errors += bad ? 1 : 0;
+    }
+  return errors;
+}
+
+ParameterList all_shape_params(int backend) {
+  ParameterList params;
+  params.set("enable_translation", 1);
+  params.set("enable_rotation", 1);
+  params.set("enable_normal_strain", 1);
+  params.set("enable_shear_strain", 1);
+  params.set("robust_solver_tolerance", 1.0E-4);
+  params.set("correlation_backend", backend);
+  return params;
+}
+
+int correlate_params(const std::vector<float> &ref, const std::vector<float> &def, int w, int h, int backend) {
+  ParameterList params = all_shape_params(backend);
+  Schema schema(std::vector<double>{30}, std::vector<double>{30}, 21, nullptr, &params);
+  schema.set_ref_image(w, h, ref.data());
+  schema.set_def_image(w, h, def.data());
+  return optimizer_sweep(schema, params, 1.0E-2, "square");
+}
+
+int correlate_conformal(const std::vector<float> &ref, const std::vector<float> &def, int w, int h, int backend) {
+  // four sided shape at odd angles (DICe_TestCorrelateConformal.cpp:94-103)
+  Conformal_Area_Def subset_def;
+  subset_def.boundary.emplace_back(std::vector<int>{25, 35, 39, 20}, std::vector<int>{22, 20, 40, 35});
+  std::map<int, Conformal_Area_Def> defs{{0, subset_def}};
+  ParameterList params = all_shape_params(backend);
+  Schema schema(std::vector<double>{30}, std::vector<double>{30}, -1, &defs, &params);
+  schema.set_ref_image(w, h, ref.data());
+  schema.set_def_image(w, h, def.data());
+  return optimizer_sweep(schema, params, 5.0E-2, "conformal");
+}
+
+int parallel_subsets(const std::vector<float> &ref, const std::vector<float> &def, int w, int h,
+                     const char *gold_path, int backend) {
+  // tests/regression/parallel_subsets: default parameters, S = 27
+  std::ifstream in(gold_path);
+  std::string line;
+  std::vector<std::vector<double>> rows;
+  while (std::getline(in, line)) {
+    if (line.empty() || !std::isdigit((unsigned char)line[0])) continue;
+    std::istringstream ss(line);
+    std::vector<double> r;
+    double v;
+    while (ss >> v) r.push_back(v);
+    rows.push_back(r);
+  }
+  std::vector<double> cx, cy;
+  for (auto &r : rows) {
+    cx.push_back(r[1]);
+    cy.push_back(r[2]);
+  }
+  ParameterList params;
+  dice_default_params(&params);
+  params.set("correlation_backend", backend);
+  Schema schema(cx, cy, 27, nullptr, &params);
+  schema.set_ref_image(w, h, ref.data());
+  schema.set_def_image(w, h, def.data());
+  schema.execute_correlation();
+  int errors = 0;
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const double du = std::abs(schema.local_field_value((int)i, SUBSET_DISPLACEMENT_X_FS) - rows[i][3]);
+    const double dv = std::abs(schema.local_field_value((int)i, SUBSET_DISPLACEMENT_Y_FS) - rows[i][4]);
+    const bool bad = du > 5e-5 || dv > 5e-5 || schema.local_field_value((int)i, ITERATIONS_FS) != rows[i][9] ||
+                     schema.local_field_value((int)i, STATUS_FLAG_FS) != rows[i][10];
+    if (bad) {
+      std::cout << "subset " << i << " FAIL du " << du << " dv " << dv << "\n";
+      ++errors;
+    }
+  }
+  std::cout << "parallel_subsets " << rows.size() << " subsets, " << errors << " failures\n";
+  return errors + (rows.size() == 49 ? 0 : 1);
+}
+
+int polygon() {
+  // the two boundary polygons of the conformal raster golden (DICe_TestSubset.cpp:248-309)
+  const Polygon a({940, 951, 964, 980, 959}, {422, 399, 410, 413, 432});
+  const Polygon b({982, 1000, 991, 978}, {426, 428, 456, 445});
+  for (const Polygon *p : {&a, &b})
+    for (const auto &yx : p->get_owned_pixels()) std::cout << yx.first << " " << yx.second << "\n";
+  return 0;
+}
+
+int errors_without_gpu() {
+  int errors = 0;
+  ParameterList params;
+  try {
+    params.set("no_such_parameter", 1.0);
+    ++errors;
+  } catch (const std::invalid_argument &) {
+  }
+  try {
+    Polygon p({1, 2}, {1, 2});
+    ++errors;
+  } catch (const std::invalid_argument &) {
+  }
+  try {
+    params.get("optimization_method");
+    ++errors;
+  } catch (const std::invalid_argument &) {
+  }
+  dice_default_params(&params);
+  if (params.get("optimization_method") != GRADIENT_BASED_THEN_SIMPLEX) ++errors;
+  if (params.get("interpolation_method") != KEYS_FOURTH) ++errors;
+  tracking_default_params(&params);
+  if (params.get("optimization_method") != SIMPLEX || params.get("correlation_routine") != TRACKING_ROUTINE) ++errors;
+  std::cout << "errors " << (errors ? "FAIL" : "PASS") << "\n";
+  return errors;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) return 99;
+  const std::string cmd = argv[1];
+  try {
+    if (cmd == "polygon") return polygon();
+    if (cmd == "errors") return errors_without_gpu();
+    if (argc < 7) return 99;
+    const int w = std::atoi(argv[4]), h = std::atoi(argv[5]);
+    const std::vector<float> ref = read_raw(argv[2], w, h), def = read_raw(argv[3], w, h);
+    if (cmd == "correlate_params") return correlate_params(ref, def, w, h, std::atoi(argv[6]));
+    if (cmd == "correlate_conformal") return correlate_conformal(ref, def, w, h, std::atoi(argv[6]));
+    if (cmd == "parallel_subsets" && argc >= 8) return parallel_subsets(ref, def, w, h, argv[6], std::atoi(argv[7]));
+  } catch (const std::exception &e) {
+    std::cerr << "exception: " << e.what() << "\n";
+    return 98;
+  }
+  return 99;
+}

@@ -1,7 +1,7 @@
 """Pins the CPU oracle against the reference's own fixtures and closed forms.
 
 Every check here uses data or known answers that the reference's tests hold
-(SURVEY-listed component tests and regression golds), or a mathematical fact
+(component tests and regression golds), or a mathematical fact
 that a plausible restatement mistake would break.  CPU only.
 """
 import math
@@ -215,6 +215,32 @@ def test_api_integer_shifts(precision):
         assert np.all(np.abs(f[o.F["ROT_Z"]]) <= 1e-4)
         assert np.all(np.abs(f[o.F["GAMMA"]]) <= 1e-4)
         assert np.all(f[o.F["STATUS"]] == 1)
+
+
+def test_condition_number_closed_form():
+    """CONDITION_NUMBER (DICe_Objective.cpp:558-564): for a translation-only solve that ends at an
+    integer shift every pixel is kept and theta = 0, so the u,v block of H is
+    sum [gx^2, gx gy; gx gy, gy^2] over the subset with the reference gradients, and the stored value is
+    ||H||_F * ||H^-1||_F = ||H||_F^2 / |det H|.  Computed here from the pinned FD gradients."""
+    d = load("speckle_512.npz")
+    ref = d["ref"].astype(np.float32)
+    defm = np.roll(ref, (-1, 2), axis=(0, 1))  # defm[y, x] = ref[y + 1, x - 2]: u = 2, v = -1
+    p = o.default_params()
+    p.optimization_method = 1
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 0
+    p.compute_uncertainty_fields = 0
+    cx = np.array([100.0, 250.0, 400.0, 180.0])
+    cy = np.array([120.0, 300.0, 200.0, 420.0])
+    f = o.correlate(p, ref, defm, o.new_fields(cx, cy), subset_size=31, precision="double")
+    assert np.all(f[o.F["STATUS"]] == 1)
+    assert np.allclose(f[o.F["DISP_X"]], 2.0, atol=1e-3) and np.allclose(f[o.F["DISP_Y"]], -1.0, atol=1e-3)
+    gx, gy = o.fd_gradients(ref.astype(np.float64))
+    for i in range(len(cx)):
+        sl = (slice(int(cy[i]) - 15, int(cy[i]) + 16), slice(int(cx[i]) - 15, int(cx[i]) + 16))
+        a, b = gx[sl].ravel(), gy[sl].ravel()
+        h00, h01, h11 = (a * a).sum(), (a * b).sum(), (b * b).sum()
+        cond = (h00 ** 2 + 2 * h01 ** 2 + h11 ** 2) / abs(h00 * h11 - h01 ** 2)
+        assert abs(f[o.F["CONDITION"]][i] - cond) <= 1e-5 * cond, (i, f[o.F["CONDITION"]][i], cond)
 
 
 def test_lu_inverse_against_numpy_and_singular():
