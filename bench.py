@@ -159,6 +159,16 @@ def main():
     dvv = np.abs(hist[:, dh.F["DISP_Y"]] - vt)[conv]
     du_w = np.abs(hist[:, dh.F["DISP_X"]] - ut_w)[conv]
 
+    # HBM-side bytes per launch of the batched kernel come from rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a copy kernel of
+    # known size; the summary that holds them is committed with the profiles
+    traffic = None
+    traffic_src = os.path.join(ROOT, "profiles", "correlate_batched_c2_hbm_traffic.md")
+    if (W, S, args.step, len(frames)) == (2048, 41, 5, 10) and os.path.exists(traffic_src):
+        for line in open(traffic_src):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
+
     # roofline of the batched correlation kernel: algorithmic bytes per launch / launch time
     bytes_per_frame = algorithmic_bytes(iters, status, S).sum(axis=1)
     avg_bytes = float(bytes_per_frame.mean())
@@ -188,7 +198,9 @@ def main():
                    "mean_abs_u_err_vs_window_avg_px": round(float(du_w.mean()), 5),
                    "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic) if traffic else None,
+                     "traffic_source": "profiles/correlate_batched_c2_hbm_traffic.md" if traffic else None,
                      "kernel": "correlate_batched_kernel (GN, Keys-4, f32)",
                      "avg_launch_ms": round(kern_ms, 4), "launches_timed": kern_cnt,
                      "algorithmic_bytes_per_launch": round(avg_bytes)},
