@@ -79,7 +79,7 @@ EXPORTS = [
     "dice_hip_correlate", "dice_hip_correlate_device", "dice_hip_correlate_sequence",
     "dice_hip_synchronize", "dice_hip_stream", "dice_hip_kernel_timing", "dice_hip_set_kernel_timing",
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
-    "dice_hip_correlate_points",
+    "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint",
 ]
 
 _lib = None
@@ -134,6 +134,8 @@ def lib():
         L.dice_hip_get_launch_shape.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
         L.dice_hip_set_launch_shape.argtypes = [P, I, I, I]
         L.dice_hip_correlate_points.argtypes = [P, I, I, P, I, I, P, I, I, P, I]
+        L.dice_hip_gamma.argtypes = [P, I, P, P]
+        L.dice_hip_gamma_adjoint.argtypes = [P, I, P, P, P, P]
         _lib = L
     return _lib
 
@@ -276,6 +278,18 @@ class Context:
     def autotune(self, fields_ptr, first_frame=True):
         self._check(self._lib.dice_hip_autotune(self.h, ctypes.c_void_p(fields_ptr), 1 if first_frame else 0),
                     "autotune")
+
+    def gamma(self, params_ptr, gamma_ptr, mode=MODE_BATCHED):
+        """dice_hip_gamma: gamma of every subset at device params (n x 6 doubles) into gamma (n doubles)."""
+        self._check(self._lib.dice_hip_gamma(self.h, mode, ctypes.c_void_p(params_ptr), ctypes.c_void_p(gamma_ptr)),
+                    "gamma")
+
+    def gamma_adjoint(self, params_ptr, dgamma_ptr, dparams_ptr, dimage_ptr=0, mode=MODE_BATCHED):
+        """dice_hip_gamma_adjoint: dL/dp (n x 6) and, if dimage_ptr, dL/dI_def added to a W x H float image."""
+        self._check(self._lib.dice_hip_gamma_adjoint(self.h, mode, ctypes.c_void_p(params_ptr),
+                                                     ctypes.c_void_p(dgamma_ptr), ctypes.c_void_p(dparams_ptr),
+                                                     ctypes.c_void_p(dimage_ptr) if dimage_ptr else None),
+                    "gamma_adjoint")
 
     def launch_shape(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -503,6 +503,38 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
   return enqueue_correlate(ctx, mode, device_fields, first_frame, false);
 }
 
+static int enqueue_gamma(dice_hip_context *ctx, int mode, const dice_hip::GammaArgs &ga) {
+  if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->params.compute_def_gradients)
+    return ctx->fail(DICE_HIP_EINVAL, "the differentiation path interpolates the deformed image itself; "
+                                      "compute_def_gradients must be 0");
+  (void)hipSetDevice(ctx->device);
+  DImage ref, def;
+  DSubsets ss;
+  fill_views(ctx, ref, def, ss);
+  const KParams kp = make_kparams(ctx->params, 0);
+  const Launch L = current_launch(ctx, mode);
+  HIP_TRY(ctx, dice_hip::launch_gamma(L, ctx->rstorage, kp, ref, def, ss, ga, ctx->work.as<int>(), ctx->stream),
+          "gamma launch");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_gamma(dice_hip_context *ctx, int mode, const double *device_params, double *device_gamma) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!device_params || !device_gamma) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  return enqueue_gamma(ctx, mode, dice_hip::GammaArgs{device_params, nullptr, device_gamma, nullptr, nullptr, 0});
+}
+
+int dice_hip_gamma_adjoint(dice_hip_context *ctx, int mode, const double *device_params,
+                           const double *device_dgamma, double *device_dparams, float *device_dimage) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!device_params || !device_dgamma || !device_dparams) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  return enqueue_gamma(ctx, mode,
+                       dice_hip::GammaArgs{device_params, device_dgamma, nullptr, device_dparams, device_dimage, 1});
+}
+
 int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int first_frame) {
   if (!ctx) return DICE_HIP_EINVAL;
   if (!fields) return ctx->fail(DICE_HIP_EINVAL, "null fields");

@@ -180,6 +180,17 @@ __device__ __forceinline__ void keys_weights(float t, float w[6]) {
   w[5] = (t2 - t3) * (1.0f / 12.0f);
 }
 
+// d/dt of keys_weights (the slopes of the six cubics).
+__device__ __forceinline__ void keys_weight_slopes(float t, float w[6]) {
+  const float t2 = t * t;
+  w[0] = (3.0f * t2 - 4.0f * t + 1.0f) * (1.0f / 12.0f);
+  w[1] = -1.75f * t2 + 2.5f * t - (2.0f / 3.0f);
+  w[2] = 4.0f * t2 - (14.0f / 3.0f) * t;
+  w[3] = -4.0f * t2 + (10.0f / 3.0f) * t + (2.0f / 3.0f);
+  w[4] = 1.75f * t2 - t - (1.0f / 12.0f);
+  w[5] = (2.0f * t - 3.0f * t2) * (1.0f / 12.0f);
+}
+
 // 6x6 Keys sum from rows of a 2-D array (LDS window or image) at `b`, pitch p.
 // Rows are reached by adding the pitch (no per-row multiply).
 template <typename P>
@@ -437,7 +448,7 @@ struct Solver {
   static constexpr bool kFast = kIsWave && !DEFGRAD;
   lds_double *pers;
   bool last_fast;
-  int wx0, wy0, wpitch;
+  int wx0, wy0, wpitch, wrows;
   bool overflow;
   // subset
   int cx, cy, n, S, hw;
@@ -452,7 +463,7 @@ struct Solver {
 
   __device__ __forceinline__ Solver(const KParams &k, const DImage &r, const DImage &d, G g, lds_float *w, int cap)
       : kp(k), ref(r), def(d), grp(g), win(w), win_cap(kFast ? cap - kPersistFloats : cap), wx0(0), wy0(0),
-        wpitch(0), overflow(false) {
+        wpitch(0), wrows(0), overflow(false) {
     pers = kFast ? (lds_double *)(w + win_cap) : nullptr;
     last_fast = false;
     en[0] = en[1] = 1.0f;
@@ -619,7 +630,7 @@ struct Solver {
     y1 = min(y1, def.h - 1);
     const int ww = x1 - x0 + 1, hh = y1 - y0 + 1;
     if (ww <= 0 || hh <= 0) {
-      wx0 = x0; wy0 = y0; wpitch = 1;
+      wx0 = x0; wy0 = y0; wpitch = 1; wrows = 0;
       return true;  // nothing is kept
     }
     if (ww * hh > win_cap) {
@@ -629,15 +640,35 @@ struct Solver {
     wx0 = __builtin_amdgcn_readfirstlane(x0);  // uniform already: keep it scalar
     wy0 = __builtin_amdgcn_readfirstlane(y0);
     wpitch = __builtin_amdgcn_readfirstlane(ww);
+    wrows = __builtin_amdgcn_readfirstlane(hh);
     grp.sync();
     const auto *P = Pix<T>::ptr(def.pix) + y0 * def.w + x0;
     const float inv = 1.0f / (float)ww;
     const int tot = ww * hh;
+    if constexpr (kIsWave && std::is_same<T, float>::value) {
+      // direct global -> LDS loads (global_load_lds_dword): the 64 lanes of one
+      // load fill 64 consecutive floats of the slot, so window element k =
+      // 64 j + lane comes from batch j.  No VGPRs hold the data and every batch
+      // is in flight before the single wait.
+      const int lane = grp.rank();
 #pragma unroll 4
-    for (int k = grp.rank(); k < tot; k += G::kSize) {
-      const int r = (int)(((float)k + 0.5f) * inv);
-      const int c = k - r * ww;
-      win[k] = (float)P[r * def.w + c];
+      for (int k0 = 0; k0 < tot; k0 += kWave) {
+        const int k = k0 + lane;
+        if (k < tot) {
+          const int r = (int)(((float)k + 0.5f) * inv);
+          const int c = k - r * ww;
+          __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(P + (r * def.w + c)),
+                                           (__attribute__((address_space(3))) void *)(win + k0), 4, 0, 0);
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+    } else {
+#pragma unroll 4
+      for (int k = grp.rank(); k < tot; k += G::kSize) {
+        const int r = (int)(((float)k + 0.5f) * inv);
+        const int c = k - r * ww;
+        win[k] = (float)P[r * def.w + c];
+      }
     }
     grp.sync();
     return true;
@@ -958,6 +989,205 @@ struct Solver {
     float g = (float)gam;
     if (kp.normalize_gamma && n_def_active > 0) g = (float)((double)g / (double)n_def_active);
     return g;
+  }
+
+  // ---------------------------------------------- differentiation path ---
+  // Objective of the adjoint: the ZNSSD gamma of the warped subset at p (the
+  // quantity Subset::gamma returns after Subset::initialize, DICe_Subset.cpp:
+  // 323-340, 373-440), gamma = 2 - 2 rho with rho = sdr / (|d - dbar| |r - rbar|)
+  // over the kept pixels; no clamp and no normalisation by the pixel count.
+  // ZStats: the FP64 sums that define it.  Returns false (gamma -1, gradient 0)
+  // for a flat or empty subset.
+  struct ZStats {
+    double n, md, mr, nd, nr, rho;
+  };
+  __device__ __forceinline__ bool znssd_stats(const Affine &A, ZStats &z) {
+    double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      float ox, oy;
+      A.disp(dx, dy, ox, oy);
+      if (!kept(ox, oy)) return;
+      const double d = sample(ox, oy);
+      const double r = ref_value(x - ref.ox, y - ref.oy);
+      acc[0] += 1.0;
+      acc[1] += d;
+      acc[2] += d * d;
+      acc[3] += r;
+      acc[4] += r * r;
+      acc[5] += d * r;
+    });
+    grp.sum(acc);
+    z.n = acc[0];
+    if (z.n <= 0.0) return false;
+    z.md = acc[1] / z.n;
+    z.mr = acc[3] / z.n;
+    const double sdd = fmax(acc[2] - z.n * z.md * z.md, 0.0);
+    const double srr = fmax(acc[4] - z.n * z.mr * z.mr, 0.0);
+    if (sdd <= 1e-13 * acc[2] || srr <= 1e-13 * acc[4]) return false;
+    z.nd = sqrt(sdd);
+    z.nr = sqrt(srr);
+    z.rho = (acc[5] - z.n * z.md * z.mr) / (z.nd * z.nr);
+    return true;
+  }
+
+  // Deformed intensity and its x / y derivative at x' = cx + ddx, y' = cy + ddy
+  // (the derivative of the interpolant itself, so that it matches finite
+  // differences of the forward): Keys weights and their t-derivatives are
+  // recomputed, nothing is stored.  tap(base offset, pitch, weights) gives the
+  // scatter of the adjoint.
+  __device__ __forceinline__ float sample_d(float ddx, float ddy, float &Ix, float &Iy) const {
+    const float flx = floorf(ddx), fly = floorf(ddy);
+    const float tx = ddx - flx, ty = ddy - fly;
+    const int ix = cx - def.ox + (int)flx;
+    const int iy = cy - def.oy + (int)fly;
+    if (INTERP == 0) {
+      float b00, b10, b01, b11;
+      if constexpr (kIsWave) {
+        const lds_float *b = win + (__umul24(iy - wy0, wpitch) + (ix - wx0));
+        b00 = b[0]; b10 = b[1]; b01 = b[wpitch]; b11 = b[wpitch + 1];
+      } else {
+        const auto *b = Pix<T>::ptr(def.pix) + iy * def.w + ix;
+        b00 = (float)b[0]; b10 = (float)b[1]; b01 = (float)b[def.w]; b11 = (float)b[def.w + 1];
+      }
+      Ix = (1.0f - ty) * (b10 - b00) + ty * (b11 - b01);
+      Iy = (1.0f - tx) * (b01 - b00) + tx * (b11 - b10);
+      return b00 * (1.0f - tx) * (1.0f - ty) + b10 * tx * (1.0f - ty) + b11 * tx * ty + b01 * (1.0f - tx) * ty;
+    }
+    float wx[6], wy[6], dwx[6], dwy[6];
+    keys_weights(tx, wx);
+    keys_weights(ty, wy);
+    keys_weight_slopes(tx, dwx);
+    keys_weight_slopes(ty, dwy);
+    float v = 0.0f, vx = 0.0f, vy = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      float rs = 0.0f, rd = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        float t;
+        if constexpr (kIsWave) t = win[__umul24(iy - 2 + m - wy0, wpitch) + (ix - 2 + k - wx0)];
+        else t = (float)Pix<T>::ptr(def.pix)[(iy - 2 + m) * def.w + ix - 2 + k];
+        rs = fmaf(t, wx[k], rs);
+        rd = fmaf(t, dwx[k], rd);
+      }
+      v = fmaf(rs, wy[m], v);
+      vx = fmaf(rd, wy[m], vx);
+      vy = fmaf(rs, dwy[m], vy);
+    }
+    Ix = vx;
+    Iy = vy;
+    return v;
+  }
+
+  // Adds g * (interpolation weight) to every tap of the sample at (ddx, ddy):
+  // into acc (LDS, the window's layout, pitch wpitch) when acc is set, else
+  // into the global image gradient.
+  __device__ __forceinline__ void scatter_taps(float ddx, float ddy, float g, lds_float *acc, float *dimg) const {
+    const float flx = floorf(ddx), fly = floorf(ddy);
+    const float tx = ddx - flx, ty = ddy - fly;
+    const int ix = cx - def.ox + (int)flx;
+    const int iy = cy - def.oy + (int)fly;
+    float wx[6], wy[6];
+    int lead = 2, taps = 6;
+    if (INTERP == 0) {
+      wx[0] = 1.0f - tx; wx[1] = tx; wy[0] = 1.0f - ty; wy[1] = ty;
+      lead = 0;
+      taps = 2;
+    } else {
+      keys_weights(tx, wx);
+      keys_weights(ty, wy);
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      if (m >= taps) break;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        if (k >= taps) break;
+        const float c = g * wy[m] * wx[k];
+        const int yy = iy - lead + m, xx = ix - lead + k;
+        if (acc) __hip_atomic_fetch_add(acc + (__umul24(yy - wy0, wpitch) + (xx - wx0)), c, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
+        else unsafeAtomicAdd(dimg + (size_t)yy * def.w + xx, c);
+      }
+    }
+  }
+
+  // Forward of the differentiation path: gamma of the subset at p.
+  __device__ __forceinline__ bool gamma_at(const float (&p)[6], double &gamma_out) {
+    Affine A;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.p[k] = p[k];
+    A.prepare();
+    if (!stage_affine(A)) return false;
+    ZStats z;
+    gamma_out = znssd_stats(A, z) ? 2.0 - 2.0 * z.rho : -1.0;
+    return true;
+  }
+
+  // Adjoint: dgamma/dd_j = (2 / nd) (rho dhat_j - rhat_j) for the kept pixels,
+  // chained through the interpolant (d/dx', d/dy') and the affine map (the
+  // residual rows of Affine::residual) into dL/dp, and through the tap weights
+  // into dL/dI_def.  Returns false if the BATCHED window did not fit.
+  __device__ __forceinline__ bool gamma_adjoint(const float (&p)[6], double dl_dgamma, double (&dp)[6],
+                                                float *dimg) {
+    Affine A;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.p[k] = p[k];
+    A.prepare();
+    if (!stage_affine(A)) return false;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dp[k] = 0.0;
+    ZStats z;
+    if (!znssd_stats(A, z) || dl_dgamma == 0.0) return true;
+    // a second copy of the window's layout in the slot accumulates dL/dI
+    // with LDS atomics when it fits; otherwise taps go to global atomics
+    lds_float *acc = nullptr;
+    if constexpr (kIsWave) {
+      const int cells = wpitch * wrows;
+      if (cells > 0 && 2 * cells <= win_cap && dimg) {
+        acc = win + cells;
+        for (int k = grp.rank(); k < cells; k += G::kSize) acc[k] = 0.0f;
+        grp.sync();
+      }
+    }
+    const double s = dl_dgamma * 2.0 / z.nd;
+    double g6[6] = {0, 0, 0, 0, 0, 0};
+    for_pixels([&](int x, int y, bool a) {
+      if (!a) return;
+      const float dx = (float)(x - cx), dy = (float)(y - cy);
+      float ox, oy;
+      A.disp(dx, dy, ox, oy);
+      if (!kept(ox, oy)) return;
+      float Ix, Iy;
+      const double d = sample_d(ox, oy, Ix, Iy);
+      const double r = ref_value(x - ref.ox, y - ref.oy);
+      const double gj = s * (z.rho * (d - z.md) / z.nd - (r - z.mr) / z.nr);
+      float row[6];
+      A.residual(dx, dy, Ix, Iy, row);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) g6[k] = fma(gj, (double)row[k], g6[k]);
+      if (dimg) scatter_taps(ox, oy, (float)gj, acc, dimg);
+    });
+    grp.sum(g6);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dp[k] = g6[k];
+    if constexpr (kIsWave) {
+      if (acc) {
+        grp.sync();
+        const int cells = wpitch * wrows;
+        for (int k = grp.rank(); k < cells; k += G::kSize) {
+          const float v = acc[k];
+          if (v != 0.0f) {
+            const int r = k / wpitch, c = k - r * wpitch;
+            unsafeAtomicAdd(dimg + (size_t)(wy0 + r) * def.w + (wx0 + c), v);
+          }
+        }
+        grp.sync();
+      }
+    }
+    return true;
   }
 
   // Objective_ZNSSD::computeUpdateFast.  Returns the solver status (or
@@ -1477,6 +1707,33 @@ struct Solver {
     A.p[5] = kp.en_shear ? (float)f(F_SS_XY) : 0.0f;
   }
 
+  // One subset of the differentiation path: gamma at p (forward) or the
+  // adjoint; a BATCHED wave whose window does not fit hands the subset over.
+  __device__ __forceinline__ void run_gamma(const DSubsets &ss, int gid, const GammaArgs &ga, int *ov_list,
+                                            int *ov_count) {
+    set_subset(ss, gid);
+    float p[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) p[k] = (float)ga.params[(size_t)gid * 6 + k];
+    if (ga.adjoint) {
+      double dp[6];
+      if (!gamma_adjoint(p, ga.dgamma[gid], dp, ga.dimage)) {
+        push(ov_list, ov_count, gid);
+        return;
+      }
+      if (grp.rank() == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ga.dparams[(size_t)gid * 6 + k] = dp[k];
+    } else {
+      double g;
+      if (!gamma_at(p, g)) {
+        push(ov_list, ov_count, gid);
+        return;
+      }
+      if (grp.rank() == 0) ga.gamma[gid] = g;
+    }
+  }
+
   __device__ __forceinline__ void push(int *list, int *count, int gid) {
     if (grp.rank() == 0) list[atomicAdd(count, 1)] = gid;
   }
@@ -1610,6 +1867,9 @@ struct Solver {
   }
 };
 
+// A PLAIN pass over a device list launches min(n, kListGroups) workgroups.
+constexpr int kListGroups = 2048;
+
 // Work items of a pass: all subsets (list == nullptr) or a device list.
 struct Items {
   const int *list;
@@ -1632,9 +1892,14 @@ __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams 
     s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, nullptr, nullptr);
 }
 
-// BATCHED: persistent waves pull items from a global counter until it passes
-// the item count (every wave reaches that exit).  The register budget is set
-// for DICE_BATCHED_WAVES_PER_EU waves per SIMD (4: 128 VGPRs).
+// BATCHED: persistent waves pull items from work counters until every range
+// is exhausted (every wave reaches that exit).  The items are split into
+// kWorkRanges contiguous ranges with a counter each.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup b starts on range b mod 8: the
+// subsets that one XCD works on at a time are neighbours in the grid, whose
+// windows overlap, and its L2 serves them.  An exhausted range sends the wave
+// on to the next one (work stealing).  The register budget is set for
+// DICE_BATCHED_WAVES_PER_EU waves per SIMD (4: 128 VGPRs).
 #ifndef DICE_BATCHED_WAVES_PER_EU
 #define DICE_BATCHED_WAVES_PER_EU 4
 #endif
@@ -1649,20 +1914,82 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   Solver<T, WaveGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                           lds_per_wave);
   const int total = items.total(ss.n);
+  int range = (int)(blockIdx.x % kWorkRanges);
 #pragma unroll 1
-  while (true) {
+  for (int visited = 0; visited < kWorkRanges;) {
+    const int lo = (int)((long long)total * range / kWorkRanges);
+    const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
     int k = 0;
-    if (lane == 0) k = atomicAdd(pa.counter, 1);
-    k = __shfl(k, 0, kWave);
-    if (k >= total) break;
+    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    k = __builtin_amdgcn_readfirstlane(k) + lo;
+    if (k >= hi) {
+      range = range + 1 == kWorkRanges ? 0 : range + 1;
+      ++visited;
+      continue;
+    }
     s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
                                 pa.ov_count);
   }
 }
 
+// Differentiation path: forward gamma or adjoint for every item.
+template <typename T, int INTERP>
+__global__ __launch_bounds__(kPlainThreads) void gamma_plain_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
+                                                                    GammaArgs ga, PassArgs pa) {
+  __shared__ double red[4 * kMaxRed];
+  const Items items{pa.items, pa.item_count};
+  const int total = items.total(ss.n);
+  BlockGroup g{(lds_double *)red};
+  Solver<T, BlockGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)red, 0);
+#pragma unroll 1
+  for (int k = blockIdx.x; k < total; k += gridDim.x) s.run_gamma(ss, items.at(k), ga, nullptr, nullptr);
+}
+
+template <typename T, int INTERP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHED_WAVES_PER_EU))) void gamma_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
+                                                            GammaArgs ga, PassArgs pa, int lds_per_wave) {
+  extern __shared__ float lds[];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const Items items{pa.items, pa.item_count};
+  WaveGroup g{lane};
+  Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
+                                        lds_per_wave);
+  const int total = items.total(ss.n);
+  int range = (int)(blockIdx.x % kWorkRanges);
+#pragma unroll 1
+  for (int visited = 0; visited < kWorkRanges;) {
+    const int lo = (int)((long long)total * range / kWorkRanges);
+    const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
+    int k = 0;
+    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    k = __builtin_amdgcn_readfirstlane(k) + lo;
+    if (k >= hi) {
+      range = range + 1 == kWorkRanges ? 0 : range + 1;
+      ++visited;
+      continue;
+    }
+    s.run_gamma(ss, items.at(k), ga, pa.ov_list, pa.ov_count);
+  }
+}
+
+template <typename T, int INTERP, int MODE>
+hipError_t launch_gamma_variant(const Launch &L, const KParams &kp, const DImage &ref, const DImage &def,
+                                const DSubsets &subs, const GammaArgs &ga, const PassArgs &pa, hipStream_t stream) {
+  if (MODE == 0) {
+    const int groups = pa.items ? (subs.n < kListGroups ? subs.n : kListGroups) : subs.n;
+    hipLaunchKernelGGL((gamma_plain_kernel<T, INTERP>), dim3(groups), dim3(kPlainThreads), 0, stream, kp, ref, def,
+                       subs, ga, pa);
+    return hipGetLastError();
+  }
+  const size_t shm = (size_t)L.waves_per_group * L.lds_floats_per_wave * sizeof(float);
+  hipLaunchKernelGGL((gamma_batched_kernel<T, INTERP>), dim3(L.groups), dim3(L.waves_per_group * kWave), shm, stream,
+                     kp, ref, def, subs, ga, pa, L.lds_floats_per_wave);
+  return hipGetLastError();
+}
+
 // Launch one pass of one kernel variant.  MODE 0: PLAIN, MODE 1: BATCHED.
 // A PLAIN pass over a device list launches min(n, list_groups) workgroups.
-constexpr int kListGroups = 2048;
 template <typename T, int INTERP, bool DEFGRAD, bool SX, int MODE>
 hipError_t launch_variant(const Launch &L, const KParams &kp, const DImage &ref, const DImage &def,
                           const DSubsets &subs, double *fields, const PassArgs &pa, hipStream_t stream) {

@@ -100,11 +100,28 @@ struct PassArgs {
   const int *item_count;
   int *fb_list, *fb_count;
   int *ov_list, *ov_count;
-  int *counter;                // BATCHED work-stealing counter (zeroed before the launch)
+  int *counter;                // BATCHED work counters, kWorkRanges ints (zeroed before the launch)
 };
 
+// Differentiation path (forward gamma / adjoint) of one launch.
+struct GammaArgs {
+  const double *params;        // n x 6: u, v, theta, ex, ey, gxy
+  const double *dgamma;        // adjoint: dL/dgamma, n
+  double *gamma;               // forward: gamma, n
+  double *dparams;             // adjoint: dL/dp, n x 6
+  float *dimage;               // adjoint: dL/dI_def, W x H (added to; may be null)
+  int adjoint;
+};
+
+// BATCHED work ranges (one per XCD of the MI355X).
+constexpr int kWorkRanges = 8;
+// Header of the work area: the range counters, then the fallback count and the
+// overflow counts of pass 0 and pass 1.
+constexpr int kWorkHeader = 16;
+constexpr int kWorkFbCount = kWorkRanges, kWorkOv0Count = kWorkRanges + 1, kWorkOv1Count = kWorkRanges + 2;
+
 // Device ints of the work area of launch_correlate.
-inline size_t work_ints(int n) { return 4 + 3 * (size_t)(n > 0 ? n : 0); }
+inline size_t work_ints(int n) { return kWorkHeader + 3 * (size_t)(n > 0 ? n : 0); }
 
 // Enqueue the correlation of all subsets (primary pass, then the fallback pass
 // when the optimization method has one).  storage: 0 f32, 1 u16.  work: device
@@ -112,6 +129,11 @@ inline size_t work_ints(int n) { return 4 + 3 * (size_t)(n > 0 ? n : 0); }
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,
                             const DImage &def, const DSubsets &subs, double *fields, int *work,
                             hipStream_t stream);
+
+// Enqueue the forward gamma or the adjoint of every subset (BATCHED, then the
+// PLAIN kernel over the subsets whose window did not fit).
+hipError_t launch_gamma(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
+                        const DSubsets &subs, const GammaArgs &ga, int *work, hipStream_t stream);
 
 // Finite-difference gradients (+ optional 5x5 smoothing) of an image.
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,

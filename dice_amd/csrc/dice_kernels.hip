@@ -81,7 +81,7 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
     return plain(storage, kp.interp, defgrad, L, kp, ref, def, subs, fields, pa, stream);
   }
   PassLauncher batched = simplex ? launch_simplex_batched : launch_gn_batched;
-  hipError_t e = launch_zero_ints(pa.counter, 1, stream);
+  hipError_t e = launch_zero_ints(pa.counter, kWorkRanges, stream);
   if (e != hipSuccess) return e;
   pa.ov_list = ov_list;
   pa.ov_count = ov_count;
@@ -97,18 +97,19 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
                             const DSubsets &subs, double *fields, int *work, hipStream_t stream) {
   if (subs.n <= 0) return hipSuccess;
-  // work: [0] work counter, [1] fallback count, [2] / [3] overflow counts of
-  // pass 0 / 1, then the fallback list and the two overflow lists (n each)
+  // work: the header (range counters, fallback and overflow counts), then the
+  // fallback list and the overflow lists of pass 0 and pass 1 (n each)
   const size_t n = (size_t)subs.n;
-  int *fb_list = work + 4, *ov0 = work + 4 + n, *ov1 = work + 4 + 2 * n;
-  hipError_t e = launch_zero_ints(work, 4, stream);
+  int *fb_list = work + kWorkHeader, *ov0 = fb_list + n, *ov1 = fb_list + 2 * n;
+  hipError_t e = launch_zero_ints(work, kWorkHeader, stream);
   if (e != hipSuccess) return e;
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
-  PassArgs pa{0, nullptr, nullptr, fb_list, work + 1, nullptr, nullptr, work};
-  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + 2, stream);
+  PassArgs pa{0, nullptr, nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work};
+  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, stream);
   if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
-  PassArgs pb{1, fb_list, work + 1, nullptr, nullptr, nullptr, nullptr, work};
-  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + 3, stream);
+  PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work};
+  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + kWorkOv1Count,
+                     stream);
 }
 
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,

@@ -114,6 +114,43 @@ def sinusoid_field(amp, period=256.0):
     return f
 
 
+def quadratic_field(coef, cx=0.0, cy=0.0, scale=1.0):
+    """u, v quadratic in the normalised coordinates x = (X - cx) / scale, y = (Y - cy) / scale:
+    u = c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 and v with c6..c11 (12 coefficients)."""
+    c = np.asarray(coef, dtype=np.float64)
+
+    def f(X, Y):
+        x, y = (X - cx) / scale, (Y - cy) / scale
+        u = c[0] + c[1] * x + c[2] * y + c[3] * x * x + c[4] * x * y + c[5] * y * y
+        v = c[6] + c[7] * x + c[8] * y + c[9] * x * x + c[10] * x * y + c[11] * y * y
+        dudx = (c[1] + 2 * c[3] * x + c[4] * y) / scale
+        dudy = (c[2] + c[4] * x + 2 * c[5] * y) / scale
+        dvdx = (c[7] + 2 * c[9] * x + c[10] * y) / scale
+        dvdy = (c[8] + c[10] * x + 2 * c[11] * y) / scale
+        return u, v, dudx, dudy, dvdx, dvdy
+    return f
+
+
+def quadratic_coefficients(peak=1.5, seed=4):
+    """12 random coefficients scaled so that max |(u, v)| over the unit square [-1, 1]^2 is `peak`."""
+    c = np.random.default_rng(seed).normal(size=12)
+    g = np.linspace(-1, 1, 65)
+    x, y = np.meshgrid(g, g)
+    u, v = quadratic_field(c)(x, y)[:2]
+    return c * (peak / max(np.abs(u).max(), np.abs(v).max()))
+
+
+def config4(size=1536, peak=1.5, seed=1, noise_sigma=2.55):
+    """size^2 speckle deformed by a quadratic field with 12 random coefficients (peak |u|, |v| = peak px):
+    returns ref, def and the field function of pixel coordinates."""
+    sp = Speckle(size, size, seed=seed)
+    ref = sp.frame(None)
+    half = (size - 1) / 2.0
+    field = quadratic_field(quadratic_coefficients(peak, seed + 3), half, half, half)
+    de = sp.frame(field, noise_sigma=noise_sigma, noise_seed=seed + 1)
+    return ref, de, field
+
+
 def config1(seed=1):
     """512 x 512 speckle, u = 0.5 + 0.01 X, v = -0.3 + 0.005 Y, noise sigma = 2.55 (1 % of range)."""
     sp = Speckle(512, 512, seed=seed)

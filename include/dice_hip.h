@@ -148,6 +148,26 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
 int dice_hip_correlate_sequence(dice_hip_context *ctx, int mode, int n_frames, const void *const *device_frames,
                                 double *device_fields, double *device_history, int use_graph);
 
+/* Differentiation path.  No counterpart entry point in the reference: the
+ * objective is the one Objective::gamma evaluates (DICe_Objective.cpp:58-76,
+ * Subset::initialize + Subset::gamma, DICe_Subset.cpp:323-440) for the current
+ * images and subsets, at given affine parameters, without a solve:
+ * gamma_i = 2 - 2 rho_i, the ZNSSD of the kept pixels (no clamp, no
+ * normalisation; -1 for a flat subset).  device_params: n x 6 doubles
+ * (u, v, theta, ex, ey, gxy of subset i at [6 i, 6 i + 6)).  mode: PLAIN or
+ * BATCHED.  compute_def_gradients must be 0.  Enqueued on the context stream.
+ *
+ * dice_hip_gamma writes gamma_i (n doubles). */
+int dice_hip_gamma(dice_hip_context *ctx, int mode, const double *device_params, double *device_gamma);
+
+/* Adjoint of dice_hip_gamma for L = sum_i w_i gamma_i with w = device_dgamma
+ * (n doubles): writes dL/dp (n x 6 doubles) and ADDS dL/dI_def to
+ * device_dimage (width x height floats of the deformed image, row-major; NULL
+ * to skip).  Interpolation weights and their derivatives are recomputed, not
+ * stored.  Flat subsets contribute 0. */
+int dice_hip_gamma_adjoint(dice_hip_context *ctx, int mode, const double *device_params,
+                           const double *device_dgamma, double *device_dparams, float *device_dimage);
+
 /* Waits for all work of the context. */
 int dice_hip_synchronize(dice_hip_context *ctx);
 
