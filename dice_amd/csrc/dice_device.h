@@ -95,11 +95,19 @@ __device__ __forceinline__ int wave_max(int v) {
   return max(max(lane_value(v, 0), lane_value(v, 16)), max(lane_value(v, 32), lane_value(v, 48)));
 }
 
+// The lane id, re-read at every use: values derived from it cannot be hoisted
+// out of the pixel loops and stay live across the whole kernel (which pushes
+// the register budget into spills).
+__device__ __forceinline__ int opaque_lane(int lane) {
+  asm volatile("" : "+v"(lane));
+  return lane;
+}
+
 // One wavefront.
 struct WaveGroup {
   int lane;
   static constexpr int kSize = kWave;
-  __device__ __forceinline__ int rank() const { return lane; }
+  __device__ __forceinline__ int rank() const { return opaque_lane(lane); }
   __device__ __forceinline__ void sync() const {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -344,7 +352,7 @@ __device__ __forceinline__ void lu_update6(double A[6][6], const double q[6], do
 // v_readlane, so they are wave-uniform.  Exactly-zero pivot: du = -(factors) q.
 // hrow / qv hold row `lane` of H and q on lanes 0..5 (anything on other lanes).
 __device__ __forceinline__ void wave_lu_update6(double (&hrow)[6], double qv, double (&du)[6]) {
-  const int lane = (int)__lane_id();
+  const int lane = opaque_lane((int)(threadIdx.x % kWave));
   const bool mine = lane < 6;
   double b = -qv;
   bool singular = false;
