@@ -631,7 +631,7 @@ struct Solver {
   // Stage the deformed-image window [x0, x1] x [y0, y1] (local coordinates,
   // clamped to the image) into the wave's LDS slot.  Returns false (and marks
   // the subset as overflowing) when it does not fit.
-  __device__ __forceinline__ bool stage(int x0, int x1, int y0, int y1) {
+  __device__ __forceinline__ bool stage(int x0, int x1, int y0, int y1, bool hand_over = true) {
     x0 = max(x0, 0);
     y0 = max(y0, 0);
     x1 = min(x1, def.w - 1);
@@ -642,7 +642,7 @@ struct Solver {
       return true;  // nothing is kept
     }
     if (ww * hh > win_cap) {
-      overflow = true;
+      if (hand_over) overflow = true;
       return false;
     }
     wx0 = __builtin_amdgcn_readfirstlane(x0);  // uniform already: keep it scalar
@@ -896,15 +896,19 @@ struct Solver {
   }
 
   // Per-subset constants of the fast pass over the active pixels: sum(phi
-  // phi^T) (a full symmetric 6 x 6) and sum(phi) into pers[].
+  // phi^T) (a full symmetric 6 x 6) and sum(phi) into pers[], and the mean of
+  // the reference intensities (computeUpdateFast's F-bar, DICe_Objective.cpp:464)
+  // into meanF, from the same pass.
   __device__ __forceinline__ void precompute_phi() {
-    float pa[27];
+    float pa[29];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) pa[k] = 0.0f;
+    for (int k = 0; k < 29; ++k) pa[k] = 0.0f;
     for_pixels([&](int x, int y, bool a) {
       if (!a) return;
       float F, gx, gy;
       ref_pixel(x - ref.ox, y - ref.oy, F, gx, gy);
+      pa[27] += F;
+      pa[28] += 1.0f;
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       const float ph[6] = {gx, gy, dx * gx, dy * gx, dx * gy, dy * gy};
       int h = 0;
@@ -916,7 +920,7 @@ struct Solver {
       for (int k = 0; k < 6; ++k) pa[21 + k] += ph[k];
     });
     lds_double *P = pers;
-    lds_chunk_sum<27>(pa, [&](int r, double t) {
+    lds_chunk_sum<29>(pa, [&](int r, double t) {
       if (r < 21) {
         int h = r, k = 0, len = 6;  // packed upper triangle: row k holds 6 - k entries
 #pragma unroll
@@ -926,9 +930,11 @@ struct Solver {
         P[6 * k + l] = t;
         P[6 * l + k] = t;
       } else {
-        P[36 + (r - 21)] = t;
+        P[36 + (r - 21)] = t;  // sum(phi), then sum(F) and the pixel count at 42, 43
       }
     });
+    const double cnt = uniform(P[43]);
+    meanF = cnt > 0.0 ? (float)(uniform(P[42]) / cnt) : 0.0f;
   }
 
   // Fast pass: sum(gmf phi) (6) and sum(G) over the active pixels, all kept.
@@ -1388,6 +1394,23 @@ struct Solver {
         return false;
     }
     double acc[6] = {0, 0, 0, 0, 0, 0};  // sssig, n_uhat_g2, n_err_g2, n_err_jg2, r_tot2, n_uhat2
+    // BATCHED, subset away from the image border: every sample x + u, y + v
+    // takes the Keys branch of interpolate_keys_fourth and the fractional
+    // offsets are the fractional parts of (u, v), so the weights are computed
+    // once (the per-pixel float sum x + u only moves them at rounding level)
+    bool uniform_w = false;
+    float wxu[6], wyu[6];
+    int fu = 0, fv = 0;
+    if constexpr (kIsWave && INTERP != 0) {
+      const float m = 1e-3f;
+      uniform_w = (float)(bx0 - ref.ox) + u > 2.5f + m && (float)(bx1 - ref.ox) + u < (float)def.w - 3.5f - m &&
+                  (float)(by0 - ref.oy) + v > 2.5f + m && (float)(by1 - ref.oy) + v < (float)def.h - 3.5f - m;
+      fu = (int)floorf(u);
+      fv = (int)floorf(v);
+      keys_weights(u - (float)fu, wxu);
+      keys_weights(v - (float)fv, wyu);
+    }
+    float p1 = 0.0f, p3 = 0.0f, p4 = 0.0f;  // FP32 lane partials of sum ug^2/g2, jg^2/g2, (sub_r - sub_e)^2/g2
     for_pixels([&](int x, int y, bool a) {
       (void)a;
       const int rx = x - ref.ox, ry = y - ref.oy;
@@ -1399,21 +1422,28 @@ struct Solver {
       const float inv = g2 < 1.0e-4f ? 0.0f : 1.0f / g2;
       const float ug = u * gx + v * gy;
       const float jg = v * gx - u * gy;
-      acc[1] += (double)(ug * ug * inv);
-      acc[2] += (double)(ug * ug * inv);
-      acc[3] += (double)(jg * jg * inv);
       float sub_r;
       if constexpr (kIsWave) {
-        sub_r = keys_single(win, wpitch, wx0, wy0, (float)rx + u, (float)ry + v) - refv;
+        if (uniform_w)
+          sub_r = keys_sum(win + (__umul24(ry + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0)), wpitch, wxu, wyu) - refv;
+        else
+          sub_r = keys_single(win, wpitch, wx0, wy0, (float)rx + u, (float)ry + v) - refv;
       } else {
         sub_r = keys_single(Pix<T>::ptr(def.pix), def.w, 0, 0, (float)rx + u, (float)ry + v) - refv;
       }
       const bool inb = rx >= 0 && ry >= 0 && rx <= def.w - 2 && ry <= def.h - 2;
       const float sub_e = (inb ? Pix<T>::at(def.pix, ry * def.w + rx) : 0.0f) - refv;
-      acc[4] += (double)((sub_r - sub_e) * (sub_r - sub_e) * inv);
-      acc[5] += (double)(u * u + v * v);
+      p1 = fmaf(ug * ug, inv, p1);
+      p3 = fmaf(jg * jg, inv, p3);
+      p4 = fmaf((sub_r - sub_e) * (sub_r - sub_e), inv, p4);
     });
+    acc[1] = p1;
+    acc[3] = p3;
+    acc[4] = p4;
     grp.sum(acc);
+    // n_uhat_g2 and n_err_g2 add the same terms; n_uhat2 adds u^2 + v^2 once per pixel
+    acc[2] = acc[1];
+    acc[5] = (double)(u * u + v * v) * (double)n;
     if (grp.rank() != 0) return true;
     const double sssig = (1.0 + acc[0]) / (double)(n == 0 ? 1 : n);
     const double cos_hat = acc[5] == 0.0 ? 1.0 : sqrt(acc[1] / acc[5]);
@@ -1609,22 +1639,38 @@ struct Solver {
     double acc[1] = {0.0};
     if (bw > 0 && bh > 0) {
       const int tot = bw * bh;
+      // BATCHED: the bbox and its 1-px ring staged in LDS when it fits (the
+      // mask taps of the interior pixels then come from the window)
+      bool in_lds = false;
+      if constexpr (kIsWave) in_lds = stage(min_x - ox - 1, max_x - ox, min_y - oy - 1, max_y - oy, false);
+      const bool fdiv = tot < (1 << 20);  // (k + 0.5) / bw in float is exact below this size
+      const float inv = 1.0f / (float)bw;
       const auto *P = Pix<T>::ptr(def.pix);
 #pragma unroll 1
       for (int k = grp.rank(); k < tot; k += G::kSize) {
-        const int r = k / bw;
+        const int r = fdiv ? (int)(((float)k + 0.5f) * inv) : k / bw;
         const int xx = min_x + (k - r * bw), yy = min_y + r;
         const int lx = xx - ox, ly = yy - oy;
         if (lx < 1 || lx >= iw - 1 || ly < 1 || ly >= ih - 1) {
           const float p = img_at(def, lx, ly);
           acc[0] += (double)sqrtf(p * p);
         } else {
-          const auto *c0 = P + (ly - 1) * iw + lx - 1;
-          const auto *c1 = c0 + iw;
-          const auto *c2 = c1 + iw;
-          const float conv = (float)c0[0] - 2.0f * (float)c0[1] + (float)c0[2] - 2.0f * (float)c1[0] +
-                             4.0f * (float)c1[1] - 2.0f * (float)c1[2] + (float)c2[0] - 2.0f * (float)c2[1] +
-                             (float)c2[2];
+          float t[9];
+          if (in_lds) {
+            const lds_float *c0 = win + (__umul24(ly - 1 - wy0, wpitch) + (lx - 1 - wx0));
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) t[3 * m + q] = c0[m * wpitch + q];
+          } else {
+            const auto *c0 = P + (ly - 1) * iw + lx - 1;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) t[3 * m + q] = (float)c0[m * iw + q];
+          }
+          const float conv = t[0] - 2.0f * t[1] + t[2] - 2.0f * t[3] + 4.0f * t[4] - 2.0f * t[5] + t[6] -
+                             2.0f * t[7] + t[8];
           acc[0] += (double)fabsf(conv);
         }
       }
@@ -1783,7 +1829,9 @@ struct Solver {
     init_params(A, fields, nsub, gid);
     const int init_status = INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
     int iters = -1;
-    if (!SIMPLEX_OPT) {
+    if constexpr (!SIMPLEX_OPT && kFast) {
+      precompute_phi();  // and meanF
+    } else if (!SIMPLEX_OPT) {
       // mean of the reference subset over defined-active pixels (DICe_Objective.cpp:464)
       double acc[2] = {0.0, 0.0};
       for_pixels([&](int x, int y, bool a) {
@@ -1793,7 +1841,6 @@ struct Solver {
       });
       grp.sum(acc);
       meanF = acc[1] > 0.0 ? (float)(acc[0] / acc[1]) : 0.0f;
-      if constexpr (kFast) precompute_phi();
     }
     if (pass == 0 && kp.init_gamma_thr != -1.0) {
       int act;

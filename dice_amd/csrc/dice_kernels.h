@@ -79,8 +79,9 @@ constexpr int kGnRedStride = kWave + 1;
 constexpr int kGnTotOffset = kGnSums * kGnRedStride + 1;
 constexpr int kGnTotQ = 0, kGnTotH = 8, kGnTotR = 44;
 // Per-subset constants of the BATCHED fast Gauss-Newton pass at the end of a
-// wave's slot: the 6 x 6 matrix sum(phi phi^T) and sum(phi), 42 doubles.
-constexpr int kPersistFloats = 84;
+// wave's slot: the 6 x 6 matrix sum(phi phi^T), sum(phi), sum(F) and the pixel
+// count, 44 doubles.
+constexpr int kPersistFloats = 88;
 constexpr int kMinSlotFloats = kGnTotOffset + 2 * 52 + kPersistFloats;  // smallest LDS slot per wave
 
 struct Launch {
