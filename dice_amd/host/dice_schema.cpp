@@ -304,3 +304,45 @@ double &Schema::local_field_value(int local_id, Field_Spec fs) {
 }
 
 }  // namespace dice_hip
+
+extern "C" {
+
+int dice_polygon_pixel_counts(int n_polygons, const int *vert_offsets, const int *vx, const int *vy,
+                              int *pixel_counts) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
+  for (int i = 0; i < n_polygons; ++i) {
+    const int b = vert_offsets[i], e = vert_offsets[i + 1];
+    if (e - b < 3) {
+      ++bad;
+      pixel_counts[i] = 0;
+      continue;
+    }
+    const dice_hip::Polygon p(std::vector<int>(vx + b, vx + e), std::vector<int>(vy + b, vy + e));
+    pixel_counts[i] = (int)p.get_owned_pixels().size();
+  }
+  return bad ? -1 : 0;
+}
+
+int dice_polygon_pixels(int n_polygons, const int *vert_offsets, const int *vx, const int *vy,
+                        const int64_t *pixel_offsets, int *px, int *py) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
+  for (int i = 0; i < n_polygons; ++i) {
+    const int b = vert_offsets[i], e = vert_offsets[i + 1];
+    if (e - b < 3) {
+      ++bad;
+      continue;
+    }
+    const dice_hip::Polygon p(std::vector<int>(vx + b, vx + e), std::vector<int>(vy + b, vy + e));
+    int64_t k = pixel_offsets[i];
+    for (const auto &yx : p.get_owned_pixels()) {
+      px[k] = yx.second;
+      py[k] = yx.first;
+      ++k;
+    }
+  }
+  return bad ? -1 : 0;
+}
+
+}  // extern "C"

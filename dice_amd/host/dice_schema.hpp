@@ -169,3 +169,15 @@ class Schema {
 };
 
 }  // namespace dice_hip
+
+extern "C" {
+// Batch rasterisation of polygons by Polygon::get_owned_pixels (OpenMP over
+// polygons).  Polygon i has the vertices [vert_offsets[i], vert_offsets[i+1]).
+// pixel_counts: owned pixels of each polygon.  pixels: fills x / y of polygon i
+// at [pixel_offsets[i], pixel_offsets[i+1]) in (y, x) order.  Return 0, or -1 on
+// a polygon with fewer than three vertices.
+int dice_polygon_pixel_counts(int n_polygons, const int *vert_offsets, const int *vx, const int *vy,
+                              int *pixel_counts);
+int dice_polygon_pixels(int n_polygons, const int *vert_offsets, const int *vx, const int *vy,
+                        const int64_t *pixel_offsets, int *px, int *py);
+}
