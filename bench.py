@@ -1,17 +1,33 @@
-"""Benchmark: converged subsets/s of the subset-correlation path on one MI355X.
+"""Benchmark: the subset-correlation path on one MI355X.  One JSON line (rank 0) per run.
 
-Workload (the headline configuration): 2048 x 2048 float32 speckle images
-(0.5 blobs/px^2, radius 2-4 px, 1 % noise), a 10-frame sequence with the
+--workload c2 (default, the headline configuration): 2048 x 2048 float32 speckle
+images (0.5 blobs/px^2, radius 2-4 px, 1 % noise), a 10-frame sequence with the
 sinusoidal field u = A sin(2 pi X / 256), v = 0.5 A cos(2 pi Y / 256),
 A = 0.1 .. 1.0; 41 x 41 subsets at step 5 (155,236 subsets); 6-parameter
 affine Gauss-Newton with Keys 4th-order interpolation; frame k starts from
 frame k-1's fields (USE_FIELD_VALUES).  One step = the whole 10-frame
-sequence, replayed from a hipGraph, with every input resident in HBM.
+sequence, replayed from a hipGraph, with every input resident in HBM.  Also
+measured in the same run: the plain one-subset-per-workgroup kernel on the same
+sequence, the per-launch duration of the batched correlation kernel from HIP
+events (for the roofline), and the CPU oracle on a bounded sample of frame 1.
 
-Printed: one JSON line (rank 0).  Also measured in the same run: the plain
-one-subset-per-workgroup kernel on the same sequence, the per-launch duration
-of the batched correlation kernel from HIP events recorded inside the graph
-(for the roofline), and the CPU oracle on a bounded sample of frame 1.
+--workload c3 (uneven batching): 1024^2 float32 speckle, 100 frames of smooth
+random rigid + affine motion (|t| <= 3 px, |theta| <= 2 deg, strains <= 1e-2),
+50,000 convex polygon subsets of 200-3000 pixels, every frame started from the
+true motion plus U(+-2 px) on u and v (so iteration counts spread), Gauss-Newton
+with max_solver_iterations_fast = 40, Keys-4, launch shape autotuned on frame 1.
+One step = the 100 frames, one persistent batched launch per frame.
+
+--workload c4 (differentiation): 1536^2 float32 speckle under a 12-coefficient
+quadratic field (peak 1.5 px), 21 x 21 subsets at step 6 (62,500 subsets).  One
+step = the ZNSSD gamma of every subset at its shape parameters (the field's own
+affine parameters perturbed as in a fit iteration) plus the adjoint: dL/dp for
+the 6 parameters of every subset and dL/dI_def into a zeroed 1536^2 image.
+
+--workload c5 (gradient-free): 4096 x 3000 12-bit uint16 low-contrast texture
+under translation + shear up to 2 px, 9 x 9 subsets at step 4 (760,920
+subsets), Nelder-Mead over the 6 affine parameters from a zero initial guess,
+200-iteration cap, bilinear interpolation.  One step = one frame pair.
 """
 import argparse
 import json
@@ -27,55 +43,105 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
-def algorithmic_bytes(iters, status, subset_size, storage_bytes=4):
-    """Bytes per subset: 12 N (reference intensity + 2 gradients, once) + n_eval * 4 (S+5)^2
-    (deformed Keys window per warp) + 64 (outputs); n_eval = GN passes + final gamma."""
-    n = subset_size * subset_size
-    passes = np.where(iters >= 0, iters + 1, 0)
-    n_eval = passes + np.where(iters >= 0, 1, 0)
-    return (12 * n + n_eval * storage_bytes * (subset_size + 5) ** 2 + 64).astype(np.float64)
+def algorithmic_bytes(iters, status, n_px, window_px, storage_bytes=4, evals=None):
+    """Bytes per subset of one correlation: 12 N (reference intensity + 2 gradients, once) + n_eval *
+    storage_bytes * window (the deformed interpolation window, gathered once per warp) + 64 (outputs).
+    Gauss-Newton: n_eval = GN passes + the final gamma.  evals overrides n_eval (Nelder-Mead)."""
+    if evals is None:
+        passes = np.where(iters >= 0, iters + 1, 0)
+        evals = passes + np.where(iters >= 0, 1, 0)
+    return (12 * n_px + evals * storage_bytes * window_px + 64).astype(np.float64)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=2048)
-    ap.add_argument("--frames", type=int, default=10)
-    ap.add_argument("--subset", type=int, default=41)
-    ap.add_argument("--step", type=int, default=5)
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run an elementwise copy kernel (y = x * 1) over MIB MiB, 5 launches: the known-size "
-                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters of a profile")
-    args = ap.parse_args()
-    if args.calibrate_copy:
-        import torch
-        n = args.calibrate_copy * (1 << 20) // 4
-        x = torch.arange(n, dtype=torch.float32, device="cuda:0")
-        y = torch.empty_like(x)
-        for _ in range(5):
-            torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
-        assert torch.equal(x, y)
-        print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 5}))
-        return 0
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: the correlation of one device's subset share is the "
-              "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)
-        return 2
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def csr_take(conf, idx):
+    """The conformal CSR (offsets, x, y, active) of the subsets idx."""
+    off, px, py, act = conf
+    cnt = np.diff(off)[idx]
+    o2 = np.zeros(len(idx) + 1, dtype=np.int32)
+    np.cumsum(cnt, out=o2[1:])
+    sel = np.concatenate([np.arange(off[i], off[i + 1]) for i in idx])
+    return o2, px[sel], py[sel], act[sel]
+
+
+def cpu_baseline(p, ref, defm, n, fields_of, seconds, subset_size=0, conformal=None, what=""):
+    """The CPU oracle (float build, OpenMP over subsets) on a bounded, evenly strided sample of the n
+    subsets: calibrated on a small slice, then sized for about `seconds` of CPU work.
+    fields_of(idx) gives the initial field store of the subsets idx."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as o
+    import dice_amd as dh
+    threads = cpu_threads()
+    q = o.default_params()
+    for name, _ in dh.Params._fields_:
+        setattr(q, name, getattr(p, name))
+
+    def run(idx):
+        f = fields_of(idx)
+        conf = csr_take(conformal, idx) if conformal is not None else None
+        t = time.perf_counter()
+        o.correlate(q, ref, defm, f, subset_size=subset_size, conformal=conf, precision="float", threads=threads)
+        return time.perf_counter() - t, f
+
+    log("cpu baseline: %d threads" % threads)
+    all_idx = np.arange(n)
+    probe = all_idx[:: max(1, n // (threads * 8))]
+    el, _ = run(probe)
+    per = el / len(probe)
+    m = int(min(n, max(len(probe), seconds / max(per, 1e-9))))
+    stride = max(1, n // m)
+    idx = all_idx[::stride]
+    el, f = run(idx)
+    conv = int(np.isin(f[o.F["STATUS"]], dh.CONVERGED_STATUSES).sum())
+    return {"value": round(conv / el, 1), "unit": "subsets/s", "cores": threads, "kind": "port",
+            "sample": "%s%d of %d subsets (every %d-th), float build of the oracle, OpenMP over subsets, %s"
+                      % (what, len(idx), n, stride, cpu_model())}
+
+
+def result_line(args, metric, value, unit, ms_per_step, dtype, config, roofline, **extra):
+    r = {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config, "roofline": roofline}
+    r.update(extra)
+    return r
+
+
+def roofline(bytes_per_launch, launch_ms, kernel, launches, traffic=None, traffic_src=None):
+    gbs = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": round(traffic) if traffic else None,
+            "traffic_source": traffic_src if traffic else None, "kernel": kernel,
+            "avg_launch_ms": round(launch_ms, 4), "launches_timed": launches,
+            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+
+
+def bench_c2(args):
     import torch
     import dice_amd as dh
     from dice_amd import synth
 
     t0 = time.time()
-    ref, frames = synth.sinusoid_sequence(size=args.size, frames=args.frames, seed=1)
+    ref, frames = synth.sinusoid_sequence(size=args.size, frames=args.frames or 10, seed=1)
     gen_s = time.time() - t0
+    log("c2 inputs generated in %.1f s" % gen_s)
     W = H = args.size
     S = args.subset
     cx, cy = dh.regular_grid(W, H, S, args.step)
@@ -134,6 +200,7 @@ def main():
 
     # batched: throughput timing without event nodes, then a pass with per-launch events
     el_b, _, _ = timed(dh.MODE_BATCHED, args.steps, args.warmup)
+    log("batched: %d steps in %.3f s" % (args.steps, el_b))
     hist = d_hist.cpu().numpy()
     el_e, kern_ms, kern_cnt = timed(dh.MODE_BATCHED, max(1, min(args.steps, 3)), 0, kernel_timing=True)
     status = hist[:, dh.F["STATUS"]]
@@ -170,42 +237,25 @@ def main():
                 traffic = float(line.split(":", 1)[1])
 
     # roofline of the batched correlation kernel: algorithmic bytes per launch / launch time
-    bytes_per_frame = algorithmic_bytes(iters, status, S).sum(axis=1)
+    bytes_per_frame = algorithmic_bytes(iters, status, S * S, (S + 5) ** 2).sum(axis=1)
     avg_bytes = float(bytes_per_frame.mean())
-    achieved_gbs = avg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
 
-    result = {
-        "metric": "converged subsets/sec at 2048^2 (batched persistent kernel, hipGraph over 10 frames)",
-        "value": round(value, 1),
-        "unit": "subsets/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (FP64 accumulation and 6x6 solve)",
-        "data": "synthetic",
-        "config": {"workload": "speckle %dx%d f32, %d frames sinusoidal A=0.1..%.1f px, %dx%d subsets step %d "
-                               "(%d subsets), 6-param affine GN, Keys-4"
-                               % (W, H, len(frames), 0.1 * len(frames), S, S, args.step, n),
-                   "subsets_per_frame": n, "frames": len(frames), "converged_per_sequence": converged_per_seq,
-                   "mean_iterations": round(float(iters[conv].mean()), 3),
-                   "mean_abs_u_err_px": round(float(du.mean()), 5),
-                   "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
-                   "mean_abs_v_err_px": round(float(dvv.mean()), 5),
-                   "mean_abs_u_err_vs_window_avg_px": round(float(du_w.mean()), 5),
-                   "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": round(traffic) if traffic else None,
-                     "traffic_source": "profiles/correlate_batched_c2_hbm_traffic.md" if traffic else None,
-                     "kernel": "correlate_batched_kernel (GN, Keys-4, f32)",
-                     "avg_launch_ms": round(kern_ms, 4), "launches_timed": kern_cnt,
-                     "algorithmic_bytes_per_launch": round(avg_bytes)},
-        "eager_ms_per_step": round(1000.0 * el_e / max(1, min(args.steps, 3)), 3),
-    }
+    result = result_line(
+        args, "converged subsets/sec at 2048^2 (batched persistent kernel, hipGraph over 10 frames)", value,
+        "subsets/s", ms_per_step, "f32 (FP64 accumulation and 6x6 solve)",
+        {"workload": "speckle %dx%d f32, %d frames sinusoidal A=0.1..%.1f px, %dx%d subsets step %d "
+                     "(%d subsets), 6-param affine GN, Keys-4"
+                     % (W, H, len(frames), 0.1 * len(frames), S, S, args.step, n),
+         "subsets_per_frame": n, "frames": len(frames), "converged_per_sequence": converged_per_seq,
+         "mean_iterations": round(float(iters[conv].mean()), 3),
+         "mean_abs_u_err_px": round(float(du.mean()), 5),
+         "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
+         "mean_abs_v_err_px": round(float(dvv.mean()), 5),
+         "mean_abs_u_err_vs_window_avg_px": round(float(du_w.mean()), 5),
+         "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
+        roofline(avg_bytes, kern_ms, "correlate_batched_kernel (GN, Keys-4, f32)", kern_cnt, traffic,
+                 "profiles/correlate_batched_c2_hbm_traffic.md"),
+        eager_ms_per_step=round(1000.0 * el_e / max(1, min(args.steps, 3)), 3))
 
     if not args.no_plain:
         el_p, _, _ = timed(dh.MODE_PLAIN, max(1, min(args.steps, 2)), 1)
@@ -218,41 +268,381 @@ def main():
                            "status_agreement_with_batched": round(float(np.mean(hp[:, dh.F["STATUS"]] == status)), 6)}
 
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as o
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        q = o.default_params()
-        for name, _ in dh.Params._fields_:
-            setattr(q, name, getattr(p, name))
-        # bounded sample of frame 1: calibrate on a small slice, then size the sample for ~cpu-seconds
-        rng_idx = np.arange(n)
-        probe = rng_idx[:: max(1, n // (threads * 8))]
-        f = o.new_fields(cx[probe], cy[probe])
-        t = time.perf_counter()
-        o.correlate(q, ref, frames[0], f, subset_size=S, precision="float", threads=threads)
-        per = (time.perf_counter() - t) / len(probe)
-        m = int(min(n, max(len(probe), args.cpu_seconds / max(per, 1e-9))))
-        idx = rng_idx[:: max(1, n // m)]
-        f = o.new_fields(cx[idx], cy[idx])
-        t = time.perf_counter()
-        o.correlate(q, ref, frames[0], f, subset_size=S, precision="float", threads=threads)
-        el_c = time.perf_counter() - t
-        conv_c = int(np.isin(f[o.F["STATUS"]], dh.CONVERGED_STATUSES).sum())
-        cpu_model = ""
-        try:
-            for line in open("/proc/cpuinfo"):
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-        except OSError:
-            pass
-        result["cpu_baseline"] = {"value": round(conv_c / el_c, 1), "unit": "subsets/s", "cores": threads,
-                                  "kind": "port",
-                                  "sample": "frame 1 (A=0.1), %d of %d subsets (every %d-th), float build of the "
-                                            "oracle, OpenMP over subsets, %s" % (len(idx), n, max(1, n // m),
-                                                                                 cpu_model)}
-    print(json.dumps(result))
+        result["cpu_baseline"] = cpu_baseline(p, ref, frames[0], n, lambda idx: dh.new_fields(cx[idx], cy[idx]),
+                                              args.cpu_seconds, subset_size=S, what="frame 1 (A=0.1), ")
     ctx.close()
+    return result
+
+
+def bench_c3(args):
+    import torch
+    import dice_amd as dh
+    from dice_amd import synth
+
+    frames = args.frames or 100
+    n_req = args.subsets or 50000
+    size = 1024
+    t0 = time.time()
+    ref, seq, motion, subs = synth.config3(n_subsets=n_req, frames=frames, size=size)
+    cx, cy, off, px, py, act = subs
+    conf = (off, px, py, act)
+    gen_s = time.time() - t0
+    log("c3 inputs generated in %.1f s" % gen_s)
+    n = len(cx)
+    cnt = np.diff(off).astype(np.float64)
+    # staged window per subset: bounding box + the Keys margin
+    bw = np.array([px[off[i]:off[i + 1]].max() - px[off[i]:off[i + 1]].min() + 1 for i in range(n)])
+    bh = np.array([py[off[i]:off[i + 1]].max() - py[off[i]:off[i + 1]].min() + 1 for i in range(n)])
+    window = (bw + 5.0) * (bh + 5.0)
+
+    # true motion at the centres, and the perturbed initial guesses of every frame
+    c = (size - 1) / 2.0
+    rng = np.random.default_rng(21)
+    truth = np.zeros((frames, 3, n))
+    init = np.zeros((frames, dh.NUM_FIELDS, n))
+    for k, (F, t) in enumerate(motion):
+        d = F - np.eye(2)
+        truth[k, 0] = d[0, 0] * (cx - c) + d[0, 1] * (cy - c) + t[0]
+        truth[k, 1] = d[1, 0] * (cx - c) + d[1, 1] * (cy - c) + t[1]
+        truth[k, 2] = synth.polar_params(F)[0]
+        f = dh.new_fields(cx, cy)
+        f[dh.F["DISP_X"]] = truth[k, 0] + rng.uniform(-2.0, 2.0, n)
+        f[dh.F["DISP_Y"]] = truth[k, 1] + rng.uniform(-2.0, 2.0, n)
+        init[k] = f
+
+    p = dh.default_params()
+    p.optimization_method = dh.GRADIENT_BASED
+    p.interpolation_method = dh.KEYS_FOURTH
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    p.max_solver_iterations_fast = 40
+
+    dev = torch.device("cuda:0")
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(ref)
+    ctx.set_subsets(cx, cy, 0, conformal=conf)
+    d_frames = [torch.from_numpy(f).to(dev) for f in seq]
+    ptrs = [t.data_ptr() for t in d_frames]
+    d_init = torch.from_numpy(init).to(dev)
+    d_fields = torch.empty_like(d_init[0])
+    d_hist = torch.zeros_like(d_init)
+    torch.cuda.synchronize()
+    ctx.set_def_image_device(ptrs[0], size, size, dh.STORAGE_F32)
+    default_shape = ctx.launch_shape()
+    if not args.no_autotune:
+        ctx.autotune(d_init[0].data_ptr(), first_frame=False)
+    stream = torch.cuda.ExternalStream(ctx.stream())
+
+    def run_sequence(mode):
+        for k in range(frames):
+            with torch.cuda.stream(stream):
+                d_fields.copy_(d_init[k])
+            ctx.set_def_image_device(ptrs[k], size, size, dh.STORAGE_F32)
+            ctx.correlate_device(d_fields.data_ptr(), mode=mode, first_frame=False)
+            with torch.cuda.stream(stream):
+                d_hist[k].copy_(d_fields)
+
+    def timed(mode, steps, warmup, kernel_timing=False):
+        ctx.set_kernel_timing(kernel_timing)
+        for _ in range(warmup):
+            run_sequence(mode)
+        ctx.synchronize()
+        if kernel_timing:
+            ctx.kernel_timing()
+        t = time.perf_counter()
+        ksum, kcnt = 0.0, 0
+        for _ in range(steps):
+            run_sequence(mode)
+            if kernel_timing:
+                ctx.synchronize()
+                cc, ms = ctx.kernel_timing()
+                ksum += ms * cc
+                kcnt += cc
+        ctx.synchronize()
+        el = time.perf_counter() - t
+        ctx.set_kernel_timing(False)
+        return el, (ksum / kcnt if kcnt else 0.0), kcnt
+
+    el_b, _, _ = timed(dh.MODE_BATCHED, args.steps, args.warmup)
+    log("batched: %d steps in %.3f s" % (args.steps, el_b))
+    hist = d_hist.cpu().numpy()
+    _, kern_ms, kern_cnt = timed(dh.MODE_BATCHED, 1, 0, kernel_timing=True)
+    status = hist[:, dh.F["STATUS"]]
+    iters = hist[:, dh.F["ITERATIONS"]]
+    conv = np.isin(status, dh.CONVERGED_STATUSES)
+    converged_per_seq = int(conv.sum())
+    value = converged_per_seq * args.steps / el_b
+    ms_per_step = 1000.0 * el_b / args.steps
+    du = np.abs(hist[:, dh.F["DISP_X"]] - truth[:, 0])[conv]
+    dv = np.abs(hist[:, dh.F["DISP_Y"]] - truth[:, 1])[conv]
+    dth = np.abs(hist[:, dh.F["ROT_Z"]] - truth[:, 2])[conv]
+    it_c = iters[conv]
+    bytes_per_frame = algorithmic_bytes(iters, status, cnt[None, :], window[None, :]).sum(axis=1)
+
+    result = result_line(
+        args, "converged subsets/sec, 50k conformal subsets on 1024^2 (batched persistent kernel, one launch per "
+              "frame)", value, "subsets/s", ms_per_step, "f32 (FP64 accumulation and 6x6 solve)",
+        {"workload": "speckle %dx%d f32, %d frames random smooth rigid+affine motion, %d conformal polygon subsets "
+                     "(%d-%d px, mean %.0f), init = truth + U(+-2 px), 6-param affine GN (<= 40 iterations), Keys-4"
+                     % (size, size, frames, n, cnt.min(), cnt.max(), cnt.mean()),
+         "subsets_per_frame": n, "frames": frames, "converged_per_sequence": converged_per_seq,
+         "converged_fraction": round(converged_per_seq / (n * frames), 5),
+         "iterations_p5_p50_p95_max": [int(np.percentile(it_c, q)) for q in (5, 50, 95)] + [int(it_c.max())],
+         "mean_abs_u_err_px": round(float(du.mean()), 5), "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
+         "mean_abs_v_err_px": round(float(dv.mean()), 5),
+         "mean_abs_theta_err_rad": round(float(dth.mean()), 7),
+         "launch_shape": ctx.launch_shape(), "default_launch_shape": default_shape,
+         "input_generation_s": round(gen_s, 1)},
+        roofline(float(bytes_per_frame.mean()), kern_ms,
+                 "correlation pass (correlate_batched_kernel + plain overflow), GN, Keys-4, f32", kern_cnt))
+
+    if not args.no_plain:
+        ps = max(1, min(args.steps, 1))
+        el_p, _, _ = timed(dh.MODE_PLAIN, ps, 0)
+        hp = d_hist.cpu().numpy()
+        conv_p = int(np.isin(hp[:, dh.F["STATUS"]], dh.CONVERGED_STATUSES).sum())
+        plain_rate = conv_p * ps / el_p
+        result["plain"] = {"value": round(plain_rate, 1), "unit": "subsets/s", "ms_per_step": round(1000 * el_p / ps, 3),
+                           "batched_over_plain": round(value / plain_rate, 3),
+                           "status_agreement_with_batched": round(float(np.mean(hp[:, dh.F["STATUS"]] == status)), 6)}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(p, ref, seq[0], n, lambda idx: np.ascontiguousarray(init[0][:, idx]),
+                                              args.cpu_seconds, conformal=conf, what="frame 1, ")
+    ctx.close()
+    return result
+
+
+def bench_c4(args):
+    import torch
+    import dice_amd as dh
+    from dice_amd import synth
+
+    size, S, step = 1536, 21, 6
+    t0 = time.time()
+    ref, de, field = synth.config4(size=size, peak=1.5, seed=1)
+    gen_s = time.time() - t0
+    log("c4 inputs generated in %.1f s" % gen_s)
+    cx, cy = dh.regular_grid(size, size, S, step)
+    n = len(cx)
+    u, v, ux, uy, vx, vy = field(cx, cy)
+    F = np.stack([np.stack([1 + ux, uy], -1), np.stack([vx, 1 + vy], -1)], -2)
+    th, ex, ey, g = synth.polar_params(F)
+    P = np.stack([u, v, th, ex, ey, g], axis=1)
+    rng = np.random.default_rng(7)
+    P += rng.normal(0, 1, P.shape) * np.array([0.3, 0.3, 0.01, 0.01, 0.01, 0.01])  # a fit iterate
+
+    p = dh.default_params()
+    p.interpolation_method = dh.KEYS_FOURTH
+    p.compute_def_gradients = 0
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(ref)
+    ctx.set_def_image(de)
+    ctx.set_subsets(cx, cy, S)
+    dev = torch.device("cuda:0")
+    d_p = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    d_g = torch.empty(n, dtype=torch.float64, device=dev)
+    d_w = torch.ones(n, dtype=torch.float64, device=dev)
+    d_dp = torch.empty_like(d_p)
+    d_img = torch.zeros((size, size), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+
+    def step_once(timing=False):
+        with torch.cuda.stream(stream):
+            d_img.zero_()
+            if timing:
+                ev[0].record(stream)
+        ctx.gamma(d_p.data_ptr(), d_g.data_ptr(), dh.MODE_BATCHED)
+        if timing:
+            ev[1].record(stream)
+        ctx.gamma_adjoint(d_p.data_ptr(), d_w.data_ptr(), d_dp.data_ptr(), d_img.data_ptr(), dh.MODE_BATCHED)
+        if timing:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step_once()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step_once()
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    log("forward + adjoint: %d steps in %.3f s" % (args.steps, el))
+    fwd, adj = [], []
+    for _ in range(max(3, args.steps)):
+        step_once(timing=True)
+        ctx.synchronize()
+        fwd.append(ev[0].elapsed_time(ev[1]))
+        adj.append(ev[1].elapsed_time(ev[2]))
+    fwd_ms, adj_ms = float(np.mean(fwd)), float(np.mean(adj))
+    g = d_g.cpu().numpy()
+    dp = d_dp.cpu().numpy()
+    img = d_img.cpu().numpy()
+
+    # forward: ref subset 4N + window 4 (S+5)^2 + params 48 + gamma 8; adjoint: the same reads plus dgamma
+    # 8, dparams 48, and the dI image read and written once (2 x 4 W H per launch)
+    fwd_bytes = n * (4 * S * S + 4 * (S + 5) ** 2 + 56.0)
+    adj_bytes = n * (4 * S * S + 4 * (S + 5) ** 2 + 112.0) + 8.0 * size * size
+    result = result_line(
+        args, "forward + adjoint subsets/sec, 62,500 21x21 subsets on 1536^2 (gamma, dL/dp, dL/dI_def)",
+        n * args.steps / el, "subsets/s", 1000.0 * el / args.steps, "f32 (FP64 gamma and dL/dp accumulation)",
+        {"workload": "speckle %dx%d f32, quadratic field (12 coefficients, peak 1.5 px), %dx%d subsets step %d "
+                     "(%d subsets), Keys-4; gamma + adjoint (6 params + image) per step" % (size, size, S, S, step, n),
+         "subsets": n, "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+         "forward_roofline_frac": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+         "mean_gamma": float(g.mean()), "dparams_finite": bool(np.isfinite(dp).all()),
+         "dimage_nonzero_px": int((img != 0).sum()), "input_generation_s": round(gen_s, 1)},
+        roofline(adj_bytes, adj_ms, "gamma_batched_kernel (adjoint: dL/dp + dL/dI_def, Keys-4)", len(adj)),
+        cpu_baseline={"value": None, "unit": "subsets/s", "cores": 0, "kind": "port",
+                      "sample": "none: the reference has no adjoint of its objective (its CPU path differentiates "
+                                "only inside computeUpdateFast's Gauss-Newton step)"})
+    ctx.close()
+    return result
+
+
+def bench_c5(args):
+    import torch
+    import dice_amd as dh
+    from dice_amd import synth
+
+    W, H, S, step = 4096, 3000, 9, 4
+    t0 = time.time()
+    ref, de, (F, t) = synth.config5(W, H)
+    gen_s = time.time() - t0
+    log("c5 inputs generated in %.1f s" % gen_s)
+    cx, cy = dh.regular_grid(W, H, S, step)
+    n = len(cx)
+    c = np.array([(W - 1) / 2.0, (H - 1) / 2.0])
+    d = F - np.eye(2)
+    ut = d[0, 0] * (cx - c[0]) + d[0, 1] * (cy - c[1]) + t[0]
+    vt = d[1, 0] * (cx - c[0]) + d[1, 1] * (cy - c[1]) + t[1]
+
+    p = dh.default_params()
+    p.optimization_method = dh.SIMPLEX
+    p.interpolation_method = dh.BILINEAR
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    p.max_solver_iterations_robust = 200
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(ref)
+    ctx.set_subsets(cx, cy, S)
+    dev = torch.device("cuda:0")
+    d_def = torch.from_numpy(de).to(dev)
+    init = torch.from_numpy(dh.new_fields(cx, cy)).to(dev)
+    d_fields = init.clone()
+    torch.cuda.synchronize()
+    ctx.set_def_image_device(d_def.data_ptr(), W, H, dh.STORAGE_U16)
+    if args.autotune:
+        ctx.autotune(init.data_ptr())
+    stream = torch.cuda.ExternalStream(ctx.stream())
+
+    def run(mode):
+        with torch.cuda.stream(stream):
+            d_fields.copy_(init)
+        ctx.correlate_device(d_fields.data_ptr(), mode=mode, first_frame=True)
+
+    def timed(mode, steps, warmup, kernel_timing=False):
+        ctx.set_kernel_timing(kernel_timing)
+        for _ in range(warmup):
+            run(mode)
+        ctx.synchronize()
+        if kernel_timing:
+            ctx.kernel_timing()
+        tt = time.perf_counter()
+        ksum, kcnt = 0.0, 0
+        for _ in range(steps):
+            run(mode)
+            if kernel_timing:
+                ctx.synchronize()
+                cc, ms = ctx.kernel_timing()
+                ksum += ms * cc
+                kcnt += cc
+        ctx.synchronize()
+        el = time.perf_counter() - tt
+        ctx.set_kernel_timing(False)
+        return el, (ksum / kcnt if kcnt else 0.0), kcnt
+
+    el_b, _, _ = timed(dh.MODE_BATCHED, args.steps, args.warmup)
+    log("batched: %d steps in %.3f s" % (args.steps, el_b))
+    f = d_fields.cpu().numpy()
+    _, kern_ms, kern_cnt = timed(dh.MODE_BATCHED, max(1, min(args.steps, 3)), 0, kernel_timing=True)
+    status = f[dh.F["STATUS"]]
+    iters = f[dh.F["ITERATIONS"]]
+    conv = np.isin(status, dh.CONVERGED_STATUSES)
+    value = int(conv.sum()) * args.steps / el_b
+    du = np.abs(f[dh.F["DISP_X"]] - ut)[conv]
+    dv = np.abs(f[dh.F["DISP_Y"]] - vt)[conv]
+    # Nelder-Mead evaluations: 7 vertices + at least 2 per iteration (the trial point and the
+    # end-of-iteration evaluation); uint16 storage, bilinear window (S+3)^2
+    evals = np.where(iters >= 0, 7 + 2 * np.maximum(iters, 0), 0)
+    bytes_launch = float(algorithmic_bytes(iters, status, S * S, (S + 3) ** 2, storage_bytes=2, evals=evals).sum()
+                         - 10.0 * S * S * n)  # reference: 2 N (uint16 intensities, no gradients)
+    result = result_line(
+        args, "converged subsets/sec, 760,920 9x9 uint16 subsets on 4096x3000 (batched Nelder-Mead, bilinear)",
+        value, "subsets/s", 1000.0 * el_b / args.steps, "f32 (uint16 storage, FP64 gamma)",
+        {"workload": "12-bit uint16 texture %dx%d (20 sinusoids, 5%% contrast, 0.5%% noise), translation + shear "
+                     "<= 2 px, %dx%d subsets step %d (%d subsets), Nelder-Mead over 6 affine params (<= 200 "
+                     "iterations), bilinear, zero initial guess" % (W, H, S, S, step, n),
+         "subsets": n, "converged": int(conv.sum()), "converged_fraction": round(float(conv.mean()), 5),
+         "iterations_p50_p95_max": [int(np.percentile(iters[conv], q)) for q in (50, 95)] + [int(iters.max())],
+         "mean_abs_u_err_px": round(float(du.mean()), 5), "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
+         "mean_abs_v_err_px": round(float(dv.mean()), 5),
+         "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
+        roofline(bytes_launch, kern_ms, "correlation pass (correlate_batched_kernel, Nelder-Mead, bilinear, u16)",
+                 kern_cnt))
+    if not args.no_plain:
+        el_p, _, _ = timed(dh.MODE_PLAIN, 1, 0)
+        fp = d_fields.cpu().numpy()
+        conv_p = int(np.isin(fp[dh.F["STATUS"]], dh.CONVERGED_STATUSES).sum())
+        result["plain"] = {"value": round(conv_p / el_p, 1), "unit": "subsets/s", "ms_per_step": round(1000 * el_p, 3),
+                           "batched_over_plain": round(value / (conv_p / el_p), 3),
+                           "status_agreement_with_batched": round(float(np.mean(fp[dh.F["STATUS"]] == status)), 6)}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(p, ref, de, n, lambda idx: dh.new_fields(cx[idx], cy[idx]),
+                                              args.cpu_seconds, subset_size=S)
+    ctx.close()
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--size", type=int, default=2048, help="c2: image side")
+    ap.add_argument("--frames", type=int, default=0, help="c2: 10, c3: 100 when 0")
+    ap.add_argument("--subsets", type=int, default=0, help="c3: 50000 when 0")
+    ap.add_argument("--subset", type=int, default=41, help="c2: subset side")
+    ap.add_argument("--step", type=int, default=5, help="c2: subset step")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="c2, c5: autotune the launch shape first")
+    ap.add_argument("--no-autotune", action="store_true", help="c3: keep the default launch shape")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise copy kernel (y = x * 1) over MIB MiB, 5 launches: the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters of a profile")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.arange(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        assert torch.equal(x, y)
+        print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 5}))
+        return 0
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the correlation of one device's subset share is the "
+              "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+    run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+    print(json.dumps(run(args)), flush=True)
     return 0
 
 

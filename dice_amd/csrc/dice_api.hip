@@ -82,6 +82,9 @@ struct dice_hip_context {
   bool def_has_grads = false;
   // subsets
   int n = 0, S = 0, fov = 0;
+  // largest and 90th-percentile bounding-box side of the subsets (S for square subsets): the
+  // autotuner sizes its LDS window candidates from them
+  int max_side = 0, p90_side = 0;
   DevBuf cx, cy, off, px, py, act, bbox, nact;
   // fields + work
   DevBuf fields, work;
@@ -449,12 +452,13 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
   ctx->fov = subset_size > 0 ? subset_size : 0;
   if (!conf_offsets) {
     ctx->S = subset_size;
+    ctx->max_side = ctx->p90_side = subset_size;
   } else {
     if (!conf_x || !conf_y) return ctx->fail(DICE_HIP_EINVAL, "conformal subsets need pixel lists");
     const int total = conf_offsets[n];
     if (conf_offsets[0] != 0 || total < 1) return ctx->fail(DICE_HIP_EINVAL, "bad conformal offsets");
     std::vector<int4> bb(n);
-    std::vector<int> na(n);
+    std::vector<int> na(n), sides(n);
     for (int i = 0; i < n; ++i) {
       const int b = conf_offsets[i], e = conf_offsets[i + 1];
       if (e <= b) return ctx->fail(DICE_HIP_EINVAL, "empty conformal subset");
@@ -470,7 +474,12 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
       }
       bb[i] = box;
       na[i] = cnt;
+      sides[i] = std::max(box.z - box.x, box.w - box.y) + 1;
     }
+    auto p90 = sides.begin() + (size_t)(0.9 * (n - 1));
+    std::nth_element(sides.begin(), p90, sides.end());
+    ctx->p90_side = *p90;
+    ctx->max_side = *std::max_element(sides.begin(), sides.end());
     HIP_TRY(ctx, ctx->off.ensure((n + 1) * sizeof(int)), "hipMalloc subsets");
     HIP_TRY(ctx, ctx->px.ensure(total * sizeof(int)), "hipMalloc subsets");
     HIP_TRY(ctx, ctx->py.ensure(total * sizeof(int)), "hipMalloc subsets");
@@ -659,8 +668,8 @@ int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int fi
   int rc = check_ready(ctx);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
-  const auto key = std::make_tuple(ctx->n, ctx->S, ctx->params.interpolation_method, ctx->params.optimization_method,
-                                   ctx->rw * 65536 + ctx->rh);
+  const auto key = std::make_tuple(ctx->n, ctx->S > 0 ? ctx->S : -ctx->max_side, ctx->params.interpolation_method,
+                                   ctx->params.optimization_method, ctx->rw * 65536 + ctx->rh);
   auto it = ctx->tuned.find(key);
   if (it != ctx->tuned.end()) {
     ctx->batched = it->second;
@@ -678,12 +687,17 @@ int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int fi
   ctx->timing = false;
   Launch best = ctx->batched;
   float best_ms = 1e30f;
-  // candidate LDS window budgets (floats per wave) cover the warped subset with margin
-  const int side = ctx->S > 0 ? ctx->S : 48;
+  // candidate LDS window budgets (floats per wave) cover the warped subset with margin: the
+  // largest subset, and for uneven subsets also the 90th percentile (the rest overflow to
+  // the plain kernel)
   std::vector<int> windows;
-  for (int extra : {8, 16, 28}) {
-    const int w = std::max(kDefaultSlotFloats, ((side + extra) * (side + extra) + 3) & ~3);
-    if ((size_t)w * 4 <= kMaxLdsBytes) windows.push_back(w);
+  for (int side : {ctx->max_side, ctx->p90_side}) {
+    for (int extra : {8, 16, 28}) {
+      const int w = std::max(kDefaultSlotFloats, ((side + extra) * (side + extra) + 3) & ~3);
+      if ((size_t)w * 4 <= kMaxLdsBytes && std::find(windows.begin(), windows.end(), w) == windows.end())
+        windows.push_back(w);
+    }
+    if (ctx->p90_side == ctx->max_side) break;
   }
   if (windows.empty()) windows.push_back(kDefaultSlotFloats);
   for (int waves : {1, 2, 4})
