@@ -477,6 +477,15 @@ def bench_c4(args):
         fwd.append(ev[0].elapsed_time(ev[1]))
         adj.append(ev[1].elapsed_time(ev[2]))
     fwd_ms, adj_ms = float(np.mean(fwd)), float(np.mean(adj))
+    # the adjoint without dL/dI_def: the parameter gradients alone
+    adj_p = []
+    for _ in range(3):
+        with torch.cuda.stream(stream):
+            ev[0].record(stream)
+        ctx.gamma_adjoint(d_p.data_ptr(), d_w.data_ptr(), d_dp.data_ptr(), 0, dh.MODE_BATCHED)
+        ev[1].record(stream)
+        ctx.synchronize()
+        adj_p.append(ev[0].elapsed_time(ev[1]))
     g = d_g.cpu().numpy()
     dp = d_dp.cpu().numpy()
     img = d_img.cpu().numpy()
@@ -491,6 +500,7 @@ def bench_c4(args):
         {"workload": "speckle %dx%d f32, quadratic field (12 coefficients, peak 1.5 px), %dx%d subsets step %d "
                      "(%d subsets), Keys-4; gamma + adjoint (6 params + image) per step" % (size, size, S, S, step, n),
          "subsets": n, "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+         "adjoint_params_only_ms": round(float(np.mean(adj_p)), 4),
          "forward_roofline_frac": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
          "mean_gamma": float(g.mean()), "dparams_finite": bool(np.isfinite(dp).all()),
          "dimage_nonzero_px": int((img != 0).sum()), "input_generation_s": round(gen_s, 1)},

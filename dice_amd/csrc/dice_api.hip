@@ -85,6 +85,7 @@ struct dice_hip_context {
   // largest and 90th-percentile bounding-box side of the subsets (S for square subsets): the
   // autotuner sizes its LDS window candidates from them
   int max_side = 0, p90_side = 0;
+  int max_count = 0;  // most pixels in one subset
   DevBuf cx, cy, off, px, py, act, bbox, nact;
   // fields + work
   DevBuf fields, work;
@@ -217,6 +218,7 @@ Launch current_launch(dice_hip_context *ctx, int mode) {
   }
   const int max_groups = (ctx->n + L.waves_per_group - 1) / L.waves_per_group;
   if (L.groups > max_groups) L.groups = max_groups > 0 ? max_groups : 1;
+  L.lane_simplex = ctx->max_count <= dice_hip::kLaneSimplexMaxPixels ? 1 : 0;
   return L;
 }
 
@@ -453,6 +455,7 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
   if (!conf_offsets) {
     ctx->S = subset_size;
     ctx->max_side = ctx->p90_side = subset_size;
+    ctx->max_count = (2 * (subset_size / 2) + 1) * (2 * (subset_size / 2) + 1);
   } else {
     if (!conf_x || !conf_y) return ctx->fail(DICE_HIP_EINVAL, "conformal subsets need pixel lists");
     const int total = conf_offsets[n];
@@ -480,6 +483,8 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     std::nth_element(sides.begin(), p90, sides.end());
     ctx->p90_side = *p90;
     ctx->max_side = *std::max_element(sides.begin(), sides.end());
+    ctx->max_count = 0;
+    for (int i = 0; i < n; ++i) ctx->max_count = std::max(ctx->max_count, conf_offsets[i + 1] - conf_offsets[i]);
     HIP_TRY(ctx, ctx->off.ensure((n + 1) * sizeof(int)), "hipMalloc subsets");
     HIP_TRY(ctx, ctx->px.ensure(total * sizeof(int)), "hipMalloc subsets");
     HIP_TRY(ctx, ctx->py.ensure(total * sizeof(int)), "hipMalloc subsets");

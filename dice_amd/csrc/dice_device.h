@@ -988,6 +988,10 @@ struct Solver {
       acc[5] += d * r;
     });
     grp.sum(acc);
+    return gamma_of(acc, active, contrast);
+  }
+  // ZNSSD gamma from the FP64 sums (n, sd, sdd, sr, srr, sdr) of the kept pixels.
+  __device__ __forceinline__ float gamma_of(const double (&acc)[6], int &active, float &contrast) const {
     const double cnt = acc[0];
     active = (int)cnt;
     if (cnt <= 0.0) return -1.0f;
@@ -1512,6 +1516,7 @@ struct Solver {
     }
     const float tiny = 1.0e-10f;
     float old_rtol = 0.0f, gamma_new = 0.0f, gamma_old = -1.0f;
+    float last_g = 0.0f;  // gamma of the most recent evaluation (at the current A.p)
     int repeats = 0;
     int iteration = 0;
 #pragma unroll 1
@@ -1560,6 +1565,7 @@ struct Solver {
 #pragma unroll
           for (int j = 2; j < 6; ++j) if (en[j] == 0.0f) A.p[j] = init[j];
           ytry = eval_here(A);
+          last_g = ytry;
           if (ytry < g_hi) {
 #pragma unroll
             for (int k = 0; k < 7; ++k)
@@ -1602,6 +1608,7 @@ struct Solver {
               A.p[j] = sums[j];
             }
             gv[i] = eval_here(A);
+            last_g = gv[i];
           }
         }
 #pragma unroll
@@ -1612,8 +1619,11 @@ struct Solver {
           sums[j] = acc;
         }
       }
-      // the extra objective evaluation at the current variables (DICe_Simplex.cpp:264)
-      gamma_new = eval_here(A);
+      // the extra objective evaluation at the current variables (DICe_Simplex.cpp:264):
+      // the variables are the last point evaluated (the trial point, or the last
+      // shrunk vertex) and the evaluation is deterministic, so its value is that
+      // evaluation's gamma
+      gamma_new = last_g;
       if (overflow) return kOverflow;
     }
     iters = iteration;
@@ -1801,9 +1811,11 @@ struct Solver {
   // SIMPLEX_OPT selects which optimizer this instantiation runs.  A BATCHED
   // wave whose window overflows its LDS slot pushes the subset to ov_list and
   // leaves it to the PLAIN kernel (nothing it wrote is an input of the solve).
+  // nm: per-subset Nelder-Mead results of simplex_lanes_kernel for this pass
+  // (SIMPLEX_OPT only), or null to run the simplex here.
   template <bool SIMPLEX_OPT>
   __device__ __forceinline__ void run(const DSubsets &ss, int gid, double *fields, int pass, int *fb_list,
-                                      int *fb_count, int *ov_list, int *ov_count) {
+                                      int *fb_count, int *ov_list, int *ov_count, const float *nm = nullptr) {
     const int nsub = ss.n;
     set_subset(ss, gid);
     last_fast = false;
@@ -1857,7 +1869,15 @@ struct Solver {
     float p_state[6];
     int corr;
     if (SIMPLEX_OPT) {
-      corr = simplex(A, iters);
+      if (nm) {
+        const float *r = nm + (size_t)gid * kNmStride;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) A.p[k] = r[k];
+        iters = __float_as_int(r[6]);
+        corr = __float_as_int(r[7]);
+      } else {
+        corr = simplex(A, iters);
+      }
 #pragma unroll
       for (int k = 0; k < 6; ++k) p_state[k] = A.p[k];
     } else {
@@ -1933,6 +1953,354 @@ struct Items {
   __device__ __forceinline__ int at(int k) const { return list ? list[k] : k; }
 };
 
+// ------------------------------------------- one subset per lane: simplex ---
+// One lane as a solver group: a Solver on it runs every pixel loop serially in
+// the lane and samples the images from global memory (no LDS window).
+struct LaneGroup {
+  static constexpr int kSize = 1;
+  __device__ __forceinline__ int rank() const { return 0; }
+  __device__ __forceinline__ void sync() const {}
+  template <int K>
+  __device__ __forceinline__ void sum(double (&v)[K]) const { (void)v; }
+  __device__ __forceinline__ int imin(int v) const { return v; }
+  __device__ __forceinline__ int imax(int v) const { return v; }
+};
+
+// Simplex::minimize (DICe_Simplex.cpp:60-277) of one subset per lane, written
+// as a state machine whose every step ends in exactly one objective
+// evaluation.  The lanes of a wave are each at their own stage of their own
+// subset (vertex set-up, reflection, expansion, contraction, shrink), and all
+// of them run the evaluation -- the expensive part -- together; only the cheap
+// bookkeeping between evaluations diverges.  Comparisons, update order and
+// arithmetic follow Solver::simplex.
+template <typename T, int INTERP>
+struct LaneSimplex {
+  enum : int { kVertex = 0, kReflect = 1, kExpand = 2, kContract = 3, kShrink = 4 };
+  Solver<T, LaneGroup, INTERP, false> s;
+  Affine A;
+  float pts[7][6], gv[7], sums[6], init[6], phi[6];  // phi: the high vertex; the low one while shrinking
+  float g_lo, g_hi, g_nhi, ysave, old_rtol, gamma_new, gamma_old;
+  int ilo, ihi, inhi, vk, iteration, repeats, phase;
+  int nd, mpts;
+
+  __device__ __forceinline__ LaneSimplex(const KParams &kp, const DImage &r, const DImage &d)
+      : s(kp, r, d, LaneGroup{}, nullptr, 0) {
+    nd = 2 + (kp.en_rot ? 1 : 0) + (kp.en_normal ? 2 : 0) + (kp.en_shear ? 1 : 0);
+    mpts = nd + 1;
+  }
+
+  // Start subset gid: vertices p0 and p0 + delta_i e_i; the first step
+  // evaluates vertex 0.
+  __device__ __forceinline__ void start(const DSubsets &ss, int gid, const double *fields) {
+    s.set_subset(ss, gid);
+    s.init_params(A, fields, ss.n, gid);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) init[j] = A.p[j];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      gv[i] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) pts[i][j] = init[j];
+    }
+    // vertex i moves the (i-1)-th enabled parameter (reference order u, v,
+    // theta, ex, ey, gxy) by its delta; rank: enabled parameters before slot j
+    int rank = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      if (s.en[j] != 0.0f) {
+#pragma unroll
+        for (int i = 1; i < 7; ++i)
+          if (i - 1 == rank) pts[i][j] += (j < 2) ? s.kp.delta_disp : s.kp.delta_theta;
+        ++rank;
+      }
+    }
+    phase = kVertex;
+    vk = 0;
+  }
+
+  // Objective::gamma at A.p (the sums of Solver::gamma_eval, pixels in subset
+  // order).  Pixels go in groups (four for bilinear) and every load of a group is issued
+  // before any is used: a pixel that is not kept samples a fixed interior
+  // position (4 px inside the deformed image, so every tap is in bounds) and
+  // adds nothing, so the loads carry no branches.
+  __device__ __forceinline__ float lane_gamma() {
+    constexpr int kU = INTERP == 0 ? 4 : 1;  // Keys: 36 taps a pixel already fill the register budget
+    const int n = s.n, S = s.S;
+    const bool square = S > 0;
+    const float safe_x = (float)(s.def.ox + 4 - s.cx), safe_y = (float)(s.def.oy + 4 - s.cy);
+    double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+    int row = 0, col = 0;
+    const int x0 = s.cx - s.hw, y0 = s.cy - s.hw;
+#pragma unroll 1
+    for (int i0 = 0; i0 < n; i0 += kU) {
+      float dv[kU], rv[kU];
+      bool keep[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = i0 + u;
+        const bool valid = i < n;
+        int x, y;
+        bool a = true;
+        if (square) {
+          x = max(x0 + col, 0);
+          y = max(y0 + row, 0);
+          if (++col == S) { col = 0; ++row; }
+        } else {
+          const int ii = valid ? i : n - 1;
+          x = max(s.px[ii], 0);
+          y = max(s.py[ii], 0);
+          if (s.pact) a = s.pact[ii] != 0;
+        }
+        float ox, oy;
+        A.disp((float)(x - s.cx), (float)(y - s.cy), ox, oy);
+        keep[u] = valid && a && s.kept(ox, oy);
+        dv[u] = s.sample(keep[u] ? ox : safe_x, keep[u] ? oy : safe_y);
+        const int rx = x - s.ref.ox, ry = y - s.ref.oy;
+        const bool rin = rx >= 0 && ry >= 0 && rx < s.ref.w && ry < s.ref.h;
+        rv[u] = Pix<T>::at(s.ref.pix, rin ? ry * s.ref.w + rx : 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (keep[u]) {
+          const double d = dv[u], r = rv[u];
+          acc[0] += 1.0;
+          acc[1] += d;
+          acc[2] += d * d;
+          acc[3] += r;
+          acc[4] += r * r;
+          acc[5] += d * r;
+        }
+      }
+    }
+    int active;
+    float contrast;
+    return s.gamma_of(acc, active, contrast);
+  }
+
+  // Trial point sums * fac1 - phi * fac2 (reflect -1, expand 2, contract 0.5);
+  // disabled parameters stay at their initial values.
+  __device__ __forceinline__ void set_trial(float fac) {
+    const float fac1 = (float)((1.0 - fac) / nd);
+    const float fac2 = fac1 - fac;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
+#pragma unroll
+    for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init[j];
+  }
+
+  __device__ __forceinline__ void column_sums() {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) if (i < mpts) acc += pts[i][j];
+      sums[j] = acc;
+    }
+  }
+
+  // Head of an iteration (no evaluation): stop tests, then the reflection.
+  // Returns the optimizer status when the subset is done, else -1.
+  __device__ __forceinline__ int top(int &iters) {
+    if (iteration >= s.kp.max_robust) { iters = iteration; return CORRELATION_SUCCESSFUL; }
+    if (iteration >= s.kp.max_robust - 1) { iters = iteration; return MAX_ITERATIONS_REACHED; }
+    ilo = 0;
+    if (gv[0] > gv[1]) { inhi = 1; ihi = 0; } else { inhi = 0; ihi = 1; }
+    g_lo = gv[0];
+    g_hi = ihi == 0 ? gv[0] : gv[1];
+    g_nhi = inhi == 0 ? gv[0] : gv[1];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (i < mpts) {
+        if (gv[i] <= g_lo) { ilo = i; g_lo = gv[i]; }
+        if (gv[i] > g_hi) { inhi = ihi; g_nhi = g_hi; ihi = i; g_hi = gv[i]; }
+        else if (gv[i] > g_nhi && i != ihi) { inhi = i; g_nhi = gv[i]; }
+      }
+    }
+    const float tiny = 1.0e-10f;
+    const float rtol = (float)(2.0 * fabsf(g_hi - g_lo) / (double)(fabsf(g_hi) + fabsf(g_lo) + tiny));
+    const float rtol_diff = fabsf(rtol - old_rtol);
+    old_rtol = rtol;
+    if (fabsf(gamma_new - gamma_old) < 1.0e-8f) repeats++;
+    gamma_old = gamma_new;
+    if (rtol < s.kp.robust_tol || rtol_diff < s.kp.robust_tol || repeats > 15) {
+      iters = iteration;
+      return CORRELATION_SUCCESSFUL;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      float v = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) if (k == ihi) v = pts[k][j];
+      phi[j] = v;
+    }
+    phase = kReflect;
+    set_trial(-1.0f);
+    return -1;
+  }
+
+  // End of an iteration: the extra evaluation of DICe_Simplex.cpp:264 is at the
+  // last evaluated point, so its value is g_last.
+  __device__ __forceinline__ int end_iteration(float g_last, int &iters) {
+    gamma_new = g_last;
+    ++iteration;
+    return top(iters);
+  }
+
+  // Next vertex of the shrink towards the low vertex (held in phi), or the end
+  // of the iteration.
+  __device__ __forceinline__ int next_shrink(float g_last, int &iters) {
+    int next = 7;
+#pragma unroll
+    for (int i = 6; i >= 0; --i)
+      if (i > vk && i < mpts && i != ilo) next = i;
+    if (next == 7) {
+      column_sums();
+      return end_iteration(g_last, iters);
+    }
+    vk = next;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (i == vk) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          pts[i][j] = sums[j] = (float)(0.5 * ((double)pts[i][j] + (double)phi[j]));
+          A.p[j] = sums[j];
+        }
+      }
+    }
+    return -1;
+  }
+
+  // Consume the objective value g at A.p; set A.p to the next point to
+  // evaluate (return -1) or finish (return the status; iters, and A.p = the
+  // last evaluated point, are the result).
+  __device__ __forceinline__ int step(float g, int &iters) {
+    if (phase == kVertex) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
+      if (vk == 0 && g < s.kp.skip_thr && g >= 0.0f) { iters = 0; return CORRELATION_SUCCESSFUL; }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) A.p[j] = init[j];
+      ++vk;
+      if (vk < mpts) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+          if (i == vk) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) A.p[j] = pts[i][j];
+          }
+        return -1;
+      }
+      column_sums();
+      old_rtol = 0.0f;
+      gamma_new = 0.0f;
+      gamma_old = -1.0f;
+      repeats = 0;
+      iteration = 0;
+      return top(iters);
+    }
+    if (phase == kShrink) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
+      return next_shrink(g, iters);
+    }
+    // a trial point: it replaces the high vertex if it is better
+    const float ytry = g;
+    if (ytry < g_hi) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k)
+        if (k == ihi) {
+          gv[k] = ytry;
+#pragma unroll
+          for (int j = 0; j < 6; ++j) { sums[j] += A.p[j] - pts[k][j]; pts[k][j] = A.p[j]; }
+        }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) phi[j] = A.p[j];
+      g_hi = ytry;
+    }
+    if (phase == kReflect) {
+      if (ytry <= g_lo) { phase = kExpand; set_trial(2.0f); return -1; }
+      if (ytry >= g_nhi) { ysave = g_hi; phase = kContract; set_trial(0.5f); return -1; }
+      return end_iteration(ytry, iters);
+    }
+    if (phase == kContract && ytry >= ysave) {
+      // shrink every vertex but the low one towards it
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        float v = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) if (k == ilo) v = pts[k][j];
+        phi[j] = v;
+      }
+      phase = kShrink;
+      vk = -1;
+      return next_shrink(ytry, iters);
+    }
+    return end_iteration(ytry, iters);
+  }
+};
+
+// BATCHED Nelder-Mead iterations, one subset per lane, for small subsets.
+// Lanes that finish a subset write its result to nm_out and take the next
+// items: one atomic per wave and refill, the lanes numbered by their rank in
+// the ballot.  A lane whose item index is past the end stays idle, and the
+// wave exits when all of its lanes are idle: every simplex ends within
+// max_robust iterations, so every wave reaches that exit.
+template <typename T, int INTERP>
+__global__ __launch_bounds__(256) void simplex_lanes_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
+                                                            const double *fields, PassArgs pa, float *nm_out,
+                                                            int *counter) {
+  const Items items{pa.items, pa.item_count};
+  const int total = items.total(ss.n);
+  const int lane = (int)(threadIdx.x % kWave);
+  LaneSimplex<T, INTERP> m(kp, ref, def);
+  int gid = 0;
+  bool need = true, live = false;
+#pragma unroll 1
+  while (true) {
+    const unsigned long long want = __ballot(need);
+    if (want) {
+      const int leader = __ffsll((unsigned long long)want) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(counter, __popcll(want));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (need) {
+        const int k = base + __popcll(want & ((1ull << lane) - 1ull));
+        live = k < total;
+        if (live) {
+          gid = items.at(k);
+          m.start(ss, gid, fields);
+        }
+        need = false;
+      }
+    }
+    if (__ballot(live) == 0) break;
+    if (live) {
+      m.A.prepare();
+      const float g = m.lane_gamma();
+      int iters = 0;
+      const int st = m.step(g, iters);
+      if (st >= 0) {
+        float *o = nm_out + (size_t)gid * kNmStride;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) o[j] = m.A.p[j];
+        o[6] = __int_as_float(iters);
+        o[7] = __int_as_float(st);
+        need = true;
+      }
+    }
+  }
+}
+
+template <typename T, int INTERP>
+hipError_t launch_simplex_lanes_variant(const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &subs,
+                                        const double *fields, const PassArgs &pa, float *nm_out, int *counter,
+                                        int groups, hipStream_t stream) {
+  hipLaunchKernelGGL((simplex_lanes_kernel<T, INTERP>), dim3(groups), dim3(256), 0, stream, kp, ref, def, subs, fields,
+                     pa, nm_out, counter);
+  return hipGetLastError();
+}
+
 // PLAIN: workgroup k handles items k, k + gridDim.x, ...
 template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
 __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams kp, DImage ref, DImage def,
@@ -1944,7 +2312,8 @@ __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams 
   Solver<T, BlockGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)red, 0);
 #pragma unroll 1
   for (int k = blockIdx.x; k < total; k += gridDim.x)
-    s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, nullptr, nullptr);
+    s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, nullptr, nullptr,
+                                pa.nm_result);
 }
 
 // BATCHED: persistent waves pull items from work counters until every range
@@ -1983,7 +2352,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
       continue;
     }
     s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
-                                pa.ov_count);
+                                pa.ov_count, pa.nm_result);
   }
 }
 

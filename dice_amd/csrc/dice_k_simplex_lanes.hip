@@ -1,0 +1,39 @@
+// simplex_lanes_kernel: BATCHED Nelder-Mead iterations with one subset per lane
+// (template instantiations of dice_device.h per pixel storage / interpolation).
+#include "dice_device.h"
+
+namespace dice_hip {
+
+namespace {
+// Persistent grid: every CU at the occupancy the kernel's registers allow.
+template <typename T, int INTERP>
+int lane_groups() {
+  static int groups = 0;
+  if (groups == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::simplex_lanes_kernel<T, INTERP>, 256, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+      per_cu = 2;
+    groups = cus * per_cu;
+  }
+  return groups;
+}
+}  // namespace
+
+hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, const DImage &ref, const DImage &def,
+                                const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out,
+                                int *counter, hipStream_t stream) {
+  const int need = (subs.n + 255) / 256;  // at most one lane per subset
+#define V(T, I)                                                                                            \
+  dev::launch_simplex_lanes_variant<T, I>(kp, ref, def, subs, fields, pa, nm_out, counter,                 \
+                                          need < lane_groups<T, I>() ? need : lane_groups<T, I>(), stream)
+  if (storage == 0) return interp == 0 ? V(float, 0) : V(float, 2);
+  return interp == 0 ? V(uint16_t, 0) : V(uint16_t, 2);
+#undef V
+}
+
+}  // namespace dice_hip

@@ -89,7 +89,16 @@ struct Launch {
   int waves_per_group;         // batched
   int groups;                  // batched: persistent grid size
   int lds_floats_per_wave;     // batched: LDS window budget
+  int lane_simplex;            // batched: Nelder-Mead passes run one subset per lane first
 };
+
+// Subsets of at most this many pixels run the Nelder-Mead iterations of the
+// BATCHED mode one subset per lane (simplex_lanes_kernel): a wave then works on
+// 64 subsets instead of spreading a few dozen pixels over 64 lanes.
+constexpr int kLaneSimplexMaxPixels = 256;
+// Per-subset result of simplex_lanes_kernel: the last evaluated point (6
+// floats), then the iteration count and the optimizer status as int bits.
+constexpr int kNmStride = 8;
 
 // Device work lists of one correlation pass.  items == nullptr: every subset;
 // otherwise the *item_count subsets listed in items.  Subsets that ask for the
@@ -102,6 +111,7 @@ struct PassArgs {
   int *fb_list, *fb_count;
   int *ov_list, *ov_count;
   int *counter;                // BATCHED work counters, kWorkRanges ints (zeroed before the launch)
+  const float *nm_result;      // Nelder-Mead passes: per-subset results of simplex_lanes_kernel, or null
 };
 
 // Differentiation path (forward gamma / adjoint) of one launch.
@@ -120,9 +130,11 @@ constexpr int kWorkRanges = 8;
 // overflow counts of pass 0 and pass 1.
 constexpr int kWorkHeader = 16;
 constexpr int kWorkFbCount = kWorkRanges, kWorkOv0Count = kWorkRanges + 1, kWorkOv1Count = kWorkRanges + 2;
+constexpr int kWorkLaneCount = kWorkRanges + 3;  // work counter of simplex_lanes_kernel
 
-// Device ints of the work area of launch_correlate.
-inline size_t work_ints(int n) { return kWorkHeader + 3 * (size_t)(n > 0 ? n : 0); }
+// Device ints of the work area of launch_correlate: the header, the fallback
+// and overflow lists (n each), the Nelder-Mead results (kNmStride per subset).
+inline size_t work_ints(int n) { return kWorkHeader + (3 + kNmStride) * (size_t)(n > 0 ? n : 0); }
 
 // Enqueue the correlation of all subsets (primary pass, then the fallback pass
 // when the optimization method has one).  storage: 0 f32, 1 u16.  work: device
@@ -135,6 +147,13 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
 // PLAIN kernel over the subsets whose window did not fit).
 hipError_t launch_gamma(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
                         const DSubsets &subs, const GammaArgs &ga, int *work, hipStream_t stream);
+
+// Enqueue the Nelder-Mead iterations of the items of pa, one subset per lane:
+// results to nm_out (kNmStride floats per subset); counter: one int, zeroed
+// before the launch.
+hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, const DImage &ref, const DImage &def,
+                                const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out,
+                                int *counter, hipStream_t stream);
 
 // Finite-difference gradients (+ optional 5x5 smoothing) of an image.
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,

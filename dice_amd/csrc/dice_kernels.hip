@@ -71,17 +71,30 @@ __global__ void copy_doubles_kernel(double *dst, const double *src, size_t count
 
 // One optimizer pass over `items` (nullptr: all subsets).  BATCHED mode runs the
 // persistent kernel, then the PLAIN kernel over the subsets it handed over.
+// A Nelder-Mead pass with L.lane_simplex first runs the iterations one subset
+// per lane (results in nm); the persistent kernel then does the checks and the
+// epilogue of each subset around them.
 static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const KParams &kp, const DImage &ref,
                               const DImage &def, const DSubsets &subs, double *fields, PassArgs pa, int *ov_list,
-                              int *ov_count, hipStream_t stream) {
+                              int *ov_count, float *nm, hipStream_t stream) {
   const int defgrad = simplex ? 0 : kp.def_grads;
   PassLauncher plain = simplex ? launch_simplex_plain : launch_gn_plain;
+  pa.nm_result = nullptr;
   if (L.mode == 0) {
     pa.ov_list = pa.ov_count = nullptr;
     return plain(storage, kp.interp, defgrad, L, kp, ref, def, subs, fields, pa, stream);
   }
   PassLauncher batched = simplex ? launch_simplex_batched : launch_gn_batched;
-  hipError_t e = launch_zero_ints(pa.counter, kWorkRanges, stream);
+  hipError_t e;
+  if (simplex && L.lane_simplex) {
+    int *lane_counter = pa.counter + kWorkLaneCount;
+    e = launch_zero_ints(lane_counter, 1, stream);
+    if (e != hipSuccess) return e;
+    e = launch_simplex_lanes(storage, kp.interp, kp, ref, def, subs, fields, pa, nm, lane_counter, stream);
+    if (e != hipSuccess) return e;
+    pa.nm_result = nm;
+  }
+  e = launch_zero_ints(pa.counter, kWorkRanges, stream);
   if (e != hipSuccess) return e;
   pa.ov_list = ov_list;
   pa.ov_count = ov_count;
@@ -101,14 +114,15 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
   // fallback list and the overflow lists of pass 0 and pass 1 (n each)
   const size_t n = (size_t)subs.n;
   int *fb_list = work + kWorkHeader, *ov0 = fb_list + n, *ov1 = fb_list + 2 * n;
+  float *nm = reinterpret_cast<float *>(fb_list + 3 * n);
   hipError_t e = launch_zero_ints(work, kWorkHeader, stream);
   if (e != hipSuccess) return e;
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
-  PassArgs pa{0, nullptr, nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work};
-  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, stream);
+  PassArgs pa{0, nullptr, nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work, nullptr};
+  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, nm, stream);
   if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
-  PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work};
-  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + kWorkOv1Count,
+  PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work, nullptr};
+  return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + kWorkOv1Count, nm,
                      stream);
 }
 

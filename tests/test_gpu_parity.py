@@ -268,6 +268,63 @@ def test_simplex_speckle_parity(ctx, mode, nparams):
     assert np.abs(fg[dh.F["DISP_X"]] - fd[o.F["DISP_X"]]).max() < 2e-2
 
 
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+@pytest.mark.parametrize("nparams", [2, 6])
+def test_simplex_lane_path_speckle(ctx, mode, nparams):
+    """SIMPLEX on 15 x 15 subsets (225 px): small enough for the BATCHED mode to run the Nelder-Mead
+    iterations one subset per lane, the persistent kernel doing the checks and the epilogue."""
+    d = load("speckle_512.npz")
+    cx, cy = dh.regular_grid(512, 512, 15, 24)
+    p = dh.default_params()
+    p.optimization_method = dh.SIMPLEX
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1 if nparams == 6 else 0
+    fg = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, 15, mode)
+    fo = run_oracle(p, d["ref"], d["defm"], cx, cy, 15, precision="float")
+    fd = run_oracle(p, d["ref"], d["defm"], cx, cy, 15, precision="double")
+    simplex_spread_check(fg, fo, fd)
+    for name in ("SIGMA", "NOISE", "CONTRAST"):
+        ok = np.isin(fg[dh.F["STATUS"]], dh.CONVERGED_STATUSES) & np.isin(fd[o.F["STATUS"]], dh.CONVERGED_STATUSES)
+        rel = np.abs(fg[dh.F[name]] - fd[o.F[name]])[ok] / np.maximum(np.abs(fd[o.F[name]][ok]), 1e-12)
+        assert np.median(rel) < 1e-3, name
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_simplex_lane_path_conformal(ctx, mode):
+    """SIMPLEX on polygon subsets of under 256 pixels (with excluded pixels) through the one-subset-per-lane
+    iterations."""
+    ref, de = synth.config1(seed=5)
+    cx, cy, conf = conformal_set(512, 512, 120, 4, rmin=7, rmax=8.8)
+    assert np.diff(conf[0]).max() <= 256
+    p = dh.default_params()
+    p.optimization_method = dh.SIMPLEX
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    fg = run_gpu(ctx, p, ref, de, cx, cy, 0, mode, conformal=conf)
+    fo = run_oracle(p, ref, de, cx, cy, 0, conformal=conf, precision="float")
+    fd = run_oracle(p, ref, de, cx, cy, 0, conformal=conf, precision="double")
+    simplex_spread_check(fg, fo, fd)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
+def test_fallback_lane_path(ctx, mode):
+    """GRADIENT_BASED_THEN_SIMPLEX on 15 x 15 subsets: the simplex fallback pass over the device list runs
+    one subset per lane; FOV and initial-guess failures keep their statuses."""
+    d = load("speckle_512.npz")
+    cx = np.array([200.0, 4.0, 300.0, 150.0, 260.0, 330.0])
+    cy = np.array([200.0, 200.0, 300.0, 150.0, 120.0, 400.0])
+    f0 = dh.new_fields(cx, cy)
+    f0[dh.F["SIGMA"], 2] = -1.0
+    p = dh.default_params()
+    p.max_solver_iterations_fast = 0
+    p.optimization_method = dh.GRADIENT_BASED_THEN_SIMPLEX
+    fg = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, 15, mode, fields=f0)
+    fo = run_oracle(p, d["ref"], d["defm"], cx, cy, 15, fields=f0)
+    assert np.array_equal(fg[dh.F["STATUS"]], fo[o.F["STATUS"]])
+    assert fg[dh.F["STATUS"], 1] == 12 and fg[dh.F["STATUS"], 2] == 5
+    ok = np.isin(fg[dh.F["STATUS"]], dh.CONVERGED_STATUSES)
+    assert ok.sum() == 4
+    assert np.abs(fg[dh.F["DISP_X"]] - fo[o.F["DISP_X"]])[ok].max() < 2e-2
+
+
 def conformal_set(w, h, n, seed, rmin=10, rmax=20):
     """Random convex polygons (vertices on a jittered circle) rasterised by the
     Polygon::get_owned_pixels rule (oracle restatement used only to build inputs here)."""
