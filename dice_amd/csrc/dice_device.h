@@ -1954,6 +1954,20 @@ struct Items {
 };
 
 // ------------------------------------------- one subset per lane: simplex ---
+// Pixels per gather group of the bilinear lane objective, and the register
+// budget of simplex_lanes_kernel as waves per SIMD (0: the compiler's choice).
+#ifndef DICE_LANE_PIXEL_GROUP
+#define DICE_LANE_PIXEL_GROUP 4
+#endif
+#ifndef DICE_LANE_WAVES_PER_EU
+#define DICE_LANE_WAVES_PER_EU 4
+#endif
+#if DICE_LANE_WAVES_PER_EU > 0
+#define DICE_LANE_BOUNDS __attribute__((amdgpu_waves_per_eu(DICE_LANE_WAVES_PER_EU)))
+#else
+#define DICE_LANE_BOUNDS
+#endif
+
 // One lane as a solver group: a Solver on it runs every pixel loop serially in
 // the lane and samples the images from global memory (no LDS window).
 struct LaneGroup {
@@ -1973,7 +1987,7 @@ struct LaneGroup {
 // of them run the evaluation -- the expensive part -- together; only the cheap
 // bookkeeping between evaluations diverges.  Comparisons, update order and
 // arithmetic follow Solver::simplex.
-template <typename T, int INTERP>
+template <typename T, int INTERP, bool SQUARE>
 struct LaneSimplex {
   enum : int { kVertex = 0, kReflect = 1, kExpand = 2, kContract = 3, kShrink = 4 };
   Solver<T, LaneGroup, INTERP, false> s;
@@ -2024,11 +2038,12 @@ struct LaneSimplex {
   // position (4 px inside the deformed image, so every tap is in bounds) and
   // adds nothing, so the loads carry no branches.
   __device__ __forceinline__ float lane_gamma() {
-    constexpr int kU = INTERP == 0 ? 4 : 1;  // Keys: 36 taps a pixel already fill the register budget
+    // Keys: 36 taps a pixel already fill the register budget
+    constexpr int kU = INTERP == 0 ? DICE_LANE_PIXEL_GROUP : 1;
     const int n = s.n, S = s.S;
-    const bool square = S > 0;
     const float safe_x = (float)(s.def.ox + 4 - s.cx), safe_y = (float)(s.def.oy + 4 - s.cy);
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+    int cnt = 0;
     int row = 0, col = 0;
     const int x0 = s.cx - s.hw, y0 = s.cy - s.hw;
 #pragma unroll 1
@@ -2041,7 +2056,7 @@ struct LaneSimplex {
         const bool valid = i < n;
         int x, y;
         bool a = true;
-        if (square) {
+        if constexpr (SQUARE) {
           x = max(x0 + col, 0);
           y = max(y0 + row, 0);
           if (++col == S) { col = 0; ++row; }
@@ -2063,7 +2078,7 @@ struct LaneSimplex {
       for (int u = 0; u < kU; ++u) {
         if (keep[u]) {
           const double d = dv[u], r = rv[u];
-          acc[0] += 1.0;
+          ++cnt;
           acc[1] += d;
           acc[2] += d * d;
           acc[3] += r;
@@ -2072,20 +2087,10 @@ struct LaneSimplex {
         }
       }
     }
+    acc[0] = (double)cnt;
     int active;
     float contrast;
     return s.gamma_of(acc, active, contrast);
-  }
-
-  // Trial point sums * fac1 - phi * fac2 (reflect -1, expand 2, contract 0.5);
-  // disabled parameters stay at their initial values.
-  __device__ __forceinline__ void set_trial(float fac) {
-    const float fac1 = (float)((1.0 - fac) / nd);
-    const float fac2 = fac1 - fac;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
-#pragma unroll
-    for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init[j];
   }
 
   __device__ __forceinline__ void column_sums() {
@@ -2098,9 +2103,10 @@ struct LaneSimplex {
     }
   }
 
-  // Head of an iteration (no evaluation): stop tests, then the reflection.
-  // Returns the optimizer status when the subset is done, else -1.
-  __device__ __forceinline__ int top(int &iters) {
+  // Head of an iteration (no evaluation): stop tests, then the high vertex
+  // into phi for the reflection.  Returns the optimizer status when the
+  // subset is done, else -1.
+  __device__ __forceinline__ int head(int &iters) {
     if (iteration >= s.kp.max_robust) { iters = iteration; return CORRELATION_SUCCESSFUL; }
     if (iteration >= s.kp.max_robust - 1) { iters = iteration; return MAX_ITERATIONS_REACHED; }
     ilo = 0;
@@ -2134,47 +2140,20 @@ struct LaneSimplex {
       phi[j] = v;
     }
     phase = kReflect;
-    set_trial(-1.0f);
-    return -1;
-  }
-
-  // End of an iteration: the extra evaluation of DICe_Simplex.cpp:264 is at the
-  // last evaluated point, so its value is g_last.
-  __device__ __forceinline__ int end_iteration(float g_last, int &iters) {
-    gamma_new = g_last;
-    ++iteration;
-    return top(iters);
-  }
-
-  // Next vertex of the shrink towards the low vertex (held in phi), or the end
-  // of the iteration.
-  __device__ __forceinline__ int next_shrink(float g_last, int &iters) {
-    int next = 7;
-#pragma unroll
-    for (int i = 6; i >= 0; --i)
-      if (i > vk && i < mpts && i != ilo) next = i;
-    if (next == 7) {
-      column_sums();
-      return end_iteration(g_last, iters);
-    }
-    vk = next;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      if (i == vk) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-          pts[i][j] = sums[j] = (float)(0.5 * ((double)pts[i][j] + (double)phi[j]));
-          A.p[j] = sums[j];
-        }
-      }
-    }
     return -1;
   }
 
   // Consume the objective value g at A.p; set A.p to the next point to
   // evaluate (return -1) or finish (return the status; iters, and A.p = the
-  // last evaluated point, are the result).
+  // last evaluated point, are the result).  Each piece of bookkeeping has one
+  // site, so lanes in different phases pay for each once.  The extra
+  // evaluation at the end of an iteration (DICe_Simplex.cpp:264) is at the
+  // last evaluated point, so its value is g.
   __device__ __forceinline__ int step(float g, int &iters) {
+    float fac = 0.0f;       // next: the trial point sums * fac1 - phi * fac2
+    bool shrink = false;    // next: the following vertex of a shrink
+    bool next_iter = false; // the iteration ended with evaluation g
+    bool first = false;     // the vertices are done: the first iteration head
     if (phase == kVertex) {
 #pragma unroll
       for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
@@ -2197,46 +2176,87 @@ struct LaneSimplex {
       gamma_old = -1.0f;
       repeats = 0;
       iteration = 0;
-      return top(iters);
-    }
-    if (phase == kShrink) {
+      first = true;
+    } else if (phase == kShrink) {
 #pragma unroll
       for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
-      return next_shrink(g, iters);
-    }
-    // a trial point: it replaces the high vertex if it is better
-    const float ytry = g;
-    if (ytry < g_hi) {
+      shrink = true;
+    } else {
+      // a trial point: it replaces the high vertex if it is better
+      if (g < g_hi) {
 #pragma unroll
-      for (int k = 0; k < 7; ++k)
-        if (k == ihi) {
-          gv[k] = ytry;
+        for (int k = 0; k < 7; ++k)
+          if (k == ihi) {
+            gv[k] = g;
 #pragma unroll
-          for (int j = 0; j < 6; ++j) { sums[j] += A.p[j] - pts[k][j]; pts[k][j] = A.p[j]; }
-        }
+            for (int j = 0; j < 6; ++j) { sums[j] += A.p[j] - pts[k][j]; pts[k][j] = A.p[j]; }
+          }
 #pragma unroll
-      for (int j = 0; j < 6; ++j) phi[j] = A.p[j];
-      g_hi = ytry;
-    }
-    if (phase == kReflect) {
-      if (ytry <= g_lo) { phase = kExpand; set_trial(2.0f); return -1; }
-      if (ytry >= g_nhi) { ysave = g_hi; phase = kContract; set_trial(0.5f); return -1; }
-      return end_iteration(ytry, iters);
-    }
-    if (phase == kContract && ytry >= ysave) {
-      // shrink every vertex but the low one towards it
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        float v = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) if (k == ilo) v = pts[k][j];
-        phi[j] = v;
+        for (int j = 0; j < 6; ++j) phi[j] = A.p[j];
+        g_hi = g;
       }
-      phase = kShrink;
-      vk = -1;
-      return next_shrink(ytry, iters);
+      if (phase == kReflect && g <= g_lo) {
+        phase = kExpand;
+        fac = 2.0f;
+      } else if (phase == kReflect && g >= g_nhi) {
+        ysave = g_hi;
+        phase = kContract;
+        fac = 0.5f;
+      } else if (phase == kContract && g >= ysave) {
+        // shrink every vertex but the low one towards it (held in phi)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          float v = 0.0f;
+#pragma unroll
+          for (int k = 0; k < 7; ++k) if (k == ilo) v = pts[k][j];
+          phi[j] = v;
+        }
+        phase = kShrink;
+        vk = -1;
+        shrink = true;
+      } else {
+        next_iter = true;
+      }
     }
-    return end_iteration(ytry, iters);
+    if (shrink) {
+      int next = 7;
+#pragma unroll
+      for (int i = 6; i >= 0; --i)
+        if (i > vk && i < mpts && i != ilo) next = i;
+      if (next < 7) {
+        vk = next;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          if (i == vk) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+              pts[i][j] = sums[j] = (float)(0.5 * ((double)pts[i][j] + (double)phi[j]));
+              A.p[j] = sums[j];
+            }
+          }
+        }
+        return -1;
+      }
+      column_sums();
+      next_iter = true;
+    }
+    if (next_iter) {
+      gamma_new = g;
+      ++iteration;
+    }
+    if (next_iter || first) {
+      const int st = head(iters);
+      if (st >= 0) return st;
+      fac = -1.0f;
+    }
+    // trial point; disabled parameters stay at their initial values
+    const float fac1 = (float)((1.0 - fac) / nd);
+    const float fac2 = fac1 - fac;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
+#pragma unroll
+    for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init[j];
+    return -1;
   }
 };
 
@@ -2246,14 +2266,14 @@ struct LaneSimplex {
 // the ballot.  A lane whose item index is past the end stays idle, and the
 // wave exits when all of its lanes are idle: every simplex ends within
 // max_robust iterations, so every wave reaches that exit.
-template <typename T, int INTERP>
-__global__ __launch_bounds__(256) void simplex_lanes_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
-                                                            const double *fields, PassArgs pa, float *nm_out,
-                                                            int *counter) {
+template <typename T, int INTERP, bool SQUARE>
+__global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_lanes_kernel(KParams kp, DImage ref, DImage def,
+                                                                             DSubsets ss, const double *fields,
+                                                                             PassArgs pa, float *nm_out, int *counter) {
   const Items items{pa.items, pa.item_count};
   const int total = items.total(ss.n);
   const int lane = (int)(threadIdx.x % kWave);
-  LaneSimplex<T, INTERP> m(kp, ref, def);
+  LaneSimplex<T, INTERP, SQUARE> m(kp, ref, def);
   int gid = 0;
   bool need = true, live = false;
 #pragma unroll 1
@@ -2292,12 +2312,12 @@ __global__ __launch_bounds__(256) void simplex_lanes_kernel(KParams kp, DImage r
   }
 }
 
-template <typename T, int INTERP>
+template <typename T, int INTERP, bool SQUARE>
 hipError_t launch_simplex_lanes_variant(const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &subs,
                                         const double *fields, const PassArgs &pa, float *nm_out, int *counter,
                                         int groups, hipStream_t stream) {
-  hipLaunchKernelGGL((simplex_lanes_kernel<T, INTERP>), dim3(groups), dim3(256), 0, stream, kp, ref, def, subs, fields,
-                     pa, nm_out, counter);
+  hipLaunchKernelGGL((simplex_lanes_kernel<T, INTERP, SQUARE>), dim3(groups), dim3(256), 0, stream, kp, ref, def,
+                     subs, fields, pa, nm_out, counter);
   return hipGetLastError();
 }
 

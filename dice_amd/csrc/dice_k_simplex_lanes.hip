@@ -6,7 +6,7 @@ namespace dice_hip {
 
 namespace {
 // Persistent grid: every CU at the occupancy the kernel's registers allow.
-template <typename T, int INTERP>
+template <typename T, int INTERP, bool SQUARE>
 int lane_groups() {
   static int groups = 0;
   if (groups == 0) {
@@ -14,7 +14,7 @@ int lane_groups() {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::simplex_lanes_kernel<T, INTERP>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::simplex_lanes_kernel<T, INTERP, SQUARE>, 256, 0) !=
             hipSuccess ||
         per_cu <= 0)
       per_cu = 2;
@@ -28,11 +28,15 @@ hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, cons
                                 const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out,
                                 int *counter, hipStream_t stream) {
   const int need = (subs.n + 255) / 256;  // at most one lane per subset
-#define V(T, I)                                                                                            \
-  dev::launch_simplex_lanes_variant<T, I>(kp, ref, def, subs, fields, pa, nm_out, counter,                 \
-                                          need < lane_groups<T, I>() ? need : lane_groups<T, I>(), stream)
-  if (storage == 0) return interp == 0 ? V(float, 0) : V(float, 2);
-  return interp == 0 ? V(uint16_t, 0) : V(uint16_t, 2);
+#define V(T, I, Q)                                                                                         \
+  dev::launch_simplex_lanes_variant<T, I, Q>(kp, ref, def, subs, fields, pa, nm_out, counter,              \
+                                             need < lane_groups<T, I, Q>() ? need : lane_groups<T, I, Q>(), stream)
+  if (subs.S > 0) {
+    if (storage == 0) return interp == 0 ? V(float, 0, true) : V(float, 2, true);
+    return interp == 0 ? V(uint16_t, 0, true) : V(uint16_t, 2, true);
+  }
+  if (storage == 0) return interp == 0 ? V(float, 0, false) : V(float, 2, false);
+  return interp == 0 ? V(uint16_t, 0, false) : V(uint16_t, 2, false);
 #undef V
 }
 
