@@ -1102,6 +1102,12 @@ struct Solver {
   // Adds g * (interpolation weight) to every tap of the sample at (ddx, ddy):
   // into acc (LDS, the window's layout, pitch wpitch) when acc is set, else
   // into the global image gradient.
+  // rmw: plain LDS read-modify-writes instead of LDS atomics, one tap at a
+  // time in program order -- only for lanes whose samples lie in distinct
+  // cells (no two lanes of the wave write one cell for the same tap); the
+  // wave's LDS operations execute in order, and the compiler barrier after
+  // each tap keeps that order.
+  template <bool RMW = false>
   __device__ __forceinline__ void scatter_taps(float ddx, float ddy, float g, lds_float *acc, float *dimg) const {
     const float flx = floorf(ddx), fly = floorf(ddy);
     const float tx = ddx - flx, ty = ddy - fly;
@@ -1125,9 +1131,16 @@ struct Solver {
         if (k >= taps) break;
         const float c = g * wy[m] * wx[k];
         const int yy = iy - lead + m, xx = ix - lead + k;
-        if (acc) __hip_atomic_fetch_add(acc + (__umul24(yy - wy0, wpitch) + (xx - wx0)), c, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
-        else unsafeAtomicAdd(dimg + (size_t)yy * def.w + xx, c);
+        if constexpr (RMW) {
+          lds_float *cell = acc + (__umul24(yy - wy0, wpitch) + (xx - wx0));
+          *cell += c;
+          asm volatile("" ::: "memory");
+        } else if (acc) {
+          __hip_atomic_fetch_add(acc + (__umul24(yy - wy0, wpitch) + (xx - wx0)), c, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
+        } else {
+          unsafeAtomicAdd(dimg + (size_t)yy * def.w + xx, c);
+        }
       }
     }
   }
@@ -1172,8 +1185,8 @@ struct Solver {
     }
     const double s = dl_dgamma * 2.0 / z.nd;
     double g6[6] = {0, 0, 0, 0, 0, 0};
-    for_pixels([&](int x, int y, bool a) {
-      if (!a) return;
+    // one pixel: dL/dp terms, and the taps of dL/dI (scatter(ox, oy, g))
+    auto pixel = [&](int x, int y, auto &&scatter) {
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
@@ -1186,8 +1199,42 @@ struct Solver {
       A.residual(dx, dy, Ix, Iy, row);
 #pragma unroll
       for (int k = 0; k < 6; ++k) g6[k] = fma(gj, (double)row[k], g6[k]);
-      if (dimg) scatter_taps(ox, oy, (float)gj, acc, dimg);
-    });
+      scatter(ox, oy, (float)gj);
+    };
+    // Square subsets with unclamped pixels whose warp moves two pixels of one
+    // (x mod 2, y mod 2) class at least 1.1 px apart in x or y (|M v| >= |v|
+    // (1 - ||M - I||) with |v| >= 2 and ||M - I||_inf <= 0.45): such samples
+    // never share a cell, so the class's taps go to LDS with plain
+    // read-modify-writes, class by class.
+    bool classes = false;
+    if constexpr (kIsWave) {
+      const float mi = fmaxf(fabsf(A.a11 - 1.0f) + fabsf(A.a12), fabsf(A.a21) + fabsf(A.a22 - 1.0f));
+      classes = acc != nullptr && S > 0 && cx - hw >= 0 && cy - hw >= 0 && mi <= 0.45f;
+    }
+    if (classes) {
+      const int lane = grp.rank();
+      const int x0 = cx - hw, y0 = cy - hw;
+#pragma unroll 1
+      for (int ph = 0; ph < 4; ++ph) {
+        const int pa = ph >> 1, pb = ph & 1;
+        const int nc = (S - pb + 1) >> 1, cnt = ((S - pa + 1) >> 1) * nc;
+        const float inv = 1.0f / (float)nc;
+#pragma unroll 1
+        for (int k = lane; k < cnt; k += G::kSize) {
+          const int rr = (int)(((float)k + 0.5f) * inv);  // exact: k < 2^20
+          const int cc = k - rr * nc;
+          pixel(x0 + pb + 2 * cc, y0 + pa + 2 * rr,
+                [&](float ox, float oy, float g) { scatter_taps<true>(ox, oy, g, acc, dimg); });
+        }
+      }
+    } else {
+      for_pixels([&](int x, int y, bool a) {
+        if (!a) return;
+        pixel(x, y, [&](float ox, float oy, float g) {
+          if (dimg) scatter_taps(ox, oy, g, acc, dimg);
+        });
+      });
+    }
     grp.sum(g6);
 #pragma unroll
     for (int k = 0; k < 6; ++k) dp[k] = g6[k];

@@ -130,6 +130,28 @@ def test_adjoint_image_matches_central_differences(scene, mode):
     c.close()
 
 
+@pytest.mark.parametrize("strain", [0.0, 0.6], ids=["classes", "atomics"])
+def test_adjoint_image_batched_matches_plain(scene, strain):
+    """The whole dL/dI_def image of the BATCHED adjoint against the PLAIN kernel's (global atomics per tap).
+    strain 0: every subset takes the class-by-class LDS read-modify-write scatter; strain 0.6: the warp
+    moves pixels too far for that (||M - I|| > 0.45) and the scatter uses LDS atomics.  Only the order
+    of the float sums differs."""
+    ref, de, field, cx, cy = scene
+    rng = np.random.default_rng(10)
+    P = params_from_field(field, cx, cy)
+    P[:, :2] += rng.normal(0, 0.2, (len(P), 2))
+    P[:, 3] += strain
+    w = rng.uniform(0.5, 1.5, len(P))
+    c = make_ctx(ref, de, cx, cy)
+    dpb, ib = adjoint(c, P, w, dh.MODE_BATCHED, shape=de.shape)
+    dpp, ip = adjoint(c, P, w, dh.MODE_PLAIN, shape=de.shape)
+    assert np.count_nonzero(ib) > 0.5 * len(cx) * S * S
+    assert np.linalg.norm(ib - ip) / np.linalg.norm(ip) < 1e-5
+    assert np.abs(ib - ip).max() <= 1e-4 * np.abs(ip).max()
+    assert np.linalg.norm(dpb - dpp) / np.linalg.norm(dpp) < 1e-6
+    c.close()
+
+
 def test_adjoint_direction_matches_gauss_newton_gradient(scene):
     """Near the optimum the ZNSSD gradient points along the Gauss-Newton gradient q of computeUpdateFast
     (ZSSD residual, the deformed image's interpolated FD gradients): dgamma/dp ~ (2 / |d - dbar|^2) q.
