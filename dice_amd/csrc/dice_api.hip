@@ -86,7 +86,8 @@ struct dice_hip_context {
   // autotuner sizes its LDS window candidates from them
   int max_side = 0, p90_side = 0;
   int max_count = 0;  // most pixels in one subset
-  DevBuf cx, cy, off, px, py, act, bbox, nact;
+  DevBuf cx, cy, off, px, py, act, bbox, nact, pxy, abox;
+  bool packed = false;  // conformal pixels also packed into pxy
   // fields + work
   DevBuf fields, work;
   // launch shape of the batched mode
@@ -203,7 +204,9 @@ void fill_views(dice_hip_context *ctx, DImage &ref, DImage &def, DSubsets &ss) {
                 ctx->S > 0 ? nullptr : ctx->py.as<int>(),
                 ctx->S > 0 ? nullptr : ctx->act.as<uint8_t>(),
                 ctx->S > 0 ? nullptr : ctx->bbox.as<int4>(),
-                ctx->S > 0 ? nullptr : ctx->nact.as<int>()};
+                ctx->S > 0 ? nullptr : ctx->nact.as<int>(),
+                ctx->S > 0 || !ctx->packed ? nullptr : ctx->pxy.as<uint32_t>(),
+                ctx->S > 0 ? nullptr : ctx->abox.as<int4>()};
 }
 
 Launch current_launch(dice_hip_context *ctx, int mode) {
@@ -460,22 +463,37 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     if (!conf_x || !conf_y) return ctx->fail(DICE_HIP_EINVAL, "conformal subsets need pixel lists");
     const int total = conf_offsets[n];
     if (conf_offsets[0] != 0 || total < 1) return ctx->fail(DICE_HIP_EINVAL, "bad conformal offsets");
-    std::vector<int4> bb(n);
+    std::vector<int4> bb(n), ab(n);
     std::vector<int> na(n), sides(n);
+    // pixels packed into one word each (one load per pixel per pass) when every
+    // coordinate fits 15 bits
+    std::vector<uint32_t> pk(total);
+    bool packed = true;
     for (int i = 0; i < n; ++i) {
       const int b = conf_offsets[i], e = conf_offsets[i + 1];
       if (e <= b) return ctx->fail(DICE_HIP_EINVAL, "empty conformal subset");
       int4 box{1 << 30, 1 << 30, -(1 << 30), -(1 << 30)};
+      int4 abx = box;
       int cnt = 0;
       for (int k = b; k < e; ++k) {
         const int x = conf_x[k] < 0 ? 0 : conf_x[k], y = conf_y[k] < 0 ? 0 : conf_y[k];
+        const bool on = conf_active ? (conf_active[k] != 0) : true;
         box.x = std::min(box.x, x);
         box.y = std::min(box.y, y);
         box.z = std::max(box.z, x);
         box.w = std::max(box.w, y);
-        cnt += conf_active ? (conf_active[k] != 0) : 1;
+        if (on) {
+          abx.x = std::min(abx.x, x);
+          abx.y = std::min(abx.y, y);
+          abx.z = std::max(abx.z, x);
+          abx.w = std::max(abx.w, y);
+        }
+        cnt += on ? 1 : 0;
+        if (x >= (1 << 15) || y >= (1 << 15)) packed = false;
+        pk[k] = (uint32_t)(x & 0x7fff) | ((uint32_t)(y & 0x7fff) << 15) | (on ? 0u : 1u << 30);
       }
       bb[i] = box;
+      ab[i] = cnt > 0 ? abx : box;
       na[i] = cnt;
       sides[i] = std::max(box.z - box.x, box.w - box.y) + 1;
     }
@@ -499,6 +517,14 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     HIP_TRY(ctx, hipMemcpy(ctx->act.p, a.data(), total, hipMemcpyHostToDevice), "hipMemcpy");
     HIP_TRY(ctx, hipMemcpy(ctx->bbox.p, bb.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
     HIP_TRY(ctx, hipMemcpy(ctx->nact.p, na.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(ctx, ctx->abox.ensure(n * sizeof(int4)), "hipMalloc subsets");
+    HIP_TRY(ctx, hipMemcpy(ctx->abox.p, ab.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
+    ctx->packed = packed;
+    if (packed) {
+      HIP_TRY(ctx, ctx->pxy.ensure((size_t)total * sizeof(uint32_t)), "hipMalloc subsets");
+      HIP_TRY(ctx, hipMemcpy(ctx->pxy.p, pk.data(), (size_t)total * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "hipMemcpy");
+    }
     ctx->S = 0;
   }
   ctx->n = n;

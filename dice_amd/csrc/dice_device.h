@@ -45,6 +45,7 @@ typedef __attribute__((address_space(1))) const float gfloat;
 typedef __attribute__((address_space(1))) const uint16_t gu16;
 typedef __attribute__((address_space(1))) const int gint;
 typedef __attribute__((address_space(1))) const uint8_t gu8;
+typedef __attribute__((address_space(1))) const uint32_t gu32;
 
 // ------------------------------------------------------------ utilities ---
 // Cross-lane movement without LDS addresses: DPP within rows of 16 lanes and
@@ -466,6 +467,8 @@ struct Solver {
   const gint *px;
   const gint *py;
   const gu8 *pact;
+  const gu32 *pk;       // conformal pixels packed (x | y << 15 | inactive << 30), or null
+  int abx0, aby0, abx1, aby1;  // conformal: bbox of the active pixels
   float meanF;
   float en[6];  // 1 for enabled parameters
 
@@ -496,17 +499,24 @@ struct Solver {
       n_def_active = n;
       px = py = nullptr;
       pact = nullptr;
+      pk = nullptr;
     } else {
       const int off = ss.off[i];
       n = ss.off[i + 1] - off;
       px = (const gint *)ss.px + off;
       py = (const gint *)ss.py + off;
       pact = ss.act ? (const gu8 *)ss.act + off : nullptr;
+      pk = ss.pxy ? (const gu32 *)ss.pxy + off : nullptr;
       const int4 b = ss.bbox[i];
       bx0 = b.x;
       by0 = b.y;
       bx1 = b.z;
       by1 = b.w;
+      const int4 ab = ss.abox[i];
+      abx0 = ab.x;
+      aby0 = ab.y;
+      abx1 = ab.z;
+      aby1 = ab.w;
       n_def_active = ss.n_active_def[i];
     }
     ref_in = bx0 - ref.ox >= 0 && by0 - ref.oy >= 0 && bx1 - ref.ox < ref.w && by1 - ref.oy < ref.h;
@@ -528,6 +538,15 @@ struct Solver {
         col += dcol;
         row += drow;
         if (col >= S) { col -= S; ++row; }
+      }
+    } else if (pk) {
+      // one packed word per pixel; the next one is loaded before this pixel's work
+      uint32_t next = r0 < n ? pk[r0] : 0u;
+#pragma unroll 1
+      for (int i = r0; i < n; i += G::kSize) {
+        const uint32_t w = next;
+        if (i + G::kSize < n) next = pk[i + G::kSize];
+        f((int)(w & 0x7fffu), (int)((w >> 15) & 0x7fffu), (w >> 30) == 0u);
       }
     } else {
 #pragma unroll 1
@@ -1741,10 +1760,12 @@ struct Solver {
   // last warp (parameters p_state) and the final parameters A.
   __device__ __forceinline__ float sigma(const float p_state[6], const Affine &A, float &noise) {
     if constexpr (kFast) {
-      if (last_fast && S > 0) {
-        // the last warp kept every pixel of the square subset: the gradient
-        // sums are sum(phi phi^T)[0][0] and [1][1], the kept bbox is the bbox
-        noise = noise_std_dev(bx0, bx1, by0, by1, A.p[0], A.p[1]);
+      if (last_fast && (S > 0 || n_def_active > 0)) {
+        // the last warp kept every active pixel: the gradient sums are
+        // sum(phi phi^T)[0][0] and [1][1], the kept bbox is the bbox of the
+        // active pixels (with the centroid, where the general path starts)
+        if (S > 0) noise = noise_std_dev(bx0, bx1, by0, by1, A.p[0], A.p[1]);
+        else noise = noise_std_dev(min(cx, abx0), max(cx, abx1), min(cy, aby0), max(cy, aby1), A.p[0], A.p[1]);
         if (noise < 0.0f) return -1.0f;
         const float sgx = (float)uniform(pers[0]), sgy = (float)uniform(pers[7]);
         const float sg = sgx > sgy ? sgy : sgx;

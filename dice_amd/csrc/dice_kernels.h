@@ -69,6 +69,9 @@ struct DSubsets {
   const uint8_t *act;          // per-pixel active flag (conformal, may be null)
   const int4 *bbox;            // per-subset pixel-coordinate bbox (min_x, min_y, max_x, max_y)
   const int *n_active_def;     // per-subset count of defined-active pixels (conformal)
+  const uint32_t *pxy;         // conformal, may be null: pixel i packed as x | y << 15 | inactive << 30
+                               // (coordinates clamped at 0, below 2^15)
+  const int4 *abox;            // conformal: bbox of the active pixels (of all pixels if none is active)
 };
 
 // BATCHED Gauss-Newton reduction: kGnSums per-lane partials (q 6, H upper
