@@ -1480,6 +1480,10 @@ struct Solver {
       keys_weights(u - (float)fu, wxu);
       keys_weights(v - (float)fv, wyu);
     }
+    bool e_in_win = false;
+    if constexpr (kIsWave)
+      e_in_win = wx0 <= bx0 - ref.ox && wx0 + wpitch > bx1 - ref.ox && wy0 <= by0 - ref.oy &&
+                 wy0 + wrows > by1 - ref.oy;
     float p1 = 0.0f, p3 = 0.0f, p4 = 0.0f;  // FP32 lane partials of sum ug^2/g2, jg^2/g2, (sub_r - sub_e)^2/g2
     for_pixels([&](int x, int y, bool a) {
       (void)a;
@@ -1502,7 +1506,16 @@ struct Solver {
         sub_r = keys_single(Pix<T>::ptr(def.pix), def.w, 0, 0, (float)rx + u, (float)ry + v) - refv;
       }
       const bool inb = rx >= 0 && ry >= 0 && rx <= def.w - 2 && ry <= def.h - 2;
-      const float sub_e = (inb ? Pix<T>::at(def.pix, ry * def.w + rx) : 0.0f) - refv;
+      float e = 0.0f;
+      if (inb) {
+        if constexpr (kIsWave) {
+          // the unshifted pixel from the window when the window holds the subset bbox
+          e = e_in_win ? (float)win[__umul24(ry - wy0, wpitch) + (rx - wx0)] : Pix<T>::at(def.pix, ry * def.w + rx);
+        } else {
+          e = Pix<T>::at(def.pix, ry * def.w + rx);
+        }
+      }
+      const float sub_e = e - refv;
       p1 = fmaf(ug * ug, inv, p1);
       p3 = fmaf(jg * jg, inv, p3);
       p4 = fmaf((sub_r - sub_e) * (sub_r - sub_e), inv, p4);
