@@ -2,8 +2,12 @@
 #include "dice_schema.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstring>
+#include <fstream>
 #include <set>
+#include <sstream>
 
 namespace dice_hip {
 
@@ -130,6 +134,283 @@ void Polygon::deactivate_pixels(int size, std::vector<uint8_t> &flags, const int
     if (inside(x[j], y[j])) flags[j] = 0;
 }
 
+Circle::Circle(int centroid_x, int centroid_y, double radius) : cx_(centroid_x), cy_(centroid_y) {
+  if (centroid_x <= 0 || centroid_y <= 0 || !(radius > 0.0))
+    throw std::invalid_argument("Error, a circle needs a positive centroid and radius");
+  radius2_ = radius * radius;
+  // integer members initialised from the floating-point extents
+  min_x_ = (int)(cx_ - radius - 1);
+  max_x_ = (int)(cx_ + radius + 1);
+  min_y_ = (int)(cy_ - radius - 1);
+  max_y_ = (int)(cy_ + radius + 1);
+}
+
+std::vector<std::pair<int, int>> Circle::get_owned_pixels() const {
+  std::vector<std::pair<int, int>> out;
+  for (int y = min_y_; y <= max_y_; ++y)
+    for (int x = min_x_; x <= max_x_; ++x) {
+      const double dx = (double)(x - cx_) * (x - cx_), dy = (double)(y - cy_) * (y - cy_);
+      if (dx + dy <= radius2_) out.emplace_back(y, x);
+    }
+  return out;
+}
+
+void Circle::deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const {
+  for (int j = 0; j < size; ++j) {
+    const double dx = (double)(x[j] - cx_) * (x[j] - cx_), dy = (double)(y[j] - cy_) * (y[j] - cy_);
+    if (dx + dy <= radius2_) flags[j] = 0;
+  }
+}
+
+Rectangle::Rectangle(int centroid_x, int centroid_y, int width, int height)
+    : origin_x_(centroid_x - width / 2), origin_y_(centroid_y - height / 2), width_(width), height_(height) {
+  if (centroid_x <= 0 || centroid_y <= 0 || width <= 0 || height <= 0 || origin_x_ < 0 || origin_y_ < 0)
+    throw std::invalid_argument("Error, a rectangle needs a positive size and must lie at non-negative coordinates");
+}
+
+std::vector<std::pair<int, int>> Rectangle::get_owned_pixels() const {
+  std::vector<std::pair<int, int>> out;
+  out.reserve((size_t)width_ * height_);
+  for (int y = 0; y < height_; ++y)
+    for (int x = 0; x < width_; ++x) out.emplace_back(origin_y_ + y, origin_x_ + x);
+  return out;
+}
+
+void Rectangle::deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const {
+  for (int j = 0; j < size; ++j)
+    if (x[j] >= origin_x_ && x[j] < origin_x_ + width_ && y[j] >= origin_y_ && y[j] < origin_y_ + height_) flags[j] = 0;
+}
+
+namespace {
+
+// tokenize_line (DICe_Parser.cpp:1046-1084): blank-delimited, capitalised
+// tokens; a line whose first token starts with '#' is empty.
+std::vector<std::string> tokenize(const std::string &line, const char *delim = " \t\r") {
+  std::vector<std::string> out;
+  std::string buf = line;
+  for (char *t = std::strtok(&buf[0], delim); t; t = std::strtok(nullptr, delim)) {
+    std::string tok(t);
+    for (char &c : tok) c = (char)std::toupper((unsigned char)c);
+    out.push_back(tok);
+  }
+  if (!out.empty() && out[0][0] == '#') out.clear();
+  return out;
+}
+
+// is_number (DICe_Parser.cpp:1088-1094)
+bool is_number(const std::string &s) {
+  if (s.empty()) return false;
+  for (char c : s)
+    if (!(std::isdigit((unsigned char)c) || c == '+' || c == '-' || c == 'e' || c == 'E' || c == '.')) return false;
+  return true;
+}
+
+struct LineReader {
+  std::ifstream in;
+  std::string name;
+  bool next(std::vector<std::string> &tok) {
+    std::string line;
+    while (std::getline(in, line)) {
+      tok = tokenize(line);
+      if (!tok.empty()) return true;
+    }
+    return false;
+  }
+  [[noreturn]] void fail(const std::string &what) const {
+    throw std::runtime_error("Error parsing subset file " + name + ": " + what);
+  }
+};
+
+int to_int(const LineReader &r, const std::string &s) {
+  if (!is_number(s)) r.fail("expected a number, found " + s);
+  return (int)std::strtol(s.c_str(), nullptr, 0);
+}
+
+std::shared_ptr<Shape> read_polygon(LineReader &r) {
+  std::vector<int> vx, vy;
+  std::vector<std::string> t;
+  while (r.next(t)) {
+    if (t[0] == "END") break;
+    if (t.size() < 2 || t[0] != "BEGIN" || t[1] != "VERTICES") r.fail("expected BEGIN VERTICES in a polygon");
+    while (r.next(t)) {
+      if (t[0] == "END") break;
+      if (t.size() < 2) r.fail("a vertex needs x and y");
+      vx.push_back(to_int(r, t[0]));
+      vy.push_back(to_int(r, t[1]));
+    }
+  }
+  if (vx.empty()) r.fail("polygon without vertices");
+  return std::make_shared<Polygon>(vx, vy);
+}
+
+std::shared_ptr<Shape> read_circle(LineReader &r) {
+  int cx = -1, cy = -1;
+  double radius = -1.0;
+  std::vector<std::string> t;
+  while (r.next(t)) {
+    if (t[0] == "END") break;
+    if (t[0] == "CENTER") {
+      if (t.size() < 3) r.fail("not enough values to define circle center point");
+      cx = to_int(r, t[1]);
+      cy = to_int(r, t[2]);
+    } else if (t[0] == "RADIUS") {
+      if (t.size() < 2 || !is_number(t[1])) r.fail("not enough values to define circle radius");
+      radius = std::strtod(t[1].c_str(), nullptr);
+    } else {
+      r.fail("invalid token in circle definition " + t[0]);
+    }
+  }
+  if (radius == -1.0 || cx == -1 || cy == -1) r.fail("incomplete circle");
+  return std::make_shared<Circle>(cx, cy, radius);
+}
+
+std::shared_ptr<Shape> read_rectangle(LineReader &r) {
+  int cx = -1, cy = -1, width = -1, height = -1, ulx = -1, uly = -1, lrx = -1, lry = -1;
+  bool corners = false;
+  std::vector<std::string> t;
+  while (r.next(t)) {
+    if (t[0] == "END") break;
+    if (t[0] == "CENTER") {
+      if (t.size() < 3) r.fail("not enough values to define rectangle center point");
+      cx = to_int(r, t[1]);
+      cy = to_int(r, t[2]);
+    } else if (t[0] == "WIDTH" || t[0] == "HEIGHT") {
+      if (t.size() < 2 || !is_number(t[1])) r.fail("not enough values to define the " + t[0]);
+      (t[0] == "WIDTH" ? width : height) = (int)std::strtod(t[1].c_str(), nullptr);
+    } else if (t[0] == "UPPER_LEFT" || t[0] == "LOWER_RIGHT") {
+      if (t.size() < 3) r.fail("not enough values to define rectangle " + t[0]);
+      corners = true;
+      (t[0] == "UPPER_LEFT" ? ulx : lrx) = to_int(r, t[1]);
+      (t[0] == "UPPER_LEFT" ? uly : lry) = to_int(r, t[2]);
+    } else {
+      r.fail("invalid token in rectangle definition " + t[0]);
+    }
+  }
+  if (corners) {
+    if (ulx < 0 || uly < 0 || lrx < 0 || lry < 0) r.fail("rectangle corners must be non-negative");
+    if (lrx < ulx || lry < uly) r.fail("Rectangle inverted or zero width");
+    width = lrx - ulx;
+    height = lry - uly;
+    cx = width / 2 + ulx;
+    cy = height / 2 + uly;
+  }
+  if (width <= 0 || height <= 0 || cx < 0 || cy < 0) r.fail("incomplete rectangle");
+  return std::make_shared<Rectangle>(cx, cy, width, height);
+}
+
+// read_shapes (DICe_Parser.cpp:1832-1870): BEGIN POLYGON / CIRCLE / RECTANGLE blocks until END
+multi_shape read_shapes(LineReader &r) {
+  multi_shape shapes;
+  std::vector<std::string> t;
+  while (r.next(t)) {
+    if (t[0] == "END") break;
+    if (t.size() < 2 || t[0] != "BEGIN") r.fail("expected BEGIN <shape>");
+    if (t[1] == "POLYGON") shapes.push_back(read_polygon(r));
+    else if (t[1] == "CIRCLE") shapes.push_back(read_circle(r));
+    else if (t[1] == "RECTANGLE") shapes.push_back(read_rectangle(r));
+    else r.fail("unknown shape " + t[1]);
+  }
+  return shapes;
+}
+
+}  // namespace
+
+Subset_File_Info read_subset_file(const std::string &file_name, int width, int height) {
+  LineReader r;
+  r.name = file_name;
+  r.in.open(file_name);
+  if (!r.in.good()) throw std::runtime_error("Error, the subset file does not exist: " + file_name);
+  Subset_File_Info info;
+  std::vector<std::string> t;
+  while (r.next(t)) {
+    if (t.size() < 2 || t[0] != "BEGIN") r.fail("invalid entry " + t[0]);
+    if (t[1] == "SUBSET_COORDINATES") {
+      while (r.next(t)) {
+        if (t[0] == "END") break;
+        if (!is_number(t[0])) r.fail("error parsing subset coordinates: " + t[0]);
+        if (t.size() < 2 || !is_number(t[1])) r.fail("invalid coordinate (not enough values)");
+        const double x = std::strtod(t[0].c_str(), nullptr), y = std::strtod(t[1].c_str(), nullptr);
+        if (x < 0 || (width != -1 && x >= width)) r.fail("invalid subset coordinate x: " + t[0]);
+        if (y < 0 || (height != -1 && y >= height)) r.fail("invalid subset coordinate y: " + t[1]);
+        info.coordinates_x.push_back(x);
+        info.coordinates_y.push_back(y);
+      }
+    } else if (t[1] == "CONFORMAL_SUBSET") {
+      int id = -1;
+      Conformal_Area_Def def;
+      std::vector<int> blocking;
+      bool force_simplex = false;
+      while (r.next(t)) {
+        if (t[0] == "END") break;
+        if (t[0] == "SUBSET_ID") {
+          if (t.size() < 2) r.fail("SUBSET_ID needs a value");
+          id = to_int(r, t[1]);
+        } else if (t[0] == "FORCE_SIMPLEX") {
+          force_simplex = true;
+        } else if (t[0] == "BEGIN" && t.size() >= 2) {
+          if (t[1] == "BOUNDARY") {
+            def.boundary = read_shapes(r);
+            if (def.boundary.empty()) r.fail("cannot define a boundary with zero shapes");
+          } else if (t[1] == "EXCLUDED") {
+            def.excluded_area = read_shapes(r);
+            if (def.excluded_area.empty()) r.fail("cannot define an exclusion with zero shapes");
+          } else if (t[1] == "OBSTRUCTED") {
+            def.obstructed_area = read_shapes(r);
+            if (def.obstructed_area.empty()) r.fail("cannot define an obstruction with zero shapes");
+          } else if (t[1] == "BLOCKING_SUBSETS") {
+            while (r.next(t)) {
+              if (t[0] == "END") break;
+              blocking.push_back(to_int(r, t[0]));
+            }
+          } else {
+            r.fail(t[1] + " blocks are not supported by this path");
+          }
+        } else {
+          r.fail(t[0] + " is not supported by this path");
+        }
+      }
+      if (id < 0) r.fail("invalid subset id for conformal subset (SUBSET_ID <id> missing)");
+      if (def.boundary.empty()) r.fail("conformal subset without a boundary");
+      info.conformal_area_defs[id] = def;
+      info.blocking_subset_ids[id] = blocking;
+      if (force_simplex) info.force_simplex.push_back(id);
+    } else {
+      r.fail(t[1] + " blocks are not supported by this path");
+    }
+  }
+  return info;
+}
+
+namespace {
+const std::vector<std::pair<Field_Spec, const char *>> &field_names() {
+  static const std::vector<std::pair<Field_Spec, const char *>> v = {
+      {SUBSET_COORDINATES_X_FS, "COORDINATE_X"}, {SUBSET_COORDINATES_Y_FS, "COORDINATE_Y"},
+      {SUBSET_DISPLACEMENT_X_FS, "DISPLACEMENT_X"}, {SUBSET_DISPLACEMENT_Y_FS, "DISPLACEMENT_Y"},
+      {ROTATION_Z_FS, "ROTATION_Z"}, {NORMAL_STRETCH_XX_FS, "NORMAL_STRAIN_X"},
+      {NORMAL_STRETCH_YY_FS, "NORMAL_STRAIN_Y"}, {SHEAR_STRETCH_XY_FS, "SHEAR_STRAIN_XY"},
+      {SIGMA_FS, "SIGMA"}, {GAMMA_FS, "GAMMA"}, {BETA_FS, "BETA"}, {MATCH_FS, "MATCH"},
+      {NOISE_LEVEL_FS, "NOISE_LEVEL"}, {CONTRAST_LEVEL_FS, "CONTRAST_LEVEL"}, {ACTIVE_PIXELS_FS, "ACTIVE_PIXELS"},
+      {STATUS_FLAG_FS, "STATUS_FLAG"}, {ITERATIONS_FS, "ITERATIONS"}, {CONDITION_NUMBER_FS, "CONDITION_NUMBER"},
+      {NEIGHBOR_ID_FS, "NEIGHBOR_ID"}, {FIELD_1_FS, "FIELD_1"}, {FIELD_2_FS, "FIELD_2"}, {FIELD_3_FS, "FIELD_3"},
+      {FIELD_4_FS, "FIELD_4"}, {FIELD_5_FS, "FIELD_5"}, {FIELD_6_FS, "FIELD_6"}, {FIELD_7_FS, "FIELD_7"}};
+  return v;
+}
+}  // namespace
+
+const char *field_name(Field_Spec fs) {
+  for (const auto &e : field_names())
+    if (e.first == fs) return e.second;
+  return "";
+}
+
+Field_Spec field_spec_from_name(const std::string &name) {
+  std::string up = name;
+  for (char &c : up) c = (char)std::toupper((unsigned char)c);
+  for (const auto &e : field_names())
+    if (up == e.second) return e.first;
+  throw std::invalid_argument("Error, the output field " + name + " is not produced by this path");
+}
+
 Schema::Schema(const ParameterList *params, int device) {
   create(device);
   if (params) set_params(*params);
@@ -154,6 +435,8 @@ Schema::Schema(int roi_width, int roi_height, int step_size_x, int step_size_y, 
   create(device);
   if (params) set_params(*params);
   init_subsets(cx, cy, subset_size, nullptr);
+  step_x_ = step_size_x;
+  step_y_ = step_size_y;
 }
 
 Schema::Schema(const std::vector<double> &coords_x, const std::vector<double> &coords_y, int subset_size,
@@ -208,6 +491,22 @@ void Schema::set_params(const ParameterList &params) {
   backend_ = backend;
 }
 
+void conformal_subset_pixels(const Conformal_Area_Def &def, std::vector<int> &x, std::vector<int> &y,
+                             std::vector<uint8_t> &active) {
+  if (def.boundary.empty()) throw std::runtime_error("Error, conformal subset without a boundary");
+  std::set<std::pair<int, int>> coords;  // (y, x): y-major order
+  for (const auto &shape : def.boundary)
+    for (const auto &yx : shape->get_owned_pixels()) coords.insert(yx);
+  x.clear();
+  y.clear();
+  for (const auto &yx : coords) {
+    x.push_back(yx.second < 0 ? 0 : yx.second);
+    y.push_back(yx.first < 0 ? 0 : yx.first);
+  }
+  active.assign(x.size(), 1);
+  for (const auto &shape : def.excluded_area) shape->deactivate_pixels((int)x.size(), active, x.data(), y.data());
+}
+
 void Schema::init_subsets(const std::vector<double> &cx, const std::vector<double> &cy, int subset_size,
                           const std::map<int, Conformal_Area_Def> *defs) {
   const int n = (int)cx.size();
@@ -226,20 +525,12 @@ void Schema::init_subsets(const std::vector<double> &cx, const std::vector<doubl
     std::vector<uint8_t> act;
     for (int i = 0; i < n; ++i) {
       auto it = defs->find(i);
-      const size_t start = px.size();
       if (it != defs->end()) {
-        if (it->second.boundary.empty()) throw std::runtime_error("Error, conformal subset without a boundary");
-        std::set<std::pair<int, int>> coords;  // (y, x): y-major order
-        for (const Polygon &poly : it->second.boundary)
-          for (const auto &yx : poly.get_owned_pixels()) coords.insert(yx);
-        for (const auto &yx : coords) {
-          px.push_back(yx.second < 0 ? 0 : yx.second);
-          py.push_back(yx.first < 0 ? 0 : yx.first);
-        }
-        const int cnt = (int)(px.size() - start);
-        std::vector<uint8_t> flags(cnt, 1);
-        for (const Polygon &poly : it->second.excluded_area)
-          poly.deactivate_pixels(cnt, flags, px.data() + start, py.data() + start);
+        std::vector<int> sx, sy;
+        std::vector<uint8_t> flags;
+        conformal_subset_pixels(it->second, sx, sy, flags);
+        px.insert(px.end(), sx.begin(), sx.end());
+        py.insert(py.end(), sy.begin(), sy.end());
         act.insert(act.end(), flags.begin(), flags.end());
       } else {
         if (subset_size <= 0) throw std::runtime_error("Error, square subsets need a subset size");
@@ -301,6 +592,176 @@ int Schema::execute_correlation() {
 double &Schema::local_field_value(int local_id, Field_Spec fs) {
   if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
   return fields_[(size_t)fs * n_ + local_id];
+}
+
+double Schema::local_field_value(int local_id, Field_Spec fs) const {
+  if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
+  return fields_[(size_t)fs * n_ + local_id];
+}
+
+namespace {
+const char *interp_name(int m) { return m == 0 ? "BILINEAR" : m == 1 ? "BICUBIC" : m == 2 ? "KEYS_FOURTH" : "UNKNOWN"; }
+const char *opt_name(int m) {
+  static const char *n[] = {"SIMPLEX", "GRADIENT_BASED", "GRADIENT_THEN_SEARCH", "SIMPLEX_THEN_GRADIENT_BASED",
+                            "GRADIENT_BASED_THEN_SIMPLEX"};
+  return m >= 0 && m < 5 ? n[m] : "UNKNOWN";
+}
+int digits(int v) {
+  int d = 0;
+  if (v == 0) return 1;
+  while (v) { v /= 10; ++d; }
+  return d;
+}
+}  // namespace
+
+Output_Spec::Output_Spec(const Schema *schema, bool omit_row_id, const std::vector<std::string> &fields,
+                         const std::string &delimiter)
+    : schema_(schema), omit_row_id_(omit_row_id), delimiter_(delimiter) {
+  if (fields.empty()) {
+    // DICe_Schema.cpp:4683-4696 with the affine shape function
+    names_ = {"COORDINATE_X", "COORDINATE_Y", "DISPLACEMENT_X", "DISPLACEMENT_Y", "ROTATION_Z",
+              "NORMAL_STRAIN_X", "NORMAL_STRAIN_Y", "SHEAR_STRAIN_XY", "SIGMA", "STATUS_FLAG"};
+  } else {
+    names_ = fields;
+  }
+  for (std::string &n : names_) {
+    for (char &c : n) c = (char)std::toupper((unsigned char)c);
+    specs_.push_back(field_spec_from_name(n));
+  }
+}
+
+void Output_Spec::write_info(std::FILE *file) const {
+  // Output_Spec::write_info (DICe_Schema.cpp:4795-4856) for a local analysis;
+  // the first line is the masthead, which the comparison skips
+  const Schema &s = *schema_;
+  std::fprintf(file, "***\n");
+  std::fprintf(file, "*** Digital Image Correlation Engine (DICe) format, written by the MI355X subset-correlation library\n");
+  std::fprintf(file, "***\n");
+  std::fprintf(file, "*** Reference image: %s \n", s.ref_image_file_name().c_str());
+  std::fprintf(file, "*** Deformed image: %s \n", s.def_image_file_name().c_str());
+  std::fprintf(file, "*** DIC method : local \n");
+  std::fprintf(file, "*** Correlation method: ZNSSD\n");
+  std::fprintf(file, "*** Interpolation method: %s\n", interp_name(s.interpolation_method()));
+  std::fprintf(file, "*** Image gradient method: %s\n",
+               s.gradient_method() == CONVOLUTION_5_POINT ? "CONVOLUTION_5_POINT" : "FINITE_DIFFERENCE");
+  std::fprintf(file, "*** Optimization method: %s\n", opt_name(s.optimization_method()));
+  std::fprintf(file, "*** Projection method: DISPLACEMENT_BASED\n");
+  std::fprintf(file, "*** Guess initialization method: USE_FIELD_VALUES\n");
+  std::fprintf(file, "*** Seed location: N/A\n");
+  std::fprintf(file, "*** Shape functions: ");
+  if (s.translation_enabled()) std::fprintf(file, "Translation (u,v) ");
+  if (s.rotation_enabled()) std::fprintf(file, "Rotation (theta) ");
+  if (s.normal_strain_enabled()) std::fprintf(file, "Normal Strain (ex,ey) ");
+  if (s.shear_strain_enabled()) std::fprintf(file, "Shear Strain (gamma_xy) ");
+  std::fprintf(file, "\n");
+  std::fprintf(file, "*** Incremental correlation: false\n");
+  std::fprintf(file, "*** Subset size: %i\n", s.subset_dim());
+  std::fprintf(file, "*** Step size: x %i y %i (-1 implies not regular grid)\n", s.step_size_x(), s.step_size_y());
+  std::fprintf(file, "*** Strain window: N/A\n");
+  std::fprintf(file, "*** Coordinates given with (0,0) as upper left corner of image, x positive right, y positive down\n");
+  std::fprintf(file, "***\n");
+}
+
+void Output_Spec::write_header(std::FILE *file, const std::string &row_id) const {
+  if (!omit_row_id_) std::fprintf(file, "%s%s", row_id.c_str(), delimiter_.c_str());
+  for (size_t i = 0; i < names_.size(); ++i)
+    std::fprintf(file, "%s%s", i == 0 ? "" : delimiter_.c_str(), names_[i].c_str());
+  std::fprintf(file, "\n");
+}
+
+void Output_Spec::write_frame(std::FILE *file, int row_index, int subset) const {
+  // values in the reference's scalar_t (float) with %4.4E
+  if (!omit_row_id_) std::fprintf(file, "%i%s", row_index, delimiter_.c_str());
+  for (size_t i = 0; i < specs_.size(); ++i) {
+    const float v = (float)schema_->local_field_value(subset, specs_[i]);
+    std::fprintf(file, "%s%4.4E", i == 0 ? "" : delimiter_.c_str(), (double)v);
+  }
+  std::fprintf(file, "\n");
+}
+
+void Schema::write_output(const std::string &output_folder, const std::string &prefix,
+                          bool separate_files_per_subset, const std::vector<std::string> &fields,
+                          const std::string &delimiter, bool omit_row_id, int num_frames) const {
+  if (n_ <= 0) throw std::runtime_error("Error, the schema has no subsets");
+  const Output_Spec spec(this, omit_row_id, fields, delimiter);
+  const int frame = frame_id_ - 1;  // written after execute_correlation advanced the frame
+  auto open = [](const std::string &name, const char *mode) {
+    std::FILE *f = std::fopen(name.c_str(), mode);
+    if (!f) throw std::runtime_error("Error, cannot write " + name);
+    return f;
+  };
+  if (separate_files_per_subset) {
+    const int total = digits(n_);
+    for (int i = 0; i < n_; ++i) {
+      std::string name = output_folder + prefix + "_" + std::string(total - digits(i), '0') + std::to_string(i) +
+                         ".txt";
+      if (frame == first_frame_id_) {
+        std::FILE *f = open(name, "w");
+        spec.write_info(f);
+        spec.write_header(f, "FRAME");
+        std::fclose(f);
+      }
+      std::FILE *f = open(name, "a");
+      spec.write_frame(f, frame, i);
+      std::fclose(f);
+    }
+  } else {
+    int zeros = 0;
+    if (num_frames > 0) zeros = std::max(0, digits(first_frame_id_ + num_frames) - digits(frame));
+    const std::string name = output_folder + prefix + "_" + std::string(zeros, '0') + std::to_string(frame) + ".txt";
+    std::FILE *f = open(name, "w");
+    spec.write_info(f);
+    spec.write_header(f, "SUBSET_ID");
+    for (int i = 0; i < n_; ++i) spec.write_frame(f, i, i);
+    std::fclose(f);
+  }
+}
+
+int diff_files(const std::string &file_a, const std::string &file_b, double rel_tol, double floor, bool use_floor,
+               bool numerical_only, std::string *report) {
+  std::ifstream a(file_a), b(file_b);
+  std::ostringstream rep;
+  int errors = 0;
+  if (!a.good() || !b.good()) {
+    if (report) *report = "Error, cannot read " + (a.good() ? file_b : file_a);
+    return 1;
+  }
+  std::string la, lb;
+  int line = 0;
+  while (std::getline(a, la)) {
+    ++line;
+    if (!std::getline(b, lb)) {
+      rep << "Error, File A has more lines than FileB\n";
+      ++errors;
+      break;
+    }
+    const std::vector<std::string> ta = tokenize(la, " ,\r"), tb = tokenize(lb, " ,\r");
+    if (ta.size() >= 2 && ta[1].find("DIGITAL") != std::string::npos) continue;  // the masthead
+    if (ta.size() != tb.size()) {
+      rep << "Error, Different number of tokens per line " << line << " A:" << ta.size() << " B: " << tb.size() << "\n";
+      ++errors;
+      break;
+    }
+    bool line_diff = false;
+    for (size_t i = 0; i < ta.size(); ++i) {
+      if (is_number(ta[i])) {
+        if (!is_number(tb[i])) { line_diff = true; continue; }
+        const double va = std::strtod(ta[i].c_str(), nullptr), vb = std::strtod(tb[i].c_str(), nullptr);
+        const double diff = std::abs((va - vb) / va);
+        const bool tiny = std::abs(va) + std::abs(vb) < 1.0e-8;
+        const bool below_floor = std::abs(va) < floor;
+        if ((!below_floor || !use_floor) && !tiny && diff > rel_tol) line_diff = true;
+      } else if (ta[i] != tb[i] && !numerical_only) {
+        line_diff = true;
+      }
+    }
+    if (line_diff) {
+      ++errors;
+      rep << "< " << line << ": " << la << "\n> " << line << ": " << lb << "\n";
+    }
+  }
+  if (report) *report = rep.str();
+  return errors;
 }
 
 }  // namespace dice_hip

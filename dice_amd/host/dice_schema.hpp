@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -86,15 +87,23 @@ class ParameterList {
 void dice_default_params(ParameterList *params);
 void tracking_default_params(ParameterList *params);
 
+// Shapes of conformal subsets (DICe_Shape.h:50-200): the pixels a shape owns,
+// as (y, x) pairs in (y, x) order (std::set<std::pair<int,int>> in the
+// reference), and clearing flags[j] for the pixels (x[j], y[j]) it covers.
+class Shape {
+ public:
+  virtual ~Shape() = default;
+  virtual std::vector<std::pair<int, int>> get_owned_pixels() const = 0;
+  virtual void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const = 0;
+};
+
 // Polygon with integer vertices (DICe_Shape.cpp:50-219): the pixels it owns
 // are those whose winding angle around the vertices is at least pi.
-class Polygon {
+class Polygon : public Shape {
  public:
   Polygon(const std::vector<int> &coords_x, const std::vector<int> &coords_y);
-  // (y, x) pairs in (y, x) order, as std::set<std::pair<int,int>> in the reference
-  std::vector<std::pair<int, int>> get_owned_pixels() const;
-  // clears flags[j] for the pixels (x[j], y[j]) inside the polygon
-  void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const;
+  std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
 
  private:
   bool inside(int x, int y) const;
@@ -102,12 +111,71 @@ class Polygon {
   int min_x_, min_y_, max_x_, max_y_;
 };
 
-// Conformal_Area_Def (DICe_Shape.h): union of boundary polygons, with excluded
-// areas whose pixels stay in the subset but are inactive.
-struct Conformal_Area_Def {
-  std::vector<Polygon> boundary;
-  std::vector<Polygon> excluded_area;
+// Circle (DICe_Shape.cpp:221-279): integer pixels with (x - cx)^2 + (y - cy)^2
+// <= radius^2, searched over [c - radius - 1, c + radius + 1].
+class Circle : public Shape {
+ public:
+  Circle(int centroid_x, int centroid_y, double radius);
+  std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
+
+ private:
+  int cx_, cy_;
+  double radius2_;
+  int min_x_, max_x_, min_y_, max_y_;
 };
+
+// Rectangle (DICe_Shape.cpp:281-420, undeformed): origin = centroid -
+// size / 2 (integer division), pixels [origin, origin + size) on each axis.
+class Rectangle : public Shape {
+ public:
+  Rectangle(int centroid_x, int centroid_y, int width, int height);
+  std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
+
+ private:
+  int origin_x_, origin_y_, width_, height_;
+};
+
+typedef std::vector<std::shared_ptr<Shape>> multi_shape;
+
+// Conformal_Area_Def (DICe_Shape.h): the union of the boundary shapes, with
+// excluded areas whose pixels stay in the subset but are inactive
+// (DICe_Subset.cpp:115-168).  Obstructed areas are kept for the caller; the
+// obstruction tracking of TRACKING_ROUTINE is not part of this path.
+struct Conformal_Area_Def {
+  multi_shape boundary;
+  multi_shape excluded_area;
+  multi_shape obstructed_area;
+};
+
+// The subset file of the reference (read_subset_file, DICe_Parser.cpp:1099-1700):
+// SUBSET_COORDINATES and CONFORMAL_SUBSET blocks (SUBSET_ID; BOUNDARY,
+// EXCLUDED and OBSTRUCTED blocks of POLYGON / CIRCLE / RECTANGLE shapes;
+// BLOCKING_SUBSETS; FORCE_SIMPLEX).  Keywords are case-insensitive, '#' starts
+// a comment.  The blocks this path has no use for (REGION_OF_INTEREST, seeds,
+// motion windows, paths, skip-solve lists) throw std::runtime_error naming
+// them.  width / height (or -1) bound the coordinates as in the reference.
+struct Subset_File_Info {
+  std::vector<double> coordinates_x, coordinates_y;
+  std::map<int, Conformal_Area_Def> conformal_area_defs;
+  std::map<int, std::vector<int>> blocking_subset_ids;
+  std::vector<int> force_simplex;
+};
+Subset_File_Info read_subset_file(const std::string &file_name, int width = -1, int height = -1);
+
+// The pixel list of a conformal subset (Subset::Subset(cx, cy, Conformal_Area_Def),
+// DICe_Subset.cpp:115-168): the union of the boundary shapes' pixels in (y, x)
+// order, negative coordinates clamped to 0, active unless an excluded shape
+// covers the pixel.
+void conformal_subset_pixels(const Conformal_Area_Def &def, std::vector<int> &x, std::vector<int> &y,
+                             std::vector<uint8_t> &active);
+
+// Output names of the Field_Specs (DICe_FieldEnums.cpp:94-171), e.g.
+// DISPLACEMENT_X for SUBSET_DISPLACEMENT_X_FS.  field_spec_from_name throws
+// std::invalid_argument for a name this path does not produce.
+const char *field_name(Field_Spec fs);
+Field_Spec field_spec_from_name(const std::string &name);
 
 class Schema {
  public:
@@ -136,6 +204,7 @@ class Schema {
   int execute_correlation();
 
   double &local_field_value(int local_id, Field_Spec fs);
+  double local_field_value(int local_id, Field_Spec fs) const;
   double &global_field_value(int global_id, Field_Spec fs) { return local_field_value(global_id, fs); }
   int local_num_subsets() const { return n_; }
   int global_num_subsets() const { return n_; }
@@ -147,7 +216,36 @@ class Schema {
   Correlation_Routine correlation_routine() const { return (Correlation_Routine)p_.correlation_routine; }
   Initialization_Method initialization_method() const { return (Initialization_Method)p_.initialization_method; }
   Correlation_Backend correlation_backend() const { return backend_; }
+  Gradient_Method gradient_method() const { return (Gradient_Method)p_.gradient_method; }
+  bool translation_enabled() const { return p_.enable_translation != 0; }
+  bool rotation_enabled() const { return p_.enable_rotation != 0; }
+  bool normal_strain_enabled() const { return p_.enable_normal_strain != 0; }
+  bool shear_strain_enabled() const { return p_.enable_shear_strain != 0; }
+  int step_size_x() const { return step_x_; }
+  int step_size_y() const { return step_y_; }
   dice_hip_context *context() { return ctx_; }
+
+  // File names written into the header block of the output (the reference
+  // takes them from its Image objects).
+  void set_image_file_names(const std::string &ref_name, const std::string &def_name) {
+    ref_name_ = ref_name;
+    def_name_ = def_name;
+  }
+  const std::string &ref_image_file_name() const { return ref_name_; }
+  const std::string &def_image_file_name() const { return def_name_; }
+
+  // Schema::write_output (DICe_Schema.cpp:4327-4470), text files only, called
+  // after execute_correlation: with separate_files_per_subset, one file
+  // <folder><prefix>_<id>.txt per subset (ids zero-padded to the digits of the
+  // subset count), the header block and the field names on the first frame,
+  // then one row per frame ("FRAME" column); otherwise one file per frame,
+  // <folder><prefix>_<frame>.txt (frame zero-padded to the digits of
+  // num_frames when num_frames > 0), with one row per subset ("SUBSET_ID").
+  // fields: output names (Output_Spec order); empty = the reference's default
+  // list.  Throws std::runtime_error if a file cannot be written.
+  void write_output(const std::string &output_folder, const std::string &prefix = "DICe_solution",
+                    bool separate_files_per_subset = false, const std::vector<std::string> &fields = {},
+                    const std::string &delimiter = " ", bool omit_row_id = false, int num_frames = 0) const;
 
  private:
   void create(int device);
@@ -165,8 +263,41 @@ class Schema {
   int frame_id_ = 0;
   int first_frame_id_ = 0;
   int ref_w_ = -1, ref_h_ = -1;
+  int step_x_ = -1, step_y_ = -1;
+  std::string ref_name_, def_name_;
   std::vector<double> fields_;
 };
+
+// Output_Spec (DICe_Schema.cpp:4673-4945, text output of a local analysis):
+// the header block, the field-name row and the %4.4E rows.
+class Output_Spec {
+ public:
+  // fields empty: COORDINATE_X, COORDINATE_Y, DISPLACEMENT_X, DISPLACEMENT_Y,
+  // ROTATION_Z, NORMAL_STRAIN_X, NORMAL_STRAIN_Y, SHEAR_STRAIN_XY, SIGMA,
+  // STATUS_FLAG (the affine shape function's default list)
+  Output_Spec(const Schema *schema, bool omit_row_id = false, const std::vector<std::string> &fields = {},
+              const std::string &delimiter = " ");
+  void write_info(std::FILE *file) const;
+  void write_header(std::FILE *file, const std::string &row_id) const;
+  void write_frame(std::FILE *file, int row_index, int subset) const;
+  const std::vector<std::string> &field_names() const { return names_; }
+
+ private:
+  const Schema *schema_;
+  bool omit_row_id_;
+  std::string delimiter_;
+  std::vector<std::string> names_;
+  std::vector<Field_Spec> specs_;
+};
+
+// DICe_Diff (tools/DICe_Diff.cpp:60-360, serial files): line by line, the
+// masthead line skipped; numbers differ when |(a - b) / a| > rel_tol, unless
+// |a| + |b| < 1e-8 or (use_floor and |a| < floor); strings must match unless
+// numerical_only.  Returns the number of differing lines (a line-count or
+// token-count mismatch, or a missing file, counts as one and stops); report,
+// when set, receives the differing lines.
+int diff_files(const std::string &file_a, const std::string &file_b, double rel_tol = 1.0e-6, double floor = 0.0,
+               bool use_floor = false, bool numerical_only = false, std::string *report = nullptr);
 
 }  // namespace dice_hip
 

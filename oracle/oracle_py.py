@@ -189,6 +189,28 @@ def polygon_pixels(vx, vy):
     return ox[:cnt], oy[:cnt]
 
 
+def circle_pixels(cx, cy, radius):
+    """Circle::get_owned_pixels (DICe_Shape.cpp:221-279): integer (x, y) with (x - cx)^2 + (y - cy)^2 <=
+    radius^2, searched over [c - radius - 1, c + radius + 1] (extents truncated to int).  Returns (x, y) in
+    (y, x) order."""
+    ys, xs = [], []
+    r2 = float(radius) * float(radius)
+    for y in range(int(cy - radius - 1), int(cy + radius + 1) + 1):
+        for x in range(int(cx - radius - 1), int(cx + radius + 1) + 1):
+            if float((x - cx) * (x - cx)) + float((y - cy) * (y - cy)) <= r2:
+                ys.append(y)
+                xs.append(x)
+    return np.array(xs, np.int32), np.array(ys, np.int32)
+
+
+def rectangle_pixels(cx, cy, width, height):
+    """Rectangle::get_owned_pixels without deformation (DICe_Shape.cpp:281-420): origin = centre - size // 2,
+    pixels [origin, origin + size) on each axis, in (y, x) order."""
+    ox, oy = cx - width // 2, cy - height // 2
+    yy, xx = np.mgrid[oy:oy + height, ox:ox + width]
+    return xx.ravel().astype(np.int32), yy.ravel().astype(np.int32)
+
+
 def lu_inverse(a):
     a = np.ascontiguousarray(a, dtype=np.float64).copy()
     info = lib().oracle_lu_inverse(a.shape[0], _p(a))

@@ -6,6 +6,9 @@
 //   test_schema correlate_conformal <ref.f32> <def.f32> <w> <h> <backend 0|1>
 //   test_schema parallel_subsets <ref.f32> <def.f32> <w> <h> <gold.txt> <backend 0|1>
 //   test_schema polygon            (prints the owned pixels "y x" of the golden's polygons)
+//   test_schema subset_file_info <subset file>   (what read_subset_file makes of it)
+//   test_schema subset_file_deck <ref.f32> <def.f32> <w> <h> <subset file> <subset size> <out dir>
+//                                <backend 0|1> <ref name> <def name> <routine 0|1>
 //   test_schema errors             (error behaviour that needs no GPU)
 //
 // Images are raw row-major float32.  The exit status is the number of failures.
@@ -107,7 +110,7 @@ int correlate_params(const std::vector<float> &ref, const std::vector<float> &de
 int correlate_conformal(const std::vector<float> &ref, const std::vector<float> &def, int w, int h, int backend) {
   // four sided shape at odd angles (DICe_TestCorrelateConformal.cpp:94-103)
   Conformal_Area_Def subset_def;
-  subset_def.boundary.emplace_back(std::vector<int>{25, 35, 39, 20}, std::vector<int>{22, 20, 40, 35});
+  subset_def.boundary.push_back(std::make_shared<Polygon>(std::vector<int>{25, 35, 39, 20}, std::vector<int>{22, 20, 40, 35}));
   std::map<int, Conformal_Area_Def> defs{{0, subset_def}};
   ParameterList params = all_shape_params(backend);
   Schema schema(std::vector<double>{30}, std::vector<double>{30}, -1, &defs, &params);
@@ -166,6 +169,52 @@ int polygon() {
   return 0;
 }
 
+// Prints what read_subset_file and conformal_subset_pixels make of a subset
+// file: "coord x y" per subset, then per conformal subset "conformal id n_pixels
+// n_active n_obstructed_shapes blocking..." and its pixels as "y x active".
+int subset_file_info(const char *path) {
+  const Subset_File_Info info = read_subset_file(path);
+  for (size_t i = 0; i < info.coordinates_x.size(); ++i)
+    std::cout << "coord " << info.coordinates_x[i] << " " << info.coordinates_y[i] << "\n";
+  for (const auto &kv : info.conformal_area_defs) {
+    std::vector<int> x, y;
+    std::vector<uint8_t> a;
+    conformal_subset_pixels(kv.second, x, y, a);
+    int na = 0;
+    for (uint8_t v : a) na += v;
+    std::cout << "conformal " << kv.first << " " << x.size() << " " << na << " " << kv.second.obstructed_area.size();
+    for (int b : info.blocking_subset_ids.at(kv.first)) std::cout << " " << b;
+    std::cout << "\n";
+    for (size_t k = 0; k < x.size(); ++k) std::cout << y[k] << " " << x[k] << " " << (int)a[k] << "\n";
+  }
+  return 0;
+}
+
+// A subset-file deck of the reference's regression suite through the Schema
+// mirror: subsets from the file, default parameters with the deck's
+// correlation routine, one frame, output written one file per subset.
+int subset_file_deck(const std::vector<float> &ref, const std::vector<float> &def, int w, int h, const char *subset_file,
+                     int subset_size, const std::string &out_dir, int backend, const char *ref_name,
+                     const char *def_name, int routine) {
+  const Subset_File_Info info = read_subset_file(subset_file, w, h);
+  ParameterList params;
+  dice_default_params(&params);
+  params.set("correlation_routine", routine);
+  params.set("correlation_backend", backend);
+  Schema schema(info.coordinates_x, info.coordinates_y, subset_size, &info.conformal_area_defs, &params);
+  schema.set_image_file_names(ref_name, def_name);
+  schema.set_ref_image(w, h, ref.data());
+  schema.set_def_image(w, h, def.data());
+  schema.execute_correlation();
+  schema.write_output(out_dir + "/", "DICe_solution", true);
+  for (int i = 0; i < schema.local_num_subsets(); ++i)
+    std::cout << "subset " << i << " u " << schema.local_field_value(i, SUBSET_DISPLACEMENT_X_FS) << " v "
+              << schema.local_field_value(i, SUBSET_DISPLACEMENT_Y_FS) << " sigma "
+              << schema.local_field_value(i, SIGMA_FS) << " status "
+              << schema.local_field_value(i, STATUS_FLAG_FS) << "\n";
+  return 0;
+}
+
 int errors_without_gpu() {
   int errors = 0;
   ParameterList params;
@@ -201,12 +250,16 @@ int main(int argc, char **argv) {
   try {
     if (cmd == "polygon") return polygon();
     if (cmd == "errors") return errors_without_gpu();
+    if (cmd == "subset_file_info" && argc >= 3) return subset_file_info(argv[2]);
     if (argc < 7) return 99;
     const int w = std::atoi(argv[4]), h = std::atoi(argv[5]);
     const std::vector<float> ref = read_raw(argv[2], w, h), def = read_raw(argv[3], w, h);
     if (cmd == "correlate_params") return correlate_params(ref, def, w, h, std::atoi(argv[6]));
     if (cmd == "correlate_conformal") return correlate_conformal(ref, def, w, h, std::atoi(argv[6]));
     if (cmd == "parallel_subsets" && argc >= 8) return parallel_subsets(ref, def, w, h, argv[6], std::atoi(argv[7]));
+    if (cmd == "subset_file_deck" && argc >= 13)
+      return subset_file_deck(ref, def, w, h, argv[6], std::atoi(argv[7]), argv[8], std::atoi(argv[9]), argv[10],
+                              argv[11], std::atoi(argv[12]));
   } catch (const std::exception &e) {
     std::cerr << "exception: " << e.what() << "\n";
     return 98;

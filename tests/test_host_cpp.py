@@ -121,3 +121,173 @@ def test_cpp_parallel_subsets_gold(tmp_path, backend):
     r = run("parallel_subsets", raw(tmp_path, "ref.f32", d["ref"]), raw(tmp_path, "def.f32", d["defm"]), w, h,
             os.path.join(GOLD, "parallel_subsets_gold.txt"), backend)
     assert r.returncode == 0
+
+
+# ---- subset files, output files and the regression deck tests/regression/conformal_subset_def ----
+
+DECK = os.path.join(GOLD, "conformal_subset_def")
+DIFF = os.path.join(ROOT, "dice_amd", "lib", "dice_diff")
+
+
+def subset_info(path):
+    """Parse test_schema subset_file_info: coordinates, and per conformal subset (pixels y x active, meta)."""
+    r = run("subset_file_info", path)
+    assert r.returncode == 0, r.stderr
+    coords, conf, cur = [], {}, None
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if t[0] == "coord":
+            coords.append((float(t[1]), float(t[2])))
+        elif t[0] == "conformal":
+            cur = int(t[1])
+            conf[cur] = {"n": int(t[2]), "active": int(t[3]), "obstructed": int(t[4]),
+                         "blocking": [int(v) for v in t[5:]], "pix": []}
+        else:
+            conf[cur]["pix"].append(tuple(int(v) for v in t))
+    return coords, conf
+
+
+def union_pixels(*shapes):
+    s = set()
+    for px, py in shapes:
+        s |= set(zip(py.tolist(), px.tolist()))
+    return sorted(s)
+
+
+def test_circle_and_rectangle_shapes(tmp_path):
+    """Circle pixel counts are the Gauss circle numbers N(r) = #{(x, y) : x^2 + y^2 <= r^2} (5, 13, 29, 81,
+    317 for r = 1, 2, 3, 5, 10); rectangles by centre/size and by corners; C++ reader and oracle agree."""
+    gauss = {1: 5, 2: 13, 3: 29, 5: 81, 10: 317}
+    lines = ["BEGIN SUBSET_COORDINATES"] + ["%d 50" % (20 + 30 * i) for i in range(len(gauss) + 2)] + \
+            ["END SUBSET_COORDINATES"]
+    for i, r in enumerate(gauss):
+        lines += ["BEGIN CONFORMAL_SUBSET", "SUBSET_ID %d" % i, "BEGIN BOUNDARY", "BEGIN CIRCLE",
+                  "CENTER %d 50" % (20 + 30 * i), "RADIUS %d" % r, "END CIRCLE", "END BOUNDARY", "END CONFORMAL_SUBSET"]
+    k = len(gauss)
+    lines += ["begin conformal_subset", "subset_id %d" % k, "begin boundary", "begin rectangle", "center 200 50",
+              "width 7", "height 4", "end rectangle", "end boundary", "end conformal_subset"]
+    lines += ["BEGIN CONFORMAL_SUBSET", "SUBSET_ID %d" % (k + 1), "BEGIN BOUNDARY", "BEGIN RECTANGLE",
+              "UPPER_LEFT 227 45 # corners", "LOWER_RIGHT 236 52", "END RECTANGLE", "END BOUNDARY",
+              "END CONFORMAL_SUBSET"]
+    f = tmp_path / "shapes.txt"
+    f.write_text("\n".join(lines) + "\n")
+    coords, conf = subset_info(str(f))
+    assert len(coords) == len(gauss) + 2
+    for i, r in enumerate(gauss):
+        assert conf[i]["n"] == gauss[r]
+        assert [p[:2] for p in conf[i]["pix"]] == union_pixels(o.circle_pixels(20 + 30 * i, 50, r))
+    assert [p[:2] for p in conf[k]["pix"]] == union_pixels(o.rectangle_pixels(200, 50, 7, 4))
+    assert conf[k]["n"] == 28 and conf[k]["pix"][0][:2] == (48, 197)
+    # corners 227,45 .. 236,52: width 9, height 7, centre (231, 48)
+    assert [p[:2] for p in conf[k + 1]["pix"]] == union_pixels(o.rectangle_pixels(231, 48, 9, 7))
+
+
+def test_regression_subset_file_reader():
+    """The deck's subset file (comments, blank lines, a union of two polygons and a circle, an excluded
+    polygon, an obstruction and blocking ids) read by read_subset_file, against the oracle's shapes."""
+    coords, conf = subset_info(os.path.join(DECK, "subset_defs.txt"))
+    assert coords == [(126, 157), (125, 250), (397, 139), (177, 314), (395, 405)]
+    assert sorted(conf) == [2, 3]
+    s2 = union_pixels(o.polygon_pixels([307, 352, 426, 421, 372], [136, 86, 109, 243, 143]),
+                      o.circle_pixels(362, 201, 45), o.polygon_pixels([359, 455, 437], [228, 175, 267]))
+    assert [p[:2] for p in conf[2]["pix"]] == s2 and conf[2]["active"] == len(s2)
+    s3 = union_pixels(o.circle_pixels(178, 352, 64))
+    ex = set(union_pixels(o.polygon_pixels([148, 160, 205], [341, 374, 340])))
+    assert [p[:2] for p in conf[3]["pix"]] == s3
+    assert [p[2] for p in conf[3]["pix"]] == [0 if p in ex else 1 for p in s3]
+    assert 0 < len(s3) - conf[3]["active"] == len(ex & set(s3))
+    assert conf[3]["obstructed"] == 1 and conf[3]["blocking"] == [0, 4]
+
+
+def test_subset_file_errors(tmp_path):
+    for body, what in [("BEGIN REGION_OF_INTEREST\nEND REGION_OF_INTEREST\n", "REGION_OF_INTEREST"),
+                       ("BEGIN CONFORMAL_SUBSET\nBEGIN BOUNDARY\nBEGIN CIRCLE\nCENTER 10 10\nRADIUS 3\nEND CIRCLE\n"
+                        "END BOUNDARY\nEND CONFORMAL_SUBSET\n", "SUBSET_ID"),
+                       ("BEGIN SUBSET_COORDINATES\n10\nEND SUBSET_COORDINATES\n", "not enough values")]:
+        f = tmp_path / "bad.txt"
+        f.write_text(body)
+        r = run("subset_file_info", str(f))
+        assert r.returncode == 98 and what in r.stderr, (what, r.stderr)
+
+
+def dice_diff(a, b, *opts):
+    if not os.path.exists(DIFF):
+        pytest.fail("dice_amd/lib/dice_diff is not built (run __graft_entry__.build())")
+    r = subprocess.run([DIFF, str(a), str(b)] + [str(x) for x in opts], capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout
+
+
+def test_dice_diff_semantics(tmp_path):
+    """tools/DICe_Diff.cpp (serial): relative differences against the first file, the floor, tiny values,
+    string tokens, the skipped masthead, and line-count mismatches."""
+    a = tmp_path / "a.txt"
+    b = tmp_path / "b.txt"
+    a.write_text("***\n*** Digital Image Correlation Engine (DICe) v1\nX Y\n0 1.0000E+00 2.0000E-05\n")
+    b.write_text("***\n*** Digital Image Correlation Engine, another build\nX Y\n0 1.0400E+00 9.0000E-05\n")
+    assert dice_diff(a, b, "-t", 0.05, "-f", 5e-5)[0] == 0          # 4 % and a value below the floor
+    assert dice_diff(a, b, "-t", 0.01, "-f", 5e-5)[0] == 1          # 4 % > 1 %
+    assert dice_diff(a, b, "-t", 0.05)[0] == 1                      # no floor: 2e-5 vs 9e-5 fails
+    b.write_text("***\n*** Digital\nX Z\n0 1.0000E+00 2.0000E-05\n")
+    assert dice_diff(a, b)[0] == 1 and dice_diff(a, b, "-n")[0] == 0  # a string token differs
+    b.write_text("***\n*** Digital\nX Y\n")
+    code, out = dice_diff(a, b)
+    assert code == 1 and "TEST FAILED" in out
+
+
+def test_oracle_on_regression_deck():
+    """The oracle (GENERIC routine) on the deck against its gold: the square subsets to within two units of
+    the fifth printed digit (the float and double builds each round one printed value the other way);
+    every value within the regression suite's 5 % (the conformal subset 3 has an obstruction, which
+    belongs to the TRACKING routine)."""
+    d = np.load(os.path.join(GOLD, "speckle_512.npz"))
+    coords, conf = subset_info(os.path.join(DECK, "subset_defs.txt"))
+    offs, xs, ys, acts = [0], [], [], []
+    for i, (cx, cy) in enumerate(coords):
+        if i in conf:
+            pix = conf[i]["pix"]
+        else:
+            pix = [(y, x, 1) for y in range(int(cy) - 13, int(cy) + 14) for x in range(int(cx) - 13, int(cx) + 14)]
+        ys += [p[0] for p in pix]
+        xs += [p[1] for p in pix]
+        acts += [p[2] for p in pix]
+        offs.append(offs[-1] + len(pix))
+    f = o.new_fields(np.array([c[0] for c in coords]), np.array([c[1] for c in coords]))
+    o.correlate(o.default_params(), d["ref"], d["defm"], f,
+                conformal=(np.array(offs, np.int32), np.array(xs, np.int32), np.array(ys, np.int32),
+                           np.array(acts, np.uint8)), precision="double", threads=4)
+    for i in range(5):
+        g = [float(v) for v in open(os.path.join(DECK, "gold", "DICe_solution_%d.txt" % i)).read().split("\n")[-2].split()]
+        u, v, sig = f[o.F["DISP_X"], i], f[o.F["DISP_Y"], i], f[o.F["SIGMA"], i]
+        assert f[o.F["STATUS"], i] == g[10] == 1
+        for val, gv in ((u, g[3]), (v, g[4]), (sig, g[9])):
+            assert abs(val - gv) <= 0.05 * abs(gv)
+            if i in (0, 1, 4):
+                assert abs(val - gv) <= 5e-5 * abs(gv)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
+def test_regression_deck_conformal_subset_def(tmp_path, backend):
+    """tests/regression/conformal_subset_def end to end through the C++ Schema mirror: read_subset_file,
+    the correlation, write_output (one file per subset), then dice_diff against each gold file with the
+    regression suite's options (-t 0.05 -f 5.0E-5, strings included); the square subsets' u, v and sigma also
+    to within two units of the fifth printed digit."""
+    import json
+    deck = json.load(open(os.path.join(DECK, "deck.json")))
+    d = np.load(os.path.join(GOLD, "speckle_512.npz"))
+    h, w = d["ref"].shape
+    routine = 1 if deck["correlation_routine"] == "TRACKING_ROUTINE" else 0
+    r = run("subset_file_deck", raw(tmp_path, "ref.f32", d["ref"]), raw(tmp_path, "def.f32", d["defm"]), w, h,
+            os.path.join(DECK, "subset_defs.txt"), deck["subset_size"], tmp_path, backend, deck["reference_image"],
+            deck["deformed_image"], routine)
+    assert r.returncode == 0, r.stderr
+    for name in deck["gold"]:
+        code, out = dice_diff(os.path.join(DECK, "gold", name), tmp_path / name, "-t", deck["diff_tolerance"],
+                              "-f", deck["diff_floor"], "-v")
+        assert code == 0 and "TEST PASSED" in out, (name, out)
+    for i in (0, 1, 4):
+        g = [float(v) for v in open(os.path.join(DECK, "gold", "DICe_solution_%d.txt" % i)).read().split("\n")[-2].split()]
+        ours = [float(v) for v in open(tmp_path / ("DICe_solution_%d.txt" % i)).read().split("\n")[-2].split()]
+        assert ours[:3] == g[:3] and ours[10] == g[10]
+        for k in (3, 4, 9):
+            assert abs(ours[k] - g[k]) <= 5e-5 * abs(g[k]), (i, k, ours[k], g[k])
