@@ -491,6 +491,67 @@ void Schema::set_params(const ParameterList &params) {
   backend_ = backend;
 }
 
+void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_size, int step_size,
+                                               std::vector<double> &points_x, std::vector<double> &points_y,
+                                               std::vector<int> &neighbor_ids,
+                                               const std::map<int, Conformal_Area_Def> *roi_defs) {
+  if (step_size <= 0) throw std::runtime_error("Error: step size is <= 0");
+  if (subset_size <= 0) throw std::runtime_error("Error, the subset size was not specified");
+  points_x.clear();
+  points_y.clear();
+  neighbor_ids.clear();
+  std::map<int, Conformal_Area_Def> whole;
+  if (!roi_defs || roi_defs->empty()) {
+    whole[0].boundary.push_back(std::make_shared<Rectangle>(img_w / 2, img_h / 2, img_w, img_h));
+    roi_defs = &whole;
+  }
+  for (const auto &roi : *roi_defs) {
+    std::set<std::pair<int, int>> coords, excluded;
+    for (const auto &shape : roi.second.boundary)
+      for (const auto &yx : shape->get_owned_pixels()) coords.insert(yx);
+    for (const auto &shape : roi.second.excluded_area)
+      for (const auto &yx : shape->get_owned_pixels()) excluded.insert(yx);
+    // valid_correlation_point (DICe_Decomp.cpp:1072-1103) without the gradient threshold
+    auto valid = [&](int x, int y) {
+      if (x < subset_size - 1 || x > img_w - subset_size || y < subset_size - 1 || y > img_h - subset_size)
+        return false;
+      if (!coords.count({y, x})) return false;
+      const int x0 = x - subset_size / 2, y0 = y - subset_size / 2;
+      const int cxs[4] = {x0, x0 + subset_size, x0 + subset_size, x0};
+      const int cys[4] = {y0, y0, y0 + subset_size, y0 + subset_size};
+      for (int i = 0; i < 4; ++i)
+        if (excluded.count({cys[i], cxs[i]})) return false;
+      return true;
+    };
+    int num_rows = 0, num_cols = 0;
+    for (int x = subset_size - 1; x < img_w - subset_size; x += step_size) ++num_cols;
+    for (int y = subset_size - 1; y < img_h - subset_size; y += step_size) ++num_rows;
+    auto add = [&](int col, int row, int neighbor) {
+      const int x = subset_size - 1 + col * step_size, y = subset_size - 1 + row * step_size;
+      if (!valid(x, y)) return false;
+      points_x.push_back(x);
+      points_y.push_back(y);
+      neighbor_ids.push_back(neighbor);
+      return true;
+    };
+    // the first point (row 0, column 0), then the serpentine to the right
+    add(0, 0, -1);
+    const int start = (int)points_x.size();
+    int direction = 1, row = 0, col = 0;
+    while (row >= 0 && row < num_rows) {
+      if ((direction == 1 && row + 1 >= num_rows) || (direction == -1 && row - 1 < 0)) {
+        direction = -direction;
+        ++col;
+      } else {
+        row += direction;
+      }
+      if (col >= num_cols) break;
+      const int id = (int)points_x.size();
+      add(col, row, id == start ? -1 : id - 1);
+    }
+  }
+}
+
 void conformal_subset_pixels(const Conformal_Area_Def &def, std::vector<int> &x, std::vector<int> &y,
                              std::vector<uint8_t> &active) {
   if (def.boundary.empty()) throw std::runtime_error("Error, conformal subset without a boundary");

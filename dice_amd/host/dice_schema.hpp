@@ -164,6 +164,18 @@ struct Subset_File_Info {
 };
 Subset_File_Info read_subset_file(const std::string &file_name, int width = -1, int height = -1);
 
+// create_regular_grid_of_correlation_points (DICe_Decomp.cpp:868-1070) without
+// a seed or a gradient threshold: in each region of interest (the whole image
+// when roi_defs is null or empty), centres at subset_size - 1 + k * step_size
+// that lie at least a subset inside the image, inside the region, with no
+// subset corner in an excluded area, ordered as the reference's serpentine
+// (down the first column, up the next, ...).  neighbor_ids: -1 for the first
+// point of a region, else the previous point.
+void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_size, int step_size,
+                                               std::vector<double> &points_x, std::vector<double> &points_y,
+                                               std::vector<int> &neighbor_ids,
+                                               const std::map<int, Conformal_Area_Def> *roi_defs = nullptr);
+
 // The pixel list of a conformal subset (Subset::Subset(cx, cy, Conformal_Area_Def),
 // DICe_Subset.cpp:115-168): the union of the boundary shapes' pixels in (y, x)
 // order, negative coordinates clamped to 0, active unless an excluded shape
@@ -223,6 +235,11 @@ class Schema {
   bool shear_strain_enabled() const { return p_.enable_shear_strain != 0; }
   int step_size_x() const { return step_x_; }
   int step_size_y() const { return step_y_; }
+  // the step sizes reported in the output header for subsets laid on a grid (-1: not a regular grid)
+  void set_step_sizes(int step_x, int step_y) {
+    step_x_ = step_x;
+    step_y_ = step_y;
+  }
   dice_hip_context *context() { return ctx_; }
 
   // File names written into the header block of the output (the reference

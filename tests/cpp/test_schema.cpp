@@ -7,8 +7,8 @@
 //   test_schema parallel_subsets <ref.f32> <def.f32> <w> <h> <gold.txt> <backend 0|1>
 //   test_schema polygon            (prints the owned pixels "y x" of the golden's polygons)
 //   test_schema subset_file_info <subset file>   (what read_subset_file makes of it)
-//   test_schema subset_file_deck <ref.f32> <def.f32> <w> <h> <subset file> <subset size> <out dir>
-//                                <backend 0|1> <ref name> <def name> <routine 0|1>
+//   test_schema deck <config>      (a regression deck end to end, see deck())
+//   test_schema grid <w> <h> <subset size> <step>   (prints "x y neighbor" of the regular grid)
 //   test_schema errors             (error behaviour that needs no GPU)
 //
 // Images are raw row-major float32.  The exit status is the number of failures.
@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -190,27 +191,54 @@ int subset_file_info(const char *path) {
   return 0;
 }
 
-// A subset-file deck of the reference's regression suite through the Schema
-// mirror: subsets from the file, default parameters with the deck's
-// correlation routine, one frame, output written one file per subset.
-int subset_file_deck(const std::vector<float> &ref, const std::vector<float> &def, int w, int h, const char *subset_file,
-                     int subset_size, const std::string &out_dir, int backend, const char *ref_name,
-                     const char *def_name, int routine) {
-  const Subset_File_Info info = read_subset_file(subset_file, w, h);
+// A deck of the reference's regression suite through the Schema mirror.  The
+// configuration file has "key value" lines: ref / def (raw float32 images),
+// width, height, subset_size, subset_file or step_size, out_dir, prefix,
+// separate (0|1), backend (0|1), ref_name, def_name, routine (0|1), and
+// optional fields (comma-separated output names) and optimization /
+// interpolation (enum values).  Default parameters otherwise; one frame.
+int deck(const char *config_path) {
+  std::ifstream cfg(config_path);
+  std::map<std::string, std::string> c;
+  std::string key, value;
+  while (cfg >> key && std::getline(cfg >> std::ws, value)) c[key] = value;
+  auto num = [&](const char *k) { return std::atoi(c.at(k).c_str()); };
+  const int w = num("width"), h = num("height");
+  const std::vector<float> ref = read_raw(c.at("ref").c_str(), w, h), def = read_raw(c.at("def").c_str(), w, h);
   ParameterList params;
   dice_default_params(&params);
-  params.set("correlation_routine", routine);
-  params.set("correlation_backend", backend);
-  Schema schema(info.coordinates_x, info.coordinates_y, subset_size, &info.conformal_area_defs, &params);
-  schema.set_image_file_names(ref_name, def_name);
+  params.set("correlation_routine", num("routine"));
+  params.set("correlation_backend", num("backend"));
+  if (c.count("optimization")) params.set("optimization_method", num("optimization"));
+  if (c.count("interpolation")) params.set("interpolation_method", num("interpolation"));
+  std::vector<double> cx, cy;
+  std::vector<int> neighbors;
+  Subset_File_Info info;
+  const bool grid = c.count("step_size") != 0;
+  if (grid) {
+    create_regular_grid_of_correlation_points(w, h, num("subset_size"), num("step_size"), cx, cy, neighbors);
+  } else {
+    info = read_subset_file(c.at("subset_file"), w, h);
+    cx = info.coordinates_x;
+    cy = info.coordinates_y;
+  }
+  Schema schema(cx, cy, num("subset_size"), &info.conformal_area_defs, &params);
+  if (grid) schema.set_step_sizes(num("step_size"), num("step_size"));
+  schema.set_image_file_names(c.at("ref_name"), c.at("def_name"));
   schema.set_ref_image(w, h, ref.data());
   schema.set_def_image(w, h, def.data());
   schema.execute_correlation();
-  schema.write_output(out_dir + "/", "DICe_solution", true);
+  std::vector<std::string> fields;
+  if (c.count("fields")) {
+    std::stringstream ss(c.at("fields"));
+    std::string f;
+    while (std::getline(ss, f, ',')) fields.push_back(f);
+  }
+  schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, " ", false, 1);
   for (int i = 0; i < schema.local_num_subsets(); ++i)
-    std::cout << "subset " << i << " u " << schema.local_field_value(i, SUBSET_DISPLACEMENT_X_FS) << " v "
-              << schema.local_field_value(i, SUBSET_DISPLACEMENT_Y_FS) << " sigma "
-              << schema.local_field_value(i, SIGMA_FS) << " status "
+    std::cout << "subset " << i << " x " << cx[i] << " y " << cy[i] << " u "
+              << schema.local_field_value(i, SUBSET_DISPLACEMENT_X_FS) << " v "
+              << schema.local_field_value(i, SUBSET_DISPLACEMENT_Y_FS) << " status "
               << schema.local_field_value(i, STATUS_FLAG_FS) << "\n";
   return 0;
 }
@@ -251,15 +279,22 @@ int main(int argc, char **argv) {
     if (cmd == "polygon") return polygon();
     if (cmd == "errors") return errors_without_gpu();
     if (cmd == "subset_file_info" && argc >= 3) return subset_file_info(argv[2]);
+    if (cmd == "deck" && argc >= 3) return deck(argv[2]);
+    if (cmd == "grid" && argc >= 6) {
+      std::vector<double> x, y;
+      std::vector<int> nb;
+      create_regular_grid_of_correlation_points(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
+                                                std::atoi(argv[5]), x, y, nb);
+      for (size_t i = 0; i < x.size(); ++i) std::cout << x[i] << " " << y[i] << " " << nb[i] << "\n";
+      return 0;
+    }
     if (argc < 7) return 99;
     const int w = std::atoi(argv[4]), h = std::atoi(argv[5]);
     const std::vector<float> ref = read_raw(argv[2], w, h), def = read_raw(argv[3], w, h);
     if (cmd == "correlate_params") return correlate_params(ref, def, w, h, std::atoi(argv[6]));
     if (cmd == "correlate_conformal") return correlate_conformal(ref, def, w, h, std::atoi(argv[6]));
     if (cmd == "parallel_subsets" && argc >= 8) return parallel_subsets(ref, def, w, h, argv[6], std::atoi(argv[7]));
-    if (cmd == "subset_file_deck" && argc >= 13)
-      return subset_file_deck(ref, def, w, h, argv[6], std::atoi(argv[7]), argv[8], std::atoi(argv[9]), argv[10],
-                              argv[11], std::atoi(argv[12]));
+
   } catch (const std::exception &e) {
     std::cerr << "exception: " << e.what() << "\n";
     return 98;
