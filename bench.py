@@ -82,14 +82,15 @@ def csr_take(conf, idx):
     return o2, px[sel], py[sel], act[sel]
 
 
-def cpu_baseline(p, ref, defm, n, fields_of, seconds, subset_size=0, conformal=None, what=""):
+def cpu_baseline(p, ref, defm, n, fields_of, seconds, subset_size=0, conformal=None, what="", threads=None):
     """The CPU oracle (float build, OpenMP over subsets) on a bounded, evenly strided sample of the n
     subsets: calibrated on a small slice, then sized for about `seconds` of CPU work.
-    fields_of(idx) gives the initial field store of the subsets idx."""
+    fields_of(idx) gives the initial field store of the subsets idx.  threads: all usable cores (at most
+    16) by default; 1 for the serial loop of one reference rank."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as o
     import dice_amd as dh
-    threads = cpu_threads()
+    threads = threads or cpu_threads()
     q = o.default_params()
     for name, _ in dh.Params._fields_:
         setattr(q, name, getattr(p, name))
@@ -229,12 +230,14 @@ def bench_c2(args):
     # HBM-side bytes per launch of the batched kernel come from rocprofv3 --pmc
     # FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a copy kernel of
     # known size; the summary that holds them is committed with the profiles
-    traffic = None
+    traffic = valu_busy = None
     traffic_src = os.path.join(ROOT, "profiles", "correlate_batched_c2_hbm_traffic.md")
     if (W, S, args.step, len(frames)) == (2048, 41, 5, 10) and os.path.exists(traffic_src):
         for line in open(traffic_src):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = float(line.split(":", 1)[1])
+            elif line.startswith("valu_busy:"):
+                valu_busy = float(line.split(":", 1)[1])
 
     # roofline of the batched correlation kernel: algorithmic bytes per launch / launch time
     bytes_per_frame = algorithmic_bytes(iters, status, S * S, (S + 5) ** 2).sum(axis=1)
@@ -256,6 +259,9 @@ def bench_c2(args):
         roofline(avg_bytes, kern_ms, "correlate_batched_kernel (GN, Keys-4, f32)", kern_cnt, traffic,
                  "profiles/correlate_batched_c2_hbm_traffic.md"),
         eager_ms_per_step=round(1000.0 * el_e / max(1, min(args.steps, 3)), 3))
+    if valu_busy is not None:
+        # the bound that binds: VALU issue, from the SQ counters of the same profile
+        result["roofline"]["valu_busy"] = valu_busy
 
     if not args.no_plain:
         el_p, _, _ = timed(dh.MODE_PLAIN, max(1, min(args.steps, 2)), 1)
@@ -270,6 +276,10 @@ def bench_c2(args):
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p, ref, frames[0], n, lambda idx: dh.new_fields(cx[idx], cy[idx]),
                                               args.cpu_seconds, subset_size=S, what="frame 1 (A=0.1), ")
+        result["cpu_baseline_serial"] = cpu_baseline(p, ref, frames[0], n,
+                                                     lambda idx: dh.new_fields(cx[idx], cy[idx]),
+                                                     args.cpu_seconds / 3.0, subset_size=S,
+                                                     what="frame 1 (A=0.1), one thread, ", threads=1)
     ctx.close()
     return result
 
