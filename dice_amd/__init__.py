@@ -68,6 +68,8 @@ class Params(ctypes.Structure):
         ("normalize_gamma_with_active_pixels", ctypes.c_int32),
         ("compute_uncertainty_fields", ctypes.c_int32),
         ("gradient_method", ctypes.c_int32),
+        ("gauss_filter_images", ctypes.c_int32),
+        ("gauss_filter_mask_size", ctypes.c_int32),
     ]
 
 
@@ -79,7 +81,7 @@ EXPORTS = [
     "dice_hip_correlate", "dice_hip_correlate_device", "dice_hip_correlate_sequence",
     "dice_hip_synchronize", "dice_hip_stream", "dice_hip_kernel_timing", "dice_hip_set_kernel_timing",
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
-    "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint",
+    "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
 ]
 
 _lib = None
@@ -136,6 +138,7 @@ def lib():
         L.dice_hip_correlate_points.argtypes = [P, I, I, P, I, I, P, I, I, P, I]
         L.dice_hip_gamma.argtypes = [P, I, P, P]
         L.dice_hip_gamma_adjoint.argtypes = [P, I, P, P, P, P]
+        L.dice_hip_gauss_filter.argtypes = [I, I, I, I, P, P, I]
         _lib = L
     return _lib
 
@@ -299,6 +302,17 @@ class Context:
     def set_launch_shape(self, waves_per_group, groups_per_cu, lds_floats_per_wave):
         self._check(self._lib.dice_hip_set_launch_shape(self.h, waves_per_group, groups_per_cu, lds_floats_per_wave),
                     "set_launch_shape")
+
+
+def gauss_filter(img, mask_size, device=0):
+    """dice_hip_gauss_filter: Image_::gauss_filter (DICe_Image.cpp:1194-1240) of a float32 or uint16 image
+    on the GPU; returns a new array of the same type."""
+    img, st = Context._img(img)
+    out = np.empty_like(img)
+    rc = lib().dice_hip_gauss_filter(device, img.shape[1], img.shape[0], st, _p(img), _p(out), mask_size)
+    if rc != 0:
+        raise DiceHipError("gauss_filter failed (%d)" % rc)
+    return out
 
 
 def correlate_points(points, subset_size, ref, defm, params=None, update_params=False):

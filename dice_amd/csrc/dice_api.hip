@@ -79,6 +79,7 @@ struct dice_hip_context {
   DevBuf def_pix;
   const void *def_ptr = nullptr;
   DevBuf def_gx, def_gy, tmp_x, tmp_y;
+  DevBuf ref_raw, def_filt;  // upload target of a filtered reference image; the filtered deformed image
   bool def_has_grads = false;
   // subsets
   int n = 0, S = 0, fov = 0;
@@ -140,6 +141,11 @@ bool params_valid(const dice_params *p, std::string &why) {
   }
   if (p->max_solver_iterations_fast < 0 || p->max_solver_iterations_robust < 1) {
     why = "iteration limits out of range";
+    return false;
+  }
+  if (p->gauss_filter_images && p->gauss_filter_mask_size != 5 && p->gauss_filter_mask_size != 7 &&
+      p->gauss_filter_mask_size != 9 && p->gauss_filter_mask_size != 11 && p->gauss_filter_mask_size != 13) {
+    why = "the Gauss filter mask size is invalid (options include 5,7,9,11,13)";
     return false;
   }
   if (p->gradient_method != 0 && p->gradient_method != 1) {
@@ -321,7 +327,7 @@ int dice_hip_destroy(dice_hip_context *ctx) {
   }
   for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
-                    &ctx->fields, &ctx->work})
+                    &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->fields, &ctx->work})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -353,6 +359,8 @@ void dice_hip_default_params(dice_params *p) {
   p->levenberg_marquardt_regularization_factor = -1.0;
   p->momentum_factor = 0.0;
   p->compute_uncertainty_fields = 1;
+  p->gauss_filter_images = 0;
+  p->gauss_filter_mask_size = 7;
 }
 
 void dice_hip_tracking_default_params(dice_params *p) {
@@ -389,8 +397,18 @@ int dice_hip_set_ref_image(dice_hip_context *ctx, int width, int height, int sto
     return ctx->fail(DICE_HIP_EINVAL, "bad reference image");
   (void)hipSetDevice(ctx->device);
   drop_graph(ctx);
-  int rc = upload_image(ctx, ctx->ref_pix, width, height, storage, host_pixels);
+  const bool filter = ctx->params.gauss_filter_images != 0;
+  if (filter && (width < ctx->params.gauss_filter_mask_size || height < ctx->params.gauss_filter_mask_size))
+    return ctx->fail(DICE_HIP_EINVAL, "image too small for gauss filtering with this mask size");
+  int rc = upload_image(ctx, filter ? ctx->ref_raw : ctx->ref_pix, width, height, storage, host_pixels);
   if (rc) return rc;
+  if (filter) {
+    // images are filtered when they are built (DICe_Image.cpp:215-230), before their gradients
+    HIP_TRY(ctx, ctx->ref_pix.ensure((size_t)width * height * pix_bytes(storage)), "hipMalloc image");
+    HIP_TRY(ctx, dice_hip::launch_gauss_filter(storage, ctx->ref_raw.p, ctx->ref_pix.p, width, height,
+                                               ctx->params.gauss_filter_mask_size, ctx->stream),
+            "gauss filter");
+  }
   ctx->rw = width;
   ctx->rh = height;
   ctx->rstorage = storage;
@@ -402,6 +420,14 @@ int dice_hip_set_ref_image(dice_hip_context *ctx, int width, int height, int sto
 }
 
 static int set_def_common(dice_hip_context *ctx, int width, int height, int storage, const void *dptr) {
+  if (ctx->params.gauss_filter_images) {
+    const int m = ctx->params.gauss_filter_mask_size;
+    if (width < m || height < m) return ctx->fail(DICE_HIP_EINVAL, "image too small for gauss filtering with this mask size");
+    HIP_TRY(ctx, ctx->def_filt.ensure((size_t)width * height * pix_bytes(storage)), "hipMalloc image");
+    HIP_TRY(ctx, dice_hip::launch_gauss_filter(storage, dptr, ctx->def_filt.p, width, height, m, ctx->stream),
+            "gauss filter");
+    dptr = ctx->def_filt.p;
+  }
   ctx->dw = width;
   ctx->dh = height;
   ctx->dstorage = storage;
@@ -758,6 +784,28 @@ int dice_hip_autotune(dice_hip_context *ctx, const double *device_fields, int fi
   ctx->tuned[key] = best;
   drop_graph(ctx);
   return DICE_HIP_OK;
+}
+
+int dice_hip_gauss_filter(int device, int width, int height, int storage, const void *host_in, void *host_out,
+                          int mask_size) {
+  if (!host_in || !host_out || (storage != DICE_STORAGE_F32 && storage != DICE_STORAGE_U16)) return DICE_HIP_EINVAL;
+  if (mask_size != 5 && mask_size != 7 && mask_size != 9 && mask_size != 11 && mask_size != 13)
+    return DICE_HIP_EINVAL;
+  if (width < mask_size || height < mask_size) return DICE_HIP_EINVAL;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return DICE_HIP_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return DICE_HIP_EHIP;
+  const size_t bytes = (size_t)width * height * pix_bytes(storage);
+  void *din = nullptr, *dout = nullptr;
+  int rc = DICE_HIP_OK;
+  if (hipMalloc(&din, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+      hipMemcpy(din, host_in, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      dice_hip::launch_gauss_filter(storage, din, dout, width, height, mask_size, nullptr) != hipSuccess ||
+      hipMemcpy(host_out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = DICE_HIP_EHIP;
+  if (din) (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  return rc;
 }
 
 int dice_hip_correlate_points(float *points, int n_points, int subset_size, const float *ref_img, int ref_w, int ref_h,

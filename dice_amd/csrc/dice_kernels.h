@@ -162,6 +162,11 @@ hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, cons
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,
                             float *tmpx, float *tmpy, hipStream_t stream);
 
+// Image_::gauss_filter: the mask_size Gauss mask (5, 7, 9, 11, 13) from in to out
+// (different buffers), border band of mask_size / 2 + 1 pixels copied.
+hipError_t launch_gauss_filter(int storage, const void *in, void *out, int w, int h, int mask_size,
+                               hipStream_t stream);
+
 // Zero a small int buffer with a kernel (graph-safe, no memset node).
 hipError_t launch_zero_ints(int *p, int count, hipStream_t stream);
 

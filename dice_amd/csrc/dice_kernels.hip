@@ -57,6 +57,95 @@ __global__ void smooth_kernel(const float *inx, const float *iny, int w, int h, 
   gy[o] = vy;
 }
 
+// Gauss filter masks of Image_::gauss_filter (DICe_Image.cpp:54-126, 1194-1240).
+// The 2-D masks are the outer products c_i c_j of these 1-D factors (listed
+// from the edge to the centre; the 5-point one sums to 0.9458, so its mask is
+// not normalised).  gauss_coefficients forms each product in float from the
+// float factors: that reproduces the reference's filtered-image goldens
+// (tests/component/images/outFilter*.rawi) bit for bit, where its source
+// tables, the same products written to nine digits, move a few pixels by an
+// ulp.
+static const double kGauss5[3] = {0.0014, 0.1574, 0.62825};
+static const double kGauss7[4] = {0.006, 0.0606, 0.2418, 0.3831};
+static const double kGauss9[5] = {0.0007, 0.0108, 0.0748, 0.2384, 0.3505};
+static const double kGauss11[6] = {0.0001, 0.0017, 0.0168, 0.087, 0.2328, 0.3231};
+static const double kGauss13[7] = {0.0001, 0.0012, 0.0085, 0.038, 0.1109, 0.2108, 0.2611};
+
+constexpr int kGfMaxMask = 13;
+struct GaussCoef {
+  float c[kGfMaxMask * kGfMaxMask];  // c[i * M + j]: i the x offset, j the y offset
+};
+
+static bool gauss_coefficients(int M, GaussCoef &g) {
+  const double *f = M == 5 ? kGauss5 : M == 7 ? kGauss7 : M == 9 ? kGauss9 : M == 11 ? kGauss11 : M == 13 ? kGauss13 : nullptr;
+  if (!f) return false;
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < M; ++j)
+      g.c[i * M + j] = (float)f[i <= M / 2 ? i : M - 1 - i] * (float)f[j <= M / 2 ? j : M - 1 - j];
+  return true;
+}
+
+// One 32 x 32 output tile per 32 x 8 block, each thread four rows; the tile
+// and its halo are staged in LDS (reads clamped to the image: clamped values
+// only reach the copied border band).  Pixels at least M / 2 + 1 from every
+// edge get sum_i sum_j c_ij I(x + i - M/2, y + j - M/2), accumulated in float
+// with a separate multiply and add per tap (no fused multiply-add) in the
+// reference's order, x offset outer; the others are copied.
+constexpr int kGfTile = 32;
+template <typename T, int M>
+__global__ __launch_bounds__(256) void gauss_filter_kernel(const void *in, void *out, int w, int h, GaussCoef g) {
+#pragma clang fp contract(off)
+  constexpr int R = M / 2, P = kGfTile + 2 * R;
+  __shared__ float tile[P * P];
+  const T *src = (const T *)in;
+  T *dst = (T *)out;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 32 + tx;
+  const int x0 = blockIdx.x * kGfTile, y0 = blockIdx.y * kGfTile;
+  for (int k = tid; k < P * P; k += 256) {
+    const int ly = k / P, lx = k - ly * P;
+    const int gx = min(max(x0 + lx - R, 0), w - 1), gy = min(max(y0 + ly - R, 0), h - 1);
+    tile[k] = (float)src[(size_t)gy * w + gx];
+  }
+  __syncthreads();
+  const int hm = R + 1;  // gauss_filter_half_mask_
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int lx = tx, ly = ty + 8 * r;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) continue;
+    const size_t o = (size_t)y * w + x;
+    if (x >= hm && x < w - hm && y >= hm && y < h - hm) {
+      float value = 0.0f;
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int j = 0; j < M; ++j) value += g.c[i * M + j] * tile[(ly + j) * P + lx + i];
+      // cast to the storage type as the reference does (truncating for uint16; the
+      // 13-point mask sums to 1.0002, so a saturated 16-bit pixel is clamped)
+      if constexpr (std::is_same<T, float>::value) dst[o] = value;
+      else dst[o] = (T)fminf(value, 65535.0f);
+    } else {
+      dst[o] = src[o];
+    }
+  }
+}
+
+template <typename T>
+hipError_t launch_gauss_t(const void *in, void *out, int w, int h, int mask, hipStream_t stream) {
+  GaussCoef g;
+  if (!gauss_coefficients(mask, g)) return hipErrorInvalidValue;
+  const dim3 block(32, 8), grid((w + kGfTile - 1) / kGfTile, (h + kGfTile - 1) / kGfTile);
+  switch (mask) {
+    case 5: hipLaunchKernelGGL((gauss_filter_kernel<T, 5>), grid, block, 0, stream, in, out, w, h, g); break;
+    case 7: hipLaunchKernelGGL((gauss_filter_kernel<T, 7>), grid, block, 0, stream, in, out, w, h, g); break;
+    case 9: hipLaunchKernelGGL((gauss_filter_kernel<T, 9>), grid, block, 0, stream, in, out, w, h, g); break;
+    case 11: hipLaunchKernelGGL((gauss_filter_kernel<T, 11>), grid, block, 0, stream, in, out, w, h, g); break;
+    case 13: hipLaunchKernelGGL((gauss_filter_kernel<T, 13>), grid, block, 0, stream, in, out, w, h, g); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 __global__ void zero_ints_kernel(int *p, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
@@ -138,6 +227,13 @@ hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *g
   if (e != hipSuccess || !smooth) return e;
   hipLaunchKernelGGL(smooth_kernel, grid, block, 0, stream, tmpx, tmpy, w, h, gx, gy);
   return hipGetLastError();
+}
+
+hipError_t launch_gauss_filter(int storage, const void *in, void *out, int w, int h, int mask_size,
+                               hipStream_t stream) {
+  if (w < mask_size || h < mask_size) return hipErrorInvalidValue;
+  return storage == 0 ? launch_gauss_t<float>(in, out, w, h, mask_size, stream)
+                      : launch_gauss_t<uint16_t>(in, out, w, h, mask_size, stream);
 }
 
 hipError_t launch_zero_ints(int *p, int count, hipStream_t stream) {

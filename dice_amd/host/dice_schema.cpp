@@ -48,6 +48,8 @@ const std::map<std::string, Slot> &slots() {
       {"normalize_gamma_with_active_pixels", {I32, offsetof(dice_params, normalize_gamma_with_active_pixels)}},
       {"compute_uncertainty_fields", {I32, offsetof(dice_params, compute_uncertainty_fields)}},
       {"gradient_method", {I32, offsetof(dice_params, gradient_method)}},
+      {"gauss_filter_images", {I32, offsetof(dice_params, gauss_filter_images)}},
+      {"gauss_filter_mask_size", {I32, offsetof(dice_params, gauss_filter_mask_size)}},
   };
   return m;
 }
@@ -639,6 +641,20 @@ void Schema::set_def_image(int width, int height, const float *intensities) {
 }
 void Schema::set_def_image(int width, int height, const uint16_t *intensities) {
   set_image(false, width, height, intensities, DICE_STORAGE_U16);
+}
+void Schema::set_ref_image(const std::string &file_name) {
+  std::vector<float> pix;
+  int w = 0, h = 0;
+  read_image(file_name, pix, w, h);
+  set_image(true, w, h, pix.data(), DICE_STORAGE_F32);
+  ref_name_ = file_name;
+}
+void Schema::set_def_image(const std::string &file_name) {
+  std::vector<float> pix;
+  int w = 0, h = 0;
+  read_image(file_name, pix, w, h);
+  set_image(false, w, h, pix.data(), DICE_STORAGE_F32);
+  def_name_ = file_name;
 }
 
 int Schema::execute_correlation() {

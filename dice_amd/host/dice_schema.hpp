@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "dice_hip.h"
+#include "dice_image_io.hpp"
 
 namespace dice_hip {
 
@@ -210,6 +211,12 @@ class Schema {
   void set_ref_image(int width, int height, const uint16_t *intensities);
   void set_def_image(int width, int height, const float *intensities);
   void set_def_image(int width, int height, const uint16_t *intensities);
+  // Schema::set_ref_image(const std::string &) / set_def_image(const std::string &)
+  // (DICe_Schema.cpp:351-394, 179-304): the whole image read by read_image
+  // (.rawi or TIFF), filtered and differentiated as the parameters say; the
+  // name goes into the output header.
+  void set_ref_image(const std::string &file_name);
+  void set_def_image(const std::string &file_name);
 
   // Schema::execute_correlation (DICe_Schema.cpp:1622-1792): all subsets of the
   // frame, then the frame id advances.  Returns 0.

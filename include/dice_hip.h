@@ -68,6 +68,8 @@ typedef struct dice_params {
   int32_t normalize_gamma_with_active_pixels;
   int32_t compute_uncertainty_fields;    /* FIELD_1..7 on converged Gauss-Newton subsets */
   int32_t gradient_method;               /* FINITE_DIFFERENCE=0, CONVOLUTION_5_POINT=1 */
+  int32_t gauss_filter_images;           /* Gauss-filter both images on upload (DICe_Image.cpp:215-230) */
+  int32_t gauss_filter_mask_size;        /* 5, 7, 9, 11 or 13 (default 7) */
 } dice_params;
 
 /* Per-subset field store: SoA of doubles, fields[slot * n + i]
@@ -111,9 +113,18 @@ int dice_hip_set_def_image(dice_hip_context *ctx, int width, int height, int sto
 
 /* Same, for pixels already in device memory (no copy; the pointer is used by
  * later correlate calls and must stay valid).  Gradients, if requested, go to
- * context-owned buffers. */
+ * context-owned buffers; with gauss_filter_images the filtered copy does too. */
 int dice_hip_set_def_image_device(dice_hip_context *ctx, int width, int height, int storage,
                                   const void *device_pixels);
+
+/* Image_::gauss_filter (DICe_Image.cpp:1194-1240) on device `device`: the
+ * mask_size x mask_size Gauss mask (5, 7, 9, 11 or 13) over the pixels at least
+ * mask_size / 2 + 1 from every edge, the others copied; the result is cast to
+ * the storage type (truncated for uint16).  host_in / host_out: width x height
+ * pixels of `storage` (may be the same buffer).  Synchronous.  The contexts
+ * apply the same filter to both images when gauss_filter_images is set. */
+int dice_hip_gauss_filter(int device, int width, int height, int storage, const void *host_in, void *host_out,
+                          int mask_size);
 
 /* The subset set of the Schema (DICe_Schema.cpp:141-150 + Subset ctors,
  * DICe_Subset.cpp:78-168).  Square subsets of side subset_size centred on

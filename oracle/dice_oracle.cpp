@@ -14,7 +14,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <set>
 #include <utility>
 #include <vector>
@@ -829,6 +832,47 @@ void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<
   f(OF_STATUS) = init_status; f(OF_ITERATIONS) = num_iterations;
 }
 
+// Image_::gauss_filter, DICe_Image.cpp:1192-1240.  Its masks (DICe_Image.cpp:54-126) are
+// the outer products c_i c_j of the 1-D factors below; each product is formed here in S
+// from the S-rounded factors, which reproduces the reference's filtered-image goldens
+// (tests/component/images/outFilter*.rawi float and outFilter*_d.rawi double) bit for bit
+// (the source tables hold the same products written to nine digits, an ulp away for a few
+// entries).  Pixels at least M/2+1 from every edge are replaced by
+// sum_i sum_j c[i*M+j] I(x+i-M/2, y+j-M/2) (x offset outer) in S, cast to the storage
+// type; uint16 results are clamped to 65535 first (the 13-point mask sums to 1.0002).
+template <typename S>
+bool gauss_table(int M, std::vector<S> &c) {
+  static const double f5[3] = {0.0014, 0.1574, 0.62825};
+  static const double f7[4] = {0.006, 0.0606, 0.2418, 0.3831};
+  static const double f9[5] = {0.0007, 0.0108, 0.0748, 0.2384, 0.3505};
+  static const double f11[6] = {0.0001, 0.0017, 0.0168, 0.087, 0.2328, 0.3231};
+  static const double f13[7] = {0.0001, 0.0012, 0.0085, 0.038, 0.1109, 0.2108, 0.2611};
+  const double *f = M == 5 ? f5 : M == 7 ? f7 : M == 9 ? f9 : M == 11 ? f11 : M == 13 ? f13 : nullptr;
+  if (!f) return false;
+  c.assign((size_t)M * M, S(0));
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < M; ++j) c[(size_t)i * M + j] = (S)f[i <= M / 2 ? i : M - 1 - i] * (S)f[j <= M / 2 ? j : M - 1 - j];
+  return true;
+}
+
+template <typename S, typename T>
+int gauss_filter_t(int M, int w, int h, T *img) {
+  std::vector<S> c;
+  if (!gauss_table<S>(M, c) || w < M || h < M) return -1;
+  const int hm = M / 2 + 1;
+  const std::vector<T> tmp(img, img + (size_t)w * h);
+  for (int y = hm; y < h - hm; ++y)
+    for (int x = hm; x < w - hm; ++x) {
+      S value = 0.0;
+      for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j)
+          value += c[(size_t)i * M + j] * (S)tmp[(size_t)(y + j - hm + 1) * w + x + (i - hm + 1)];
+      if (std::is_same<T, uint16_t>::value && value > (S)65535) value = (S)65535;
+      img[(size_t)y * w + x] = static_cast<T>(value);
+    }
+  return 0;
+}
+
 template <typename S, typename T>
 int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def, int n, int subset_size,
                 const int *co, const int *cxs, const int *cys, const unsigned char *cact, double *F,
@@ -840,6 +884,18 @@ int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def
   refi.h = defi.h = h;
   refi.I = ref;
   defi.I = def;
+  // both images are filtered as they are built, before their gradients (DICe_Image.cpp:303-318,
+  // DICe_Schema.cpp:313-314,429-431)
+  std::vector<T> ref_f, def_f;
+  if (P->gauss_filter_images) {
+    ref_f.assign(ref, ref + (size_t)w * h);
+    def_f.assign(def, def + (size_t)w * h);
+    if (gauss_filter_t<S, T>(P->gauss_filter_mask_size, w, h, ref_f.data()) ||
+        gauss_filter_t<S, T>(P->gauss_filter_mask_size, w, h, def_f.data()))
+      return -1;
+    refi.I = ref_f.data();
+    defi.I = def_f.data();
+  }
   // Ref gradients are always on for gradient methods (DICe_Schema.cpp:759-761) and by
   // default otherwise (compute_ref_gradients = true, :628); def gradients only on request.
   refi.compute_gradients_fd();
@@ -928,6 +984,7 @@ void oracle_default_params(oracle_params *p) {
   p->levenberg_marquardt_regularization_factor = -1.0;
   p->momentum_factor = 0.0;
   p->compute_uncertainty_fields = 1;
+  p->gauss_filter_mask_size = 7;  // DICe_Schema.cpp:465
 }
 
 void oracle_tracking_default_params(oracle_params *p) {
@@ -1044,6 +1101,17 @@ int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const 
                        int n, int subset_size, const int *co, const int *cx, const int *cy,
                        const unsigned char *ca, double *fields, int first_frame, int threads) {
   return correlate_any<double>(p, w, h, storage, ref, def, n, subset_size, co, cx, cy, ca, fields, first_frame, threads);
+}
+
+int oracle_gauss_filter_f(int w, int h, int storage, void *img, int mask_size) {
+  if (!img) return -1;
+  if (storage == 0) return gauss_filter_t<float, float>(mask_size, w, h, (float *)img);
+  if (storage == 1) return gauss_filter_t<float, uint16_t>(mask_size, w, h, (uint16_t *)img);
+  return -1;
+}
+int oracle_gauss_filter_d(int w, int h, double *img, int mask_size) {
+  if (!img) return -1;
+  return gauss_filter_t<double, double>(mask_size, w, h, img);
 }
 
 int oracle_lu_inverse(int n, double *a) {

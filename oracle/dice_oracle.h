@@ -70,6 +70,8 @@ typedef struct oracle_params {
   int32_t normalize_gamma_with_active_pixels;
   int32_t compute_uncertainty_fields;
   int32_t gradient_method;               /* FINITE_DIFFERENCE=0 CONVOLUTION_5_POINT=1 */
+  int32_t gauss_filter_images;           /* Gauss-filter both images first (DICe_Image.cpp:215-230) */
+  int32_t gauss_filter_mask_size;        /* 5, 7, 9, 11, 13 */
 } oracle_params;
 
 /* Field slots of the per-subset SoA double store: fields[slot*n + i]. */
@@ -117,6 +119,12 @@ int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const 
                        int n, int subset_size, const int *conf_offsets, const int *conf_x,
                        const int *conf_y, const unsigned char *conf_active, double *fields,
                        int first_frame, int threads);
+
+/* Image_::gauss_filter in place (mask 5, 7, 9, 11 or 13): float arithmetic over float32
+ * (storage 0) or uint16 (storage 1) pixels, or double arithmetic over double pixels.
+ * Returns 0, or -1 on a bad mask or an image smaller than the mask. */
+int oracle_gauss_filter_f(int w, int h, int storage, void *img, int mask_size);
+int oracle_gauss_filter_d(int w, int h, double *img, int mask_size);
 
 /* 6x6 LU with partial pivoting + inverse (GETRF/GETRI semantics); returns INFO. */
 int oracle_lu_inverse(int n, double *a_rowmajor);

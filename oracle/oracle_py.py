@@ -49,6 +49,8 @@ class Params(ctypes.Structure):
         ("normalize_gamma_with_active_pixels", ctypes.c_int32),
         ("compute_uncertainty_fields", ctypes.c_int32),
         ("gradient_method", ctypes.c_int32),
+        ("gauss_filter_images", ctypes.c_int32),
+        ("gauss_filter_mask_size", ctypes.c_int32),
     ]
 
 
@@ -79,6 +81,8 @@ def lib():
         _lib.oracle_interp_d.argtypes = [i, i, i, P, i, P, P, P]
         _lib.oracle_polygon_pixels.argtypes = [i, P, P, i, P, P]
         _lib.oracle_lu_inverse.argtypes = [i, P]
+        _lib.oracle_gauss_filter_f.argtypes = [i, i, i, P, i]
+        _lib.oracle_gauss_filter_d.argtypes = [i, i, P, i]
     return _lib
 
 
@@ -209,6 +213,22 @@ def rectangle_pixels(cx, cy, width, height):
     ox, oy = cx - width // 2, cy - height // 2
     yy, xx = np.mgrid[oy:oy + height, ox:ox + width]
     return xx.ravel().astype(np.int32), yy.ravel().astype(np.int32)
+
+
+def gauss_filter(img, mask_size):
+    """Image_::gauss_filter on a copy of img: float32 and uint16 images in float arithmetic,
+    float64 images in double arithmetic."""
+    out = np.ascontiguousarray(img).copy()
+    h, w = out.shape
+    if out.dtype == np.float64:
+        rc = lib().oracle_gauss_filter_d(w, h, _p(out), mask_size)
+    elif out.dtype in (np.float32, np.uint16):
+        rc = lib().oracle_gauss_filter_f(w, h, 0 if out.dtype == np.float32 else 1, _p(out), mask_size)
+    else:
+        raise TypeError("gauss_filter: float32, uint16 or float64 image")
+    if rc != 0:
+        raise ValueError("gauss_filter: bad mask size or image too small")
+    return out
 
 
 def lu_inverse(a):

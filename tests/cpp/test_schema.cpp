@@ -10,6 +10,8 @@
 //   test_schema deck <config>      (a regression deck end to end, see deck())
 //   test_schema grid <w> <h> <subset size> <step>   (prints "x y neighbor" of the regular grid)
 //   test_schema errors             (error behaviour that needs no GPU)
+//   test_schema read_image <image file> <out.rawi> [ox oy w h]   (read_image, then write_rawi_image)
+//   test_schema tiff_samples <tiff file> <out>   (uint32 w, h, bits, then the uint16 samples)
 //
 // Images are raw row-major float32.  The exit status is the number of failures.
 #include <cmath>
@@ -192,11 +194,13 @@ int subset_file_info(const char *path) {
 }
 
 // A deck of the reference's regression suite through the Schema mirror.  The
-// configuration file has "key value" lines: ref / def (raw float32 images),
+// configuration file has "key value" lines: ref / def (raw float32 images,
+// or ref_file / def_file: image files read by Schema::set_ref_image(name)),
 // width, height, subset_size, subset_file or step_size, out_dir, prefix,
 // separate (0|1), backend (0|1), ref_name, def_name, routine (0|1), and
-// optional fields (comma-separated output names) and optimization /
-// interpolation (enum values).  Default parameters otherwise; one frame.
+// optional fields (comma-separated output names), optimization /
+// interpolation (enum values) and gauss_filter_mask_size (turns the Gauss
+// filter on).  Default parameters otherwise; one frame.
 int deck(const char *config_path) {
   std::ifstream cfg(config_path);
   std::map<std::string, std::string> c;
@@ -204,13 +208,22 @@ int deck(const char *config_path) {
   while (cfg >> key && std::getline(cfg >> std::ws, value)) c[key] = value;
   auto num = [&](const char *k) { return std::atoi(c.at(k).c_str()); };
   const int w = num("width"), h = num("height");
-  const std::vector<float> ref = read_raw(c.at("ref").c_str(), w, h), def = read_raw(c.at("def").c_str(), w, h);
+  const bool files = c.count("ref_file") != 0;
+  std::vector<float> ref, def;
+  if (!files) {
+    ref = read_raw(c.at("ref").c_str(), w, h);
+    def = read_raw(c.at("def").c_str(), w, h);
+  }
   ParameterList params;
   dice_default_params(&params);
   params.set("correlation_routine", num("routine"));
   params.set("correlation_backend", num("backend"));
   if (c.count("optimization")) params.set("optimization_method", num("optimization"));
   if (c.count("interpolation")) params.set("interpolation_method", num("interpolation"));
+  if (c.count("gauss_filter_mask_size")) {
+    params.set("gauss_filter_images", 1);
+    params.set("gauss_filter_mask_size", num("gauss_filter_mask_size"));
+  }
   std::vector<double> cx, cy;
   std::vector<int> neighbors;
   Subset_File_Info info;
@@ -224,9 +237,14 @@ int deck(const char *config_path) {
   }
   Schema schema(cx, cy, num("subset_size"), &info.conformal_area_defs, &params);
   if (grid) schema.set_step_sizes(num("step_size"), num("step_size"));
+  if (files) {
+    schema.set_ref_image(c.at("ref_file"));
+    schema.set_def_image(c.at("def_file"));
+  } else {
+    schema.set_ref_image(w, h, ref.data());
+    schema.set_def_image(w, h, def.data());
+  }
   schema.set_image_file_names(c.at("ref_name"), c.at("def_name"));
-  schema.set_ref_image(w, h, ref.data());
-  schema.set_def_image(w, h, def.data());
   schema.execute_correlation();
   std::vector<std::string> fields;
   if (c.count("fields")) {
@@ -280,6 +298,32 @@ int main(int argc, char **argv) {
     if (cmd == "errors") return errors_without_gpu();
     if (cmd == "subset_file_info" && argc >= 3) return subset_file_info(argv[2]);
     if (cmd == "deck" && argc >= 3) return deck(argv[2]);
+    if (cmd == "read_image" && argc >= 4) {
+      std::vector<float> pix;
+      int w = 0, h = 0;
+      if (argc >= 8)
+        read_image(argv[2], pix, w, h, std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]));
+      else
+        read_image(argv[2], pix, w, h);
+      int dw = 0, dh = 0;
+      if (argc < 8) {
+        read_image_dimensions(argv[2], dw, dh);
+        if (dw != w || dh != h) return 1;
+      }
+      write_rawi_image(argv[3], w, h, pix.data());
+      return 0;
+    }
+    if (cmd == "tiff_samples" && argc >= 4) {
+      std::vector<uint16_t> v;
+      int w = 0, h = 0, bits = 0;
+      read_tiff_samples(argv[2], v, w, h, bits);
+      std::FILE *f = std::fopen(argv[3], "wb");
+      if (!f) return 1;
+      const uint32_t hdr[3] = {(uint32_t)w, (uint32_t)h, (uint32_t)bits};
+      const bool ok = std::fwrite(hdr, sizeof hdr, 1, f) == 1 && std::fwrite(v.data(), 2, v.size(), f) == v.size();
+      std::fclose(f);
+      return ok ? 0 : 1;
+    }
     if (cmd == "grid" && argc >= 6) {
       std::vector<double> x, y;
       std::vector<int> nb;
