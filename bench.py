@@ -28,6 +28,11 @@ the 6 parameters of every subset and dL/dI_def into a zeroed 1536^2 image.
 under translation + shear up to 2 px, 9 x 9 subsets at step 4 (760,920
 subsets), Nelder-Mead over the 6 affine parameters from a zero initial guess,
 200-iteration cap, bilinear interpolation.  One step = one frame pair.
+
+--workload gauss (image ingest): random --size^2 float32 frames (2048 by
+default) resident in HBM, each set as the deformed image with
+gauss_filter_images on (--mask, 7 by default), no gradients.  One step = one
+frame filtered.
 """
 import argparse
 import json
@@ -626,12 +631,86 @@ def bench_c5(args):
     return result
 
 
+def bench_gauss(args):
+    """Image ingest: the Gauss filter applied as a deformed frame is set (gauss_filter_images),
+    device-resident frames, timed on the context's stream."""
+    import torch
+    import dice_amd as dh
+
+    size, mask = args.size, args.mask
+    rng = np.random.default_rng(3)
+    img = (rng.random((size, size)) * 255).astype(np.float32)
+    p = dh.default_params()
+    p.gauss_filter_images = 1
+    p.gauss_filter_mask_size = mask
+    p.compute_def_gradients = 0
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(img[:64, :64])
+    dev = torch.device("cuda:0")
+    frames = [torch.from_numpy(np.roll(img, k, axis=1)).to(dev) for k in range(4)]
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for k in range(args.warmup):
+        ctx.set_def_image_device(frames[k % 4].data_ptr(), size, size, dh.STORAGE_F32)
+    ctx.synchronize()
+    steps = max(args.steps, 20)
+    t = time.perf_counter()
+    for k in range(steps):
+        ctx.set_def_image_device(frames[k % 4].data_ptr(), size, size, dh.STORAGE_F32)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    # batches of 10 back-to-back launches between two events, so the queue never runs dry
+    times = []
+    for _ in range(max(3, steps // 10)):
+        with torch.cuda.stream(stream):
+            ev[0].record(stream)
+        for k in range(10):
+            ctx.set_def_image_device(frames[k % 4].data_ptr(), size, size, dh.STORAGE_F32)
+        with torch.cuda.stream(stream):
+            ev[1].record(stream)
+        ctx.synchronize()
+        times.append(ev[0].elapsed_time(ev[1]) / 10)
+    kernel_ms = float(np.mean(times))
+    log("gauss %d on %d^2: %d frames in %.4f s, %.4f ms per launch" % (mask, size, steps, el, kernel_ms))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as o
+        sample = img[: max(mask + 2, size // 8)]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            o.gauss_filter(sample, mask)
+            reps += 1
+        cpu_s = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(sample.size / cpu_s / (size * size), 3), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": "the oracle's Image_::gauss_filter (float, one thread) on the first %d rows of the frame, "
+                         "%d repetitions, scaled to whole frames" % (sample.shape[0], reps)}
+    px = size * size
+    flops = 2.0 * mask * mask * px
+    result = result_line(
+        args, "Gauss-filtered frames/s (Image_::gauss_filter on image upload), %d^2 float32, mask %d" % (size, mask),
+        steps / el, "frames/s", 1000.0 * el / steps, "f32",
+        {"workload": "random %dx%d float32 frames in HBM, gauss_filter_images with mask %d, set as the deformed "
+                     "image (no gradients)" % (size, size, mask),
+         "kernel_ms": round(kernel_ms, 4),
+         "valu_frac": round(flops / (kernel_ms * 1e-3) / 1e12 / 157.3, 4),
+         "flop_per_launch": flops},
+        roofline(8.0 * px, kernel_ms, "gauss_filter_kernel<float, %d>" % mask, 10 * len(times)),
+        cpu_baseline=cpu)
+    ctx.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "gauss"], default="c2")
+    ap.add_argument("--mask", type=int, default=7, help="gauss: mask size (5, 7, 9, 11, 13)")
     ap.add_argument("--size", type=int, default=2048, help="c2: image side")
     ap.add_argument("--frames", type=int, default=0, help="c2: 10, c3: 100 when 0")
     ap.add_argument("--subsets", type=int, default=0, help="c3: 50000 when 0")
@@ -661,7 +740,7 @@ def main():
         print("bench.py runs on one GPU only: the correlation of one device's subset share is the "
               "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
-    run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+    run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "gauss": bench_gauss}[args.workload]
     print(json.dumps(run(args)), flush=True)
     return 0
 

@@ -33,11 +33,13 @@ STORAGE_U16 = 1
 
 # Status_Flag values (DICe.h:609-677) used by the path
 INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL = 1
+INITIALIZE_USING_NEIGHBOR_VALUE_SUCCESSFUL = 3
 CONVERGED_STATUSES = (1, 2, 3, 4)
 
 # enum values of DICe.h
 BILINEAR, BICUBIC, KEYS_FOURTH = 0, 1, 2
 SIMPLEX, GRADIENT_BASED, GRADIENT_THEN_SEARCH, SIMPLEX_THEN_GRADIENT_BASED, GRADIENT_BASED_THEN_SIMPLEX = 0, 1, 2, 3, 4
+USE_FIELD_VALUES, USE_NEIGHBOR_VALUES, USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY = 0, 1, 2
 
 
 class Params(ctypes.Structure):
@@ -70,6 +72,7 @@ class Params(ctypes.Structure):
         ("gradient_method", ctypes.c_int32),
         ("gauss_filter_images", ctypes.c_int32),
         ("gauss_filter_mask_size", ctypes.c_int32),
+        ("neighbor_settle_tolerance", ctypes.c_double),
     ]
 
 
@@ -82,6 +85,7 @@ EXPORTS = [
     "dice_hip_synchronize", "dice_hip_stream", "dice_hip_kernel_timing", "dice_hip_set_kernel_timing",
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
+    "dice_hip_neighbor_rounds",
 ]
 
 _lib = None
@@ -139,6 +143,7 @@ def lib():
         L.dice_hip_gamma.argtypes = [P, I, P, P]
         L.dice_hip_gamma_adjoint.argtypes = [P, I, P, P, P, P]
         L.dice_hip_gauss_filter.argtypes = [I, I, I, I, P, P, I]
+        L.dice_hip_neighbor_rounds.argtypes = [P]
         _lib = L
     return _lib
 
@@ -176,6 +181,24 @@ def regular_grid(width, height, subset_size, step):
     ys = subset_size + np.arange(ny) * step - 1
     gx, gy = np.meshgrid(xs, ys)
     return gx.ravel().astype(np.float64), gy.ravel().astype(np.float64)
+
+
+def serpentine_grid(width, height, subset_size, step):
+    """create_regular_grid_of_correlation_points (DICe_Decomp.cpp:868-1070) over the whole image with no
+    seed: centres subset_size - 1 + k * step down the first column, up the next, ...; neighbour ids -1 for
+    the first point and for the first point of the serpentine after it (the reference gives that one the
+    seed's id, -1 without a seed), else the previous point.  Returns (cx, cy, neighbor_ids)."""
+    xs = list(range(subset_size - 1, width - subset_size, step))
+    ys = list(range(subset_size - 1, height - subset_size, step))
+    cx, cy = [], []
+    for c, x in enumerate(xs):
+        for y in (ys if c % 2 == 0 else ys[::-1]):
+            if x <= width - subset_size and y <= height - subset_size:
+                cx.append(x)
+                cy.append(y)
+    nb = np.arange(-1, len(cx) - 1, dtype=np.float64)
+    nb[1:2] = -1.0
+    return np.array(cx, np.float64), np.array(cy, np.float64), nb
 
 
 class Context:
@@ -263,6 +286,10 @@ class Context:
                                                           ctypes.c_void_p(fields_ptr),
                                                           ctypes.c_void_p(history_ptr) if history_ptr else None,
                                                           1 if use_graph else 0), "correlate_sequence")
+
+    def neighbor_rounds(self):
+        """Correlation rounds of the last frame with neighbour initialization (0 otherwise)."""
+        return self._lib.dice_hip_neighbor_rounds(self.h)
 
     def synchronize(self):
         self._check(self._lib.dice_hip_synchronize(self.h), "synchronize")

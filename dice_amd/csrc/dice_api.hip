@@ -80,6 +80,9 @@ struct dice_hip_context {
   const void *def_ptr = nullptr;
   DevBuf def_gx, def_gy, tmp_x, tmp_y;
   DevBuf ref_raw, def_filt;  // upload target of a filtered reference image; the filtered deformed image
+  DevBuf nb_start, nb_guess[2], nb_list, nb_counts;  // USE_NEIGHBOR_VALUES: frame-start fields, guesses,
+                                                     // subsets to correlate again, their count per round
+  int nb_rounds = 0;                         // correlation rounds of the last neighbour-initialized frame
   bool def_has_grads = false;
   // subsets
   int n = 0, S = 0, fov = 0;
@@ -131,8 +134,12 @@ bool params_valid(const dice_params *p, std::string &why) {
           "GRADIENT_BASED_THEN_SIMPLEX";
     return false;
   }
-  if (p->initialization_method != 0) {
-    why = "only USE_FIELD_VALUES initialization is supported";
+  if (!(p->neighbor_settle_tolerance >= 0.0)) {
+    why = "neighbor_settle_tolerance must be >= 0";
+    return false;
+  }
+  if (p->initialization_method < 0 || p->initialization_method > 2) {
+    why = "initialization_method must be USE_FIELD_VALUES, USE_NEIGHBOR_VALUES or USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY";
     return false;
   }
   if (!p->enable_translation) {
@@ -240,11 +247,13 @@ hipEvent_t *next_event_pair(dice_hip_context *ctx) {
   return &ctx->events[ctx->events_used++].first;
 }
 
-int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int first_frame, bool capturing) {
+int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int first_frame, bool capturing,
+                      const double *guess = nullptr, const int *items = nullptr, const int *item_count = nullptr) {
   DImage ref, def;
   DSubsets ss;
   fill_views(ctx, ref, def, ss);
-  const KParams kp = make_kparams(ctx->params, first_frame);
+  KParams kp = make_kparams(ctx->params, first_frame);
+  kp.guess = guess;
   const Launch L = current_launch(ctx, mode);
   // per-launch timing events on the context stream (eager launches only; a captured
   // graph is timed as a whole by the caller)
@@ -256,7 +265,8 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
     ev = ctx->events[ctx->events_used - 1];
     HIP_TRY(ctx, hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
   }
-  HIP_TRY(ctx, dice_hip::launch_correlate(L, ctx->rstorage, kp, ref, def, ss, d_fields, ctx->work.as<int>(), ctx->stream),
+  HIP_TRY(ctx, dice_hip::launch_correlate(L, ctx->rstorage, kp, ref, def, ss, d_fields, ctx->work.as<int>(), ctx->stream,
+                                          items, item_count),
           "correlation launch");
   if (timed) HIP_TRY(ctx, hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
   return DICE_HIP_OK;
@@ -327,7 +337,8 @@ int dice_hip_destroy(dice_hip_context *ctx) {
   }
   for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
-                    &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->fields, &ctx->work})
+                    &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
+                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->fields, &ctx->work})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -559,6 +570,73 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
   return DICE_HIP_OK;
 }
 
+static bool neighbor_mode(const dice_hip_context *ctx, int first_frame) {
+  return ctx->params.initialization_method == 1 || (ctx->params.initialization_method == 2 && first_frame);
+}
+
+// USE_NEIGHBOR_VALUES.  The reference correlates the subsets one after another,
+// each starting from the values of its neighbour (NEIGHBOR_ID_FS): this frame's
+// result when the neighbour comes earlier in subset order, else its frame-start
+// values.  Here round 0 correlates every subset at once from the guesses
+// gathered from the frame-start fields; each later round gathers the guesses
+// again from the latest results and correlates again only the subsets whose
+// guess changed, each from its own frame-start fields.  A subset's result is a
+// fixed function of its guess, so when a round finds no guess changed, every
+// result is the one its current guess gives, and these are the sequential
+// results: the one solution, found subset by subset in order.  Round r has
+// every subset right whose chain of earlier neighbours is shorter than r, so
+// the rounds end after at most n + 1.  With neighbor_settle_tolerance > 0 a
+// guess counts as changed only when it moves by more than the tolerance.
+// Rounds are enqueued kNeighborBatch at a time between two waits on the host.
+static constexpr int kNeighborBatch = 16;
+
+static int correlate_neighbors(dice_hip_context *ctx, int mode, double *d_fields, int first_frame) {
+  const int n = ctx->n;
+  const size_t per = (size_t)DICE_F_COUNT * n;
+  std::vector<double> ids(n);
+  HIP_TRY(ctx, hipMemcpyAsync(ids.data(), d_fields + (size_t)DICE_F_NEIGHBOR_ID * n, n * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy neighbor ids");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "neighbor ids");
+  for (int i = 0; i < n; ++i)
+    if (ids[i] != (double)(int)ids[i] || ids[i] < -1.0 || ids[i] >= n)
+      return ctx->fail(DICE_HIP_EINVAL, "NEIGHBOR_ID must be -1 or the id of a subset");
+  HIP_TRY(ctx, ctx->nb_start.ensure(per * sizeof(double)), "hipMalloc neighbor fields");
+  HIP_TRY(ctx, ctx->nb_guess[0].ensure((size_t)dice_hip::kGuessSlots * n * sizeof(double)), "hipMalloc guesses");
+  HIP_TRY(ctx, ctx->nb_guess[1].ensure((size_t)dice_hip::kGuessSlots * n * sizeof(double)), "hipMalloc guesses");
+  HIP_TRY(ctx, ctx->nb_list.ensure((size_t)n * sizeof(int)), "hipMalloc neighbor list");
+  HIP_TRY(ctx, ctx->nb_counts.ensure(kNeighborBatch * sizeof(int)), "hipMalloc neighbor counts");
+  double *start = ctx->nb_start.as<double>();
+  int *list = ctx->nb_list.as<int>(), *counts = ctx->nb_counts.as<int>();
+  const double tol = ctx->params.neighbor_settle_tolerance;
+  HIP_TRY(ctx, dice_hip::launch_copy_doubles(start, d_fields, per, ctx->stream), "copy fields");
+  const double *prev = nullptr;
+  int which = 0, round = 0;
+  while (true) {
+    HIP_TRY(ctx, dice_hip::launch_zero_ints(counts, kNeighborBatch, ctx->stream), "zero counts");
+    for (int b = 0; b < kNeighborBatch; ++b) {
+      double *guess = ctx->nb_guess[which].as<double>();
+      HIP_TRY(ctx, dice_hip::launch_neighbor_guess(start, d_fields, guess, prev, tol, list, counts + b, n, ctx->stream),
+              "neighbor guess");
+      HIP_TRY(ctx, dice_hip::launch_reset_listed(list, counts + b, start, d_fields, n, ctx->stream), "reset fields");
+      int rc = enqueue_correlate(ctx, mode, d_fields, first_frame, false, guess, list, counts + b);
+      if (rc) return rc;
+      prev = guess;
+      which ^= 1;
+    }
+    int host_counts[kNeighborBatch];
+    HIP_TRY(ctx, hipMemcpyAsync(host_counts, counts, sizeof host_counts, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpy counts");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "neighbor rounds");
+    for (int b = 0; b < kNeighborBatch; ++b)
+      if (host_counts[b] == 0) {
+        ctx->nb_rounds = round + b;  // rounds that correlated something
+        return DICE_HIP_OK;
+      }
+    round += kNeighborBatch;
+    if (round > n + 1 + kNeighborBatch) return ctx->fail(DICE_HIP_EHIP, "neighbor initialization did not settle");
+  }
+}
+
 int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame) {
   if (!ctx) return DICE_HIP_EINVAL;
   if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
@@ -566,8 +644,12 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
   int rc = check_ready(ctx);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
+  if (neighbor_mode(ctx, first_frame)) return correlate_neighbors(ctx, mode, device_fields, first_frame);
+  ctx->nb_rounds = 0;
   return enqueue_correlate(ctx, mode, device_fields, first_frame, false);
 }
+
+int dice_hip_neighbor_rounds(dice_hip_context *ctx) { return ctx ? ctx->nb_rounds : DICE_HIP_EINVAL; }
 
 static int enqueue_gamma(dice_hip_context *ctx, int mode, const dice_hip::GammaArgs &ga) {
   if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
@@ -624,6 +706,9 @@ int dice_hip_correlate_sequence(dice_hip_context *ctx, int mode, int n_frames, c
   (void)hipSetDevice(ctx->device);
   if (ctx->params.compute_def_gradients)
     return ctx->fail(DICE_HIP_EINVAL, "sequences need compute_def_gradients off (gradients are per frame)");
+  if (ctx->params.initialization_method != 0)
+    return ctx->fail(DICE_HIP_EINVAL, "sequences need USE_FIELD_VALUES (the neighbour rounds of USE_NEIGHBOR_VALUES "
+                                      "wait on the host between launches)");
   if (!ctx->def_ptr) {
     int rc = set_def_common(ctx, ctx->rw, ctx->rh, ctx->rstorage, device_frames[0]);
     if (rc) return rc;

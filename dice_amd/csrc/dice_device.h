@@ -1842,14 +1842,18 @@ struct Solver {
     f(F_ITERATIONS) = iters;
   }
 
+  // initialize_parameters_from_fields of the subset the initializer picked:
+  // the subset itself, or its neighbour's values gathered into kp.guess.
   __device__ __forceinline__ void init_params(Affine &A, const double *fields, int nsub, int gid) {
-    auto f = [&](int s) { return fields[(size_t)s * nsub + gid]; };
-    A.p[0] = (float)f(F_DISP_X);
-    A.p[1] = (float)f(F_DISP_Y);
-    A.p[2] = kp.en_rot ? (float)f(F_ROT_Z) : 0.0f;
-    A.p[3] = kp.en_normal ? (float)f(F_NS_XX) : 0.0f;
-    A.p[4] = kp.en_normal ? (float)f(F_NS_YY) : 0.0f;
-    A.p[5] = kp.en_shear ? (float)f(F_SS_XY) : 0.0f;
+    auto f = [&](int s, int k) {
+      return kp.guess ? kp.guess[(size_t)k * nsub + gid] : fields[(size_t)s * nsub + gid];
+    };
+    A.p[0] = (float)f(F_DISP_X, 0);
+    A.p[1] = (float)f(F_DISP_Y, 1);
+    A.p[2] = kp.en_rot ? (float)f(F_ROT_Z, 2) : 0.0f;
+    A.p[3] = kp.en_normal ? (float)f(F_NS_XX, 3) : 0.0f;
+    A.p[4] = kp.en_normal ? (float)f(F_NS_YY, 4) : 0.0f;
+    A.p[5] = kp.en_shear ? (float)f(F_SS_XY, 5) : 0.0f;
   }
 
   // One subset of the differentiation path: gamma at p (forward) or the
@@ -1912,15 +1916,15 @@ struct Solver {
           return;
         }
       }
-      // Field_Value_Initializer (DICe_Initializer.cpp:398-434)
-      if (fv(F_SIGMA) == -1.0) {
+      // Field_Value_Initializer (DICe_Initializer.cpp:398-434): the sigma of the source subset
+      if ((kp.guess ? kp.guess[(size_t)6 * nsub + gid] : fv(F_SIGMA)) == -1.0) {
         record_failed(fields, nsub, gid, INITIALIZE_FAILED, -1);
         return;
       }
     }
     Affine A;
     init_params(A, fields, nsub, gid);
-    const int init_status = INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
+    const int init_status = kp.guess ? (int)kp.guess[(size_t)7 * nsub + gid] : INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
     int iters = -1;
     if constexpr (!SIMPLEX_OPT && kFast) {
       precompute_phi();  // and meanF

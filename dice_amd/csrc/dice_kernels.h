@@ -49,7 +49,12 @@ struct KParams {
   int normalize_gamma;
   int uncertainty;
   int first_frame;
+  // USE_NEIGHBOR_VALUES: per-subset initial guesses, slot-major n x kGuessSlots
+  // (u, v, theta, ex, ey, gxy, sigma of the source subset, init status), or null
+  // to start every subset from its own fields (USE_FIELD_VALUES)
+  const double *guess;
 };
+constexpr int kGuessSlots = 8;
 
 // Device image: pixels (float or uint16) + optional float gradients.
 struct DImage {
@@ -142,9 +147,27 @@ inline size_t work_ints(int n) { return kWorkHeader + (3 + kNmStride) * (size_t)
 // Enqueue the correlation of all subsets (primary pass, then the fallback pass
 // when the optimization method has one).  storage: 0 f32, 1 u16.  work: device
 // ints, at least work_ints(n) (counters, fallback and overflow lists).
+// Field_Value_Initializer with USE_NEIGHBOR_VALUES (DICe_Initializer.cpp:398-434)
+// for one round of the neighbour iteration: the guess of subset i from the
+// fields of i's neighbour (NEIGHBOR_ID of start; itself when -1): from cur when
+// the neighbour comes earlier in subset order (the reference has already
+// correlated it in this frame), else from start (its frame-start values).
+// prev_guess null (the first round): every subset goes to list.  Otherwise a
+// subset whose guess moved by more than tol (any parameter), or whose sigma = -1
+// flag or init status changed, takes the new guess and goes to list; the others
+// keep prev_guess.  list / count (zeroed by the caller) receive the subsets to
+// correlate again; fields of the listed subsets are then reset to start by
+// launch_reset_listed.
+hipError_t launch_neighbor_guess(const double *start, const double *cur, double *guess, const double *prev_guess,
+                                 double tol, int *list, int *count, int n, hipStream_t stream);
+hipError_t launch_reset_listed(const int *list, const int *count, const double *start, double *fields, int n,
+                               hipStream_t stream);
+
+// Enqueue the correlation of all subsets, or of the device list items (count
+// *item_count) when items is not null.
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,
                             const DImage &def, const DSubsets &subs, double *fields, int *work,
-                            hipStream_t stream);
+                            hipStream_t stream, const int *items = nullptr, const int *item_count = nullptr);
 
 // Enqueue the forward gamma or the adjoint of every subset (BATCHED, then the
 // PLAIN kernel over the subsets whose window did not fit).

@@ -156,7 +156,58 @@ __global__ void copy_doubles_kernel(double *dst, const double *src, size_t count
     dst[i] = src[i];
 }
 
+// One subset per thread: gather the initial guess of subset i from its
+// neighbour's fields (see launch_neighbor_guess).
+__global__ void neighbor_guess_kernel(const double *start, const double *cur, double *guess, const double *prev,
+                                      double tol, int *list, int *count, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nb = (int)start[(size_t)F_NEIGHBOR_ID * n + i];
+  const bool self = nb < 0 || nb == i;  // the host has checked nb < n
+  const int j = self ? i : nb;
+  const double *src = (!self && nb < i) ? cur : start;
+  const int slots[7] = {F_DISP_X, F_DISP_Y, F_ROT_Z, F_NS_XX, F_NS_YY, F_SS_XY, F_SIGMA};
+  double g[kGuessSlots];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) g[k] = src[(size_t)slots[k] * n + j];
+  g[7] = self ? 1.0 : 3.0;  // INITIALIZE_USING_PREVIOUS_FRAME / _NEIGHBOR_VALUE_SUCCESSFUL
+  bool redo = prev == nullptr;
+  if (!redo) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) redo |= tol > 0.0 ? fabs(g[k] - prev[(size_t)k * n + i]) > tol : g[k] != prev[(size_t)k * n + i];
+    const double ps = prev[(size_t)6 * n + i];
+    redo |= tol > 0.0 ? (g[6] == -1.0) != (ps == -1.0) : g[6] != ps;
+    redo |= g[7] != prev[(size_t)7 * n + i];
+  }
+#pragma unroll
+  for (int k = 0; k < kGuessSlots; ++k) guess[(size_t)k * n + i] = redo ? g[k] : prev[(size_t)k * n + i];
+  if (redo) list[atomicAdd(count, 1)] = i;
+}
+
+__global__ void reset_listed_kernel(const int *list, const int *count, const double *start, double *fields, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= *count) return;
+  const int i = list[k];
+#pragma unroll
+  for (int s = 0; s < kNumFields; ++s) fields[(size_t)s * n + i] = start[(size_t)s * n + i];
+}
+
 }  // namespace
+
+hipError_t launch_neighbor_guess(const double *start, const double *cur, double *guess, const double *prev_guess,
+                                 double tol, int *list, int *count, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(neighbor_guess_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, start, cur, guess, prev_guess,
+                     tol, list, count, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_listed(const int *list, const int *count, const double *start, double *fields, int n,
+                               hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reset_listed_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, list, count, start, fields, n);
+  return hipGetLastError();
+}
 
 // One optimizer pass over `items` (nullptr: all subsets).  BATCHED mode runs the
 // persistent kernel, then the PLAIN kernel over the subsets it handed over.
@@ -197,7 +248,8 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
 }
 
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
-                            const DSubsets &subs, double *fields, int *work, hipStream_t stream) {
+                            const DSubsets &subs, double *fields, int *work, hipStream_t stream, const int *items,
+                            const int *item_count) {
   if (subs.n <= 0) return hipSuccess;
   // work: the header (range counters, fallback and overflow counts), then the
   // fallback list and the overflow lists of pass 0 and pass 1 (n each)
@@ -207,7 +259,7 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
   hipError_t e = launch_zero_ints(work, kWorkHeader, stream);
   if (e != hipSuccess) return e;
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
-  PassArgs pa{0, nullptr, nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work, nullptr};
+  PassArgs pa{0, items, items ? item_count : nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work, nullptr};
   e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, nm, stream);
   if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
   PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work, nullptr};

@@ -50,6 +50,7 @@ const std::map<std::string, Slot> &slots() {
       {"gradient_method", {I32, offsetof(dice_params, gradient_method)}},
       {"gauss_filter_images", {I32, offsetof(dice_params, gauss_filter_images)}},
       {"gauss_filter_mask_size", {I32, offsetof(dice_params, gauss_filter_mask_size)}},
+      {"neighbor_settle_tolerance", {F64, offsetof(dice_params, neighbor_settle_tolerance)}},
   };
   return m;
 }
@@ -376,6 +377,57 @@ Subset_File_Info read_subset_file(const std::string &file_name, int width, int h
       info.conformal_area_defs[id] = def;
       info.blocking_subset_ids[id] = blocking;
       if (force_simplex) info.force_simplex.push_back(id);
+    } else if (t[1] == "REGION_OF_INTEREST") {
+      // DICe_Parser.cpp:1173-1367
+      const int roi = (int)info.conformal_area_defs.size();
+      info.region_of_interest = true;
+      Conformal_Area_Def def;
+      while (r.next(t)) {
+        if (t[0] == "END") break;
+        if (t[0] == "BEGIN" && t.size() >= 2) {
+          if (t[1] == "BOUNDARY") {
+            def.boundary = read_shapes(r);
+            if (def.boundary.empty()) r.fail("cannot define a boundary with zero shapes");
+          } else if (t[1] == "EXCLUDED") {
+            def.excluded_area = read_shapes(r);
+            if (def.excluded_area.empty()) r.fail("cannot define an exclusion with zero shapes");
+          } else if (t[1] == "SEED") {
+            bool has_location = false, has_disp = false;
+            while (r.next(t)) {
+              if (t[0] == "END") break;
+              auto num = [&](size_t k) {
+                if (t.size() <= k || !is_number(t[k])) r.fail("not enough values specified for seed " + t[0]);
+                return std::strtod(t[k].c_str(), nullptr);
+              };
+              if (t[0] == "LOCATION") {
+                info.size_map[roi] = {(int)std::strtol(t.size() > 1 ? t[1].c_str() : "", nullptr, 0),
+                                      (int)std::strtol(t.size() > 2 ? t[2].c_str() : "", nullptr, 0)};
+                num(1);
+                num(2);
+                has_location = true;
+              } else if (t[0] == "DISPLACEMENT") {
+                info.displacement_map[roi] = {num(1), num(2)};
+                has_disp = true;
+              } else if (t[0] == "NORMAL_STRAIN") {
+                info.normal_strain_map[roi] = {num(1), num(2)};
+              } else if (t[0] == "SHEAR_STRAIN") {
+                info.shear_strain_map[roi] = num(1);
+              } else if (t[0] == "ROTATION") {
+                info.rotation_map[roi] = num(1);
+              } else {
+                r.fail("unrecognized command in seed block " + t[0]);
+              }
+            }
+            if (!has_location) r.fail("seed must have location specified");
+            if (!has_disp) r.fail("seed must have displacement guess specified");
+          } else {
+            r.fail(t[1] + " blocks are not supported by this path");
+          }
+        } else {
+          r.fail(t[0] + " is not supported by this path");
+        }
+      }
+      info.conformal_area_defs[roi] = def;
     } else {
       r.fail(t[1] + " blocks are not supported by this path");
     }
@@ -496,7 +548,8 @@ void Schema::set_params(const ParameterList &params) {
 void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_size, int step_size,
                                                std::vector<double> &points_x, std::vector<double> &points_y,
                                                std::vector<int> &neighbor_ids,
-                                               const std::map<int, Conformal_Area_Def> *roi_defs) {
+                                               const std::map<int, Conformal_Area_Def> *roi_defs,
+                                               Subset_File_Info *seeds) {
   if (step_size <= 0) throw std::runtime_error("Error: step size is <= 0");
   if (subset_size <= 0) throw std::runtime_error("Error, the subset size was not specified");
   points_x.clear();
@@ -507,6 +560,9 @@ void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_
     whole[0].boundary.push_back(std::make_shared<Rectangle>(img_w / 2, img_h / 2, img_w, img_h));
     roi_defs = &whole;
   }
+  const bool seeded = seeds && !seeds->size_map.empty();
+  if (seeded && seeds->size_map.size() != seeds->displacement_map.size())
+    throw std::runtime_error("Error the number of displacement guesses and seed locations must be the same");
   for (const auto &roi : *roi_defs) {
     std::set<std::pair<int, int>> coords, excluded;
     for (const auto &shape : roi.second.boundary)
@@ -525,9 +581,16 @@ void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_
         if (excluded.count({cys[i], cxs[i]})) return false;
       return true;
     };
-    int num_rows = 0, num_cols = 0;
-    for (int x = subset_size - 1; x < img_w - subset_size; x += step_size) ++num_cols;
-    for (int y = subset_size - 1; y < img_h - subset_size; y += step_size) ++num_rows;
+    // the seed's grid row and column: the nearest grid point to its location
+    const bool has_seed = seeded && seeds->size_map.count(roi.first);
+    const int seed_x = has_seed ? seeds->size_map.at(roi.first).first : 0;
+    const int seed_y = has_seed ? seeds->size_map.at(roi.first).second : 0;
+    int num_rows = 0, num_cols = 0, seed_row = 0, seed_col = 0;
+    for (int x = subset_size - 1; x < img_w - subset_size; x += step_size, ++num_cols)
+      if (x + step_size / 2 < seed_x) ++seed_col;
+    for (int y = subset_size - 1; y < img_h - subset_size; y += step_size, ++num_rows)
+      if (y + step_size / 2 < seed_y) ++seed_row;
+    int seed_id = -1;
     auto add = [&](int col, int row, int neighbor) {
       const int x = subset_size - 1 + col * step_size, y = subset_size - 1 + row * step_size;
       if (!valid(x, y)) return false;
@@ -536,20 +599,29 @@ void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_
       neighbor_ids.push_back(neighbor);
       return true;
     };
-    // the first point (row 0, column 0), then the serpentine to the right
-    add(0, 0, -1);
-    const int start = (int)points_x.size();
-    int direction = 1, row = 0, col = 0;
-    while (row >= 0 && row < num_rows) {
-      if ((direction == 1 && row + 1 >= num_rows) || (direction == -1 && row - 1 < 0)) {
-        direction = -direction;
-        ++col;
-      } else {
-        row += direction;
+    if (add(seed_col, seed_row, -1)) {
+      if (has_seed) {
+        seed_id = (int)points_x.size() - 1;
+        seeds->seed_subset_ids[seed_id] = roi.first;
       }
-      if (col >= num_cols) break;
-      const int id = (int)points_x.size();
-      add(col, row, id == start ? -1 : id - 1);
+    } else if (!(seed_row == 0 && seed_col == 0)) {
+      throw std::runtime_error("Error, seed specified does not meet the SSSIG criteria for suffient image gradients");
+    }
+    // the serpentine to the right of the seed, then the one to the left
+    for (int side = 1; side >= -1; side -= 2) {
+      const int start = (int)points_x.size();
+      int direction = side, row = seed_row, col = seed_col;
+      while (row >= 0 && row < num_rows) {
+        if ((direction == 1 && row + 1 >= num_rows) || (direction == -1 && row - 1 < 0)) {
+          direction = -direction;
+          col += side;
+        } else {
+          row += direction;
+        }
+        if (col >= num_cols || col < 0) break;
+        const int id = (int)points_x.size();
+        add(col, row, id == start ? seed_id : id - 1);
+      }
     }
   }
 }
@@ -642,6 +714,34 @@ void Schema::set_def_image(int width, int height, const float *intensities) {
 void Schema::set_def_image(int width, int height, const uint16_t *intensities) {
   set_image(false, width, height, intensities, DICE_STORAGE_U16);
 }
+void Schema::set_neighbor_ids(const std::vector<int> &neighbor_ids) {
+  if ((int)neighbor_ids.size() != n_) throw std::invalid_argument("Error, one neighbor id per subset is needed");
+  for (int i = 0; i < n_; ++i) {
+    if (neighbor_ids[i] < -1 || neighbor_ids[i] >= n_) throw std::invalid_argument("Error, invalid neighbor id");
+    fields_[(size_t)DICE_F_NEIGHBOR_ID * n_ + i] = neighbor_ids[i];
+  }
+}
+
+void Schema::set_seed_values(const Subset_File_Info &info) {
+  for (const auto &seed : info.seed_subset_ids) {
+    const int id = seed.first, roi = seed.second;
+    if (id < 0 || id >= n_) throw std::runtime_error("Error, seed subset id out of range");
+    auto d = info.displacement_map.find(roi);
+    if (d == info.displacement_map.end()) throw std::runtime_error("Error, seed without a displacement guess");
+    local_field_value(id, SUBSET_DISPLACEMENT_X_FS) = d->second.first;
+    local_field_value(id, SUBSET_DISPLACEMENT_Y_FS) = d->second.second;
+    auto e = info.normal_strain_map.find(roi);
+    if (e != info.normal_strain_map.end()) {
+      local_field_value(id, NORMAL_STRETCH_XX_FS) = e->second.first;
+      local_field_value(id, NORMAL_STRETCH_YY_FS) = e->second.second;
+    }
+    auto g = info.shear_strain_map.find(roi);
+    if (g != info.shear_strain_map.end()) local_field_value(id, SHEAR_STRETCH_XY_FS) = g->second;
+    auto t = info.rotation_map.find(roi);
+    if (t != info.rotation_map.end()) local_field_value(id, ROTATION_Z_FS) = t->second;
+  }
+}
+
 void Schema::set_ref_image(const std::string &file_name) {
   std::vector<float> pix;
   int w = 0, h = 0;
@@ -723,7 +823,10 @@ void Output_Spec::write_info(std::FILE *file) const {
                s.gradient_method() == CONVOLUTION_5_POINT ? "CONVOLUTION_5_POINT" : "FINITE_DIFFERENCE");
   std::fprintf(file, "*** Optimization method: %s\n", opt_name(s.optimization_method()));
   std::fprintf(file, "*** Projection method: DISPLACEMENT_BASED\n");
-  std::fprintf(file, "*** Guess initialization method: USE_FIELD_VALUES\n");
+  static const char *const init_names[] = {"USE_FIELD_VALUES", "USE_NEIGHBOR_VALUES",
+                                           "USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY"};
+  const int im = s.initialization_method();
+  std::fprintf(file, "*** Guess initialization method: %s\n", im >= 0 && im <= 2 ? init_names[im] : "UNKNOWN");
   std::fprintf(file, "*** Seed location: N/A\n");
   std::fprintf(file, "*** Shape functions: ");
   if (s.translation_enabled()) std::fprintf(file, "Translation (u,v) ");

@@ -34,7 +34,11 @@ enum Optimization_Method {
   GRADIENT_BASED_THEN_SIMPLEX = 4
 };
 enum Correlation_Routine { GENERIC_ROUTINE = 0, TRACKING_ROUTINE = 1 };
-enum Initialization_Method { USE_FIELD_VALUES = 0 };
+enum Initialization_Method {
+  USE_FIELD_VALUES = 0,
+  USE_NEIGHBOR_VALUES = 1,
+  USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY = 2
+};
 enum Gradient_Method { FINITE_DIFFERENCE = 0, CONVOLUTION_5_POINT = 1 };
 // Where execute_correlation runs (parameter "correlation_backend"; no
 // counterpart in the reference, which has only its CPU loop).
@@ -153,29 +157,48 @@ struct Conformal_Area_Def {
 // The subset file of the reference (read_subset_file, DICe_Parser.cpp:1099-1700):
 // SUBSET_COORDINATES and CONFORMAL_SUBSET blocks (SUBSET_ID; BOUNDARY,
 // EXCLUDED and OBSTRUCTED blocks of POLYGON / CIRCLE / RECTANGLE shapes;
-// BLOCKING_SUBSETS; FORCE_SIMPLEX).  Keywords are case-insensitive, '#' starts
-// a comment.  The blocks this path has no use for (REGION_OF_INTEREST, seeds,
-// motion windows, paths, skip-solve lists) throw std::runtime_error naming
-// them.  width / height (or -1) bound the coordinates as in the reference.
+// BLOCKING_SUBSETS; FORCE_SIMPLEX), and REGION_OF_INTEREST blocks (BOUNDARY,
+// EXCLUDED, and a SEED with LOCATION, DISPLACEMENT and optional NORMAL_STRAIN,
+// SHEAR_STRAIN, ROTATION).  Keywords are case-insensitive, '#' starts a
+// comment.  The blocks this path has no use for (motion windows, paths,
+// skip-solve lists, the boundary conditions and IC values of global DIC) throw
+// std::runtime_error naming them.  width / height (or -1) bound the
+// coordinates as in the reference.
 struct Subset_File_Info {
+  // REGION_OF_INTEREST_INFO: conformal_area_defs are regions of interest (keyed
+  // by region number) for create_regular_grid_of_correlation_points, not subsets
+  bool region_of_interest = false;
   std::vector<double> coordinates_x, coordinates_y;
   std::map<int, Conformal_Area_Def> conformal_area_defs;
   std::map<int, std::vector<int>> blocking_subset_ids;
   std::vector<int> force_simplex;
+  // seeds of the regions of interest, keyed by region (the reference's size_map
+  // holds the seed locations)
+  std::map<int, std::pair<int, int>> size_map;
+  std::map<int, std::pair<double, double>> displacement_map, normal_strain_map;
+  std::map<int, double> shear_strain_map, rotation_map;
+  // subset id of each region's seed -> region, filled by create_regular_grid_of_correlation_points
+  std::map<int, int> seed_subset_ids;
 };
 Subset_File_Info read_subset_file(const std::string &file_name, int width = -1, int height = -1);
 
 // create_regular_grid_of_correlation_points (DICe_Decomp.cpp:868-1070) without
-// a seed or a gradient threshold: in each region of interest (the whole image
-// when roi_defs is null or empty), centres at subset_size - 1 + k * step_size
-// that lie at least a subset inside the image, inside the region, with no
-// subset corner in an excluded area, ordered as the reference's serpentine
-// (down the first column, up the next, ...).  neighbor_ids: -1 for the first
-// point of a region, else the previous point.
+// a gradient threshold: in each region of interest (the whole image when
+// roi_defs is null or empty), centres at subset_size - 1 + k * step_size that
+// lie at least a subset inside the image, inside the region, with no subset
+// corner in an excluded area.  Per region: the seed point first (the grid point
+// nearest the seed location, or row 0 / column 0 without a seed), then the
+// serpentine to the right of it (down the seed's column, up the next, ...),
+// then the one to the left (up the seed's column, down the previous, ...).
+// neighbor_ids: -1 for the seed point, the seed for the first point of each
+// serpentine (-1 without a seed), else the previous point.  seeds (optional):
+// the regions' seed locations (size_map); seed_subset_ids is filled.  Throws
+// std::runtime_error when a seed's grid point is not a valid correlation point.
 void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_size, int step_size,
                                                std::vector<double> &points_x, std::vector<double> &points_y,
                                                std::vector<int> &neighbor_ids,
-                                               const std::map<int, Conformal_Area_Def> *roi_defs = nullptr);
+                                               const std::map<int, Conformal_Area_Def> *roi_defs = nullptr,
+                                               Subset_File_Info *seeds = nullptr);
 
 // The pixel list of a conformal subset (Subset::Subset(cx, cy, Conformal_Area_Def),
 // DICe_Subset.cpp:115-168): the union of the boundary shapes' pixels in (y, x)
@@ -211,6 +234,12 @@ class Schema {
   void set_ref_image(int width, int height, const uint16_t *intensities);
   void set_def_image(int width, int height, const float *intensities);
   void set_def_image(int width, int height, const uint16_t *intensities);
+  // NEIGHBOR_ID_FS of every subset (Schema::initialize, DICe_Schema.cpp:1298-1303), the
+  // source of the initial guess under USE_NEIGHBOR_VALUES.
+  void set_neighbor_ids(const std::vector<int> &neighbor_ids);
+  // The seeds' initial values (Schema::initialize, DICe_Schema.cpp:1111-1143): the
+  // displacement, and the strains and rotation when given, of each seed subset.
+  void set_seed_values(const Subset_File_Info &info);
   // Schema::set_ref_image(const std::string &) / set_def_image(const std::string &)
   // (DICe_Schema.cpp:351-394, 179-304): the whole image read by read_image
   // (.rawi or TIFF), filtered and differentiated as the parameters say; the

@@ -46,7 +46,8 @@ typedef struct dice_params {
   int32_t optimization_method;           /* SIMPLEX=0, GRADIENT_BASED=1, SIMPLEX_THEN_GRADIENT_BASED=3,
                                             GRADIENT_BASED_THEN_SIMPLEX=4 (GRADIENT_THEN_SEARCH=2 rejected) */
   int32_t correlation_routine;           /* GENERIC_ROUTINE=0, TRACKING_ROUTINE=1 */
-  int32_t initialization_method;         /* USE_FIELD_VALUES=0 only */
+  int32_t initialization_method;         /* USE_FIELD_VALUES=0 USE_NEIGHBOR_VALUES=1
+                                            USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY=2 */
   int32_t enable_translation;            /* must be 1 */
   int32_t enable_rotation;
   int32_t enable_normal_strain;
@@ -70,6 +71,11 @@ typedef struct dice_params {
   int32_t gradient_method;               /* FINITE_DIFFERENCE=0, CONVOLUTION_5_POINT=1 */
   int32_t gauss_filter_images;           /* Gauss-filter both images on upload (DICe_Image.cpp:215-230) */
   int32_t gauss_filter_mask_size;        /* 5, 7, 9, 11 or 13 (default 7) */
+  double neighbor_settle_tolerance;      /* USE_NEIGHBOR_VALUES on this library: 0 (default) gives the
+                                            reference's subset-by-subset results; > 0 stops the rounds of
+                                            dice_hip_correlate once no initial guess moves by more than this
+                                            (px for u, v; rad / strain for the others).  No counterpart in
+                                            the reference. */
 } dice_params;
 
 /* Per-subset field store: SoA of doubles, fields[slot * n + i]
@@ -140,19 +146,31 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
 /* Schema::execute_correlation (DICe_Schema.cpp:1622-1792, GENERIC_ROUTINE with
  * generic_correlation_routine :2025-2368 per subset).  fields: host SoA,
  * DICE_F_COUNT * n doubles; read (displacements, rotation, strains, SIGMA as
- * the USE_FIELD_VALUES initial guess, DICe_Initializer.cpp:398-434) and
- * written (record_step / record_failed_step, :1958-2009).  first_frame = 1
- * enables the field-of-view test that the reference applies when
- * frame_id == first_frame_id.  Synchronous. */
+ * the initial guess of Field_Value_Initializer, DICe_Initializer.cpp:398-434;
+ * NEIGHBOR_ID with USE_NEIGHBOR_VALUES) and written (record_step /
+ * record_failed_step, :1958-2009).  first_frame = 1 enables the field-of-view
+ * test that the reference applies when frame_id == first_frame_id.
+ * USE_NEIGHBOR_VALUES (and USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY on a first
+ * frame) gives the results of the reference's subset-by-subset order, in which
+ * a subset starts from its neighbour's result of this frame when the neighbour
+ * has a lower id (else from the neighbour's frame-start values, or its own when
+ * NEIGHBOR_ID is -1); NEIGHBOR_ID must be -1 or a subset id.  Synchronous. */
 int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int first_frame);
 
 /* Same on a device-resident field store (DICE_F_COUNT * n doubles); enqueued on
- * the context stream, returns without waiting. */
+ * the context stream and returns without waiting, except with neighbour
+ * initialization, whose rounds wait on the stream between launches. */
 int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame);
+
+/* Correlation rounds the last neighbour-initialized frame took (0 after a
+ * USE_FIELD_VALUES frame).  No counterpart in the reference, which correlates
+ * the subsets in order, once each: a diagnostic of this library's rounds. */
+int dice_hip_neighbor_rounds(dice_hip_context *ctx);
 
 /* A multi-frame sequence: the CLI frame loop of execute_correlation calls
  * (DICe_Main.cpp:324-374) for n_frames deformed frames already in device
- * memory, each initialised from the previous frame's fields.  When
+ * memory, each initialised from the previous frame's fields (USE_FIELD_VALUES
+ * only).  When
  * use_graph != 0 the whole sequence is captured once into a hipGraph and
  * replayed.  device_history (optional, may be NULL): n_frames * DICE_F_COUNT * n
  * doubles receiving each frame's fields.  Asynchronous. */

@@ -758,10 +758,12 @@ void record_failed(double *F, int n, int gid, int status, int iters) {
   f(OF_STATUS) = status; f(OF_ITERATIONS) = iters;
 }
 
-// Schema::generic_correlation_routine (GENERIC_ROUTINE, USE_FIELD_VALUES), DICe_Schema.cpp:2025-2368.
+// Schema::generic_correlation_routine (GENERIC_ROUTINE), DICe_Schema.cpp:2025-2368.  sid: the subset
+// whose fields give the initial guess (Field_Value_Initializer::initial_guess,
+// DICe_Initializer.cpp:398-434): gid itself, or its neighbour with USE_NEIGHBOR_VALUES.
 template <typename S, typename T>
 void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<S, T> &defi,
-                   Subset<S> &sub, double *F, int n, int gid, int subset_size, bool first_frame) {
+                   Subset<S> &sub, double *F, int n, int gid, int subset_size, bool first_frame, int sid) {
   auto f = [&](int slot) -> double & { return F[(size_t)slot * n + gid]; };
   // Field-of-view check on the first frame for square subsets (:2033-2049).
   if (subset_size > 0 && first_frame) {
@@ -779,9 +781,11 @@ void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<
           (S)P->robust_delta_disp, (S)P->robust_delta_theta);
   int num_iterations = -1;
   // Field_Value_Initializer::initial_guess, DICe_Initializer.cpp:398-434.
-  if (f(OF_SIGMA) == -1.0) { record_failed(F, n, gid, INITIALIZE_FAILED, num_iterations); return; }
-  init_from_fields(sf, F, n, gid);
-  int init_status = INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
+  const int guess_status = sid == gid ? INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL
+                                      : INITIALIZE_USING_NEIGHBOR_VALUE_SUCCESSFUL;
+  if (F[(size_t)OF_SIGMA * n + sid] == -1.0) { record_failed(F, n, gid, INITIALIZE_FAILED, num_iterations); return; }
+  init_from_fields(sf, F, n, sid);
+  int init_status = guess_status;
   if (P->initial_gamma_threshold != -1.0) {
     const S g0 = obj.gamma(sf);
     if (g0 > P->initial_gamma_threshold || g0 < 0.0) { record_failed(F, n, gid, INITIALIZE_FAILED, num_iterations); return; }
@@ -800,14 +804,14 @@ void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<
     if (om == SIMPLEX || om == GRADIENT_BASED) {
       second_failed = true;
     } else if (om == GRADIENT_BASED_THEN_SIMPLEX || om == GRADIENT_THEN_SEARCH) {
-      init_from_fields(sf, F, n, gid);
-      init_status = INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
+      init_from_fields(sf, F, n, sid);
+      init_status = guess_status;
       if (om == GRADIENT_BASED_THEN_SIMPLEX) corr = obj.update_robust(sf, num_iterations);
       else corr = CORRELATION_FAILED;  // search initializer: out of scope
       if (corr != CORRELATION_SUCCESSFUL) second_failed = true;
     } else if (om == SIMPLEX_THEN_GRADIENT_BASED) {
-      init_from_fields(sf, F, n, gid);
-      init_status = INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
+      init_from_fields(sf, F, n, sid);
+      init_status = guess_status;
       corr = obj.update_fast(sf, num_iterations);
       if (corr != CORRELATION_SUCCESSFUL) second_failed = true;
     }
@@ -904,11 +908,7 @@ int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def
     defi.compute_gradients_fd();
     if (P->gradient_method == 1) defi.smooth_gradients();
   }
-#ifdef _OPENMP
-  if (threads < 1) threads = 1;
-#pragma omp parallel for schedule(static) num_threads(threads)
-#endif
-  for (int gid = 0; gid < n; ++gid) {
+  auto one = [&](int gid, int sid) {
     Subset<S> sub;
     const int cx = (int)F[(size_t)OF_COORD_X * n + gid];
     const int cy = (int)F[(size_t)OF_COORD_Y * n + gid];
@@ -919,8 +919,25 @@ int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def
       sub.make_square(cx, cy, subset_size);
     }
     sub.init_ref(refi);
-    correlate_one<S, T>(P, refi, defi, sub, F, n, gid, co ? 0 : subset_size, first_frame != 0);
+    correlate_one<S, T>(P, refi, defi, sub, F, n, gid, co ? 0 : subset_size, first_frame != 0, sid);
+  };
+  const bool neighbors = P->initialization_method == 1 || (P->initialization_method == 2 && first_frame);
+  if (neighbors) {
+    // USE_NEIGHBOR_VALUES: the subsets one after another in id order (execute_correlation,
+    // DICe_Schema.cpp:1720-1735), each starting from NEIGHBOR_ID's fields as they stand
+    // (this frame's result for a lower id), or its own when the id is -1.
+    for (int gid = 0; gid < n; ++gid) {
+      const double nb = F[(size_t)OF_NEIGHBOR_ID * n + gid];
+      if (nb != (double)(int)nb || nb < -1.0 || nb >= n) return -1;
+      one(gid, nb == -1.0 ? gid : (int)nb);
+    }
+    return 0;
   }
+#ifdef _OPENMP
+  if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(static) num_threads(threads)
+#endif
+  for (int gid = 0; gid < n; ++gid) one(gid, gid);
   (void)threads;
   return 0;
 }

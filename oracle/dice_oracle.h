@@ -72,6 +72,7 @@ typedef struct oracle_params {
   int32_t gradient_method;               /* FINITE_DIFFERENCE=0 CONVOLUTION_5_POINT=1 */
   int32_t gauss_filter_images;           /* Gauss-filter both images first (DICe_Image.cpp:215-230) */
   int32_t gauss_filter_mask_size;        /* 5, 7, 9, 11, 13 */
+  double neighbor_settle_tolerance;      /* not used: the oracle correlates neighbour chains in order */
 } oracle_params;
 
 /* Field slots of the per-subset SoA double store: fields[slot*n + i]. */

@@ -51,6 +51,7 @@ class Params(ctypes.Structure):
         ("gradient_method", ctypes.c_int32),
         ("gauss_filter_images", ctypes.c_int32),
         ("gauss_filter_mask_size", ctypes.c_int32),
+        ("neighbor_settle_tolerance", ctypes.c_double),
     ]
 
 

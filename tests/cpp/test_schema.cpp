@@ -8,7 +8,8 @@
 //   test_schema polygon            (prints the owned pixels "y x" of the golden's polygons)
 //   test_schema subset_file_info <subset file>   (what read_subset_file makes of it)
 //   test_schema deck <config>      (a regression deck end to end, see deck())
-//   test_schema grid <w> <h> <subset size> <step>   (prints "x y neighbor" of the regular grid)
+//   test_schema grid <w> <h> <subset size> <step> [subset file]   (prints "x y neighbor" of the
+//                                  regular grid, then "seed id u v" for each seed)
 //   test_schema errors             (error behaviour that needs no GPU)
 //   test_schema read_image <image file> <out.rawi> [ox oy w h]   (read_image, then write_rawi_image)
 //   test_schema tiff_samples <tiff file> <out>   (uint32 w, h, bits, then the uint16 samples)
@@ -199,8 +200,12 @@ int subset_file_info(const char *path) {
 // width, height, subset_size, subset_file or step_size, out_dir, prefix,
 // separate (0|1), backend (0|1), ref_name, def_name, routine (0|1), and
 // optional fields (comma-separated output names), optimization /
-// interpolation (enum values) and gauss_filter_mask_size (turns the Gauss
-// filter on).  Default parameters otherwise; one frame.
+// interpolation / initialization (enum values), gauss_filter_mask_size (turns
+// the Gauss filter on) and frames (the deformed image correlated that many
+// times, one output file per frame; def_names: the comma-separated names for
+// the header of each frame).  A subset file of regions of interest
+// gives the grid (step_size) inside them, with the seeds' values and the
+// neighbour ids.  Default parameters otherwise.
 int deck(const char *config_path) {
   std::ifstream cfg(config_path);
   std::map<std::string, std::string> c;
@@ -220,6 +225,7 @@ int deck(const char *config_path) {
   params.set("correlation_backend", num("backend"));
   if (c.count("optimization")) params.set("optimization_method", num("optimization"));
   if (c.count("interpolation")) params.set("interpolation_method", num("interpolation"));
+  if (c.count("initialization")) params.set("initialization_method", num("initialization"));
   if (c.count("gauss_filter_mask_size")) {
     params.set("gauss_filter_images", 1);
     params.set("gauss_filter_mask_size", num("gauss_filter_mask_size"));
@@ -228,31 +234,43 @@ int deck(const char *config_path) {
   std::vector<int> neighbors;
   Subset_File_Info info;
   const bool grid = c.count("step_size") != 0;
+  if (c.count("subset_file")) info = read_subset_file(c.at("subset_file"), w, h);
   if (grid) {
-    create_regular_grid_of_correlation_points(w, h, num("subset_size"), num("step_size"), cx, cy, neighbors);
+    create_regular_grid_of_correlation_points(w, h, num("subset_size"), num("step_size"), cx, cy, neighbors,
+                                              &info.conformal_area_defs, &info);
   } else {
-    info = read_subset_file(c.at("subset_file"), w, h);
     cx = info.coordinates_x;
     cy = info.coordinates_y;
   }
-  Schema schema(cx, cy, num("subset_size"), &info.conformal_area_defs, &params);
-  if (grid) schema.set_step_sizes(num("step_size"), num("step_size"));
-  if (files) {
-    schema.set_ref_image(c.at("ref_file"));
-    schema.set_def_image(c.at("def_file"));
-  } else {
-    schema.set_ref_image(w, h, ref.data());
-    schema.set_def_image(w, h, def.data());
+  Schema schema(cx, cy, num("subset_size"), grid ? nullptr : &info.conformal_area_defs, &params);
+  if (grid) {
+    schema.set_step_sizes(num("step_size"), num("step_size"));
+    schema.set_neighbor_ids(neighbors);
+    schema.set_seed_values(info);
   }
+  if (files) schema.set_ref_image(c.at("ref_file"));
+  else schema.set_ref_image(w, h, ref.data());
   schema.set_image_file_names(c.at("ref_name"), c.at("def_name"));
-  schema.execute_correlation();
   std::vector<std::string> fields;
   if (c.count("fields")) {
     std::stringstream ss(c.at("fields"));
     std::string f;
     while (std::getline(ss, f, ',')) fields.push_back(f);
   }
-  schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, " ", false, 1);
+  const int frames = c.count("frames") ? num("frames") : 1;
+  std::vector<std::string> def_names;
+  if (c.count("def_names")) {
+    std::stringstream ss(c.at("def_names"));
+    std::string f;
+    while (std::getline(ss, f, ',')) def_names.push_back(f);
+  }
+  for (int frame = 0; frame < frames; ++frame) {
+    if (files) schema.set_def_image(c.at("def_file"));
+    else schema.set_def_image(w, h, def.data());
+    schema.set_image_file_names(c.at("ref_name"), frame < (int)def_names.size() ? def_names[frame] : c.at("def_name"));
+    schema.execute_correlation();
+    schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, " ", false, frames);
+  }
   for (int i = 0; i < schema.local_num_subsets(); ++i)
     std::cout << "subset " << i << " x " << cx[i] << " y " << cy[i] << " u "
               << schema.local_field_value(i, SUBSET_DISPLACEMENT_X_FS) << " v "
@@ -327,9 +345,15 @@ int main(int argc, char **argv) {
     if (cmd == "grid" && argc >= 6) {
       std::vector<double> x, y;
       std::vector<int> nb;
+      Subset_File_Info info;
+      if (argc >= 7) info = read_subset_file(argv[6], std::atoi(argv[2]), std::atoi(argv[3]));
       create_regular_grid_of_correlation_points(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
-                                                std::atoi(argv[5]), x, y, nb);
+                                                std::atoi(argv[5]), x, y, nb, &info.conformal_area_defs, &info);
       for (size_t i = 0; i < x.size(); ++i) std::cout << x[i] << " " << y[i] << " " << nb[i] << "\n";
+      for (const auto &sd : info.seed_subset_ids) {
+        const auto &d = info.displacement_map.at(sd.second);
+        std::cout << "seed " << sd.first << " " << d.first << " " << d.second << "\n";
+      }
       return 0;
     }
     if (argc < 7) return 99;

@@ -4,7 +4,8 @@ parameters this path needs, read from each deck's input.xml / params.xml into de
 
 Run from the repository root: python tests/golden/make_regression_decks.py
 (the images of these decks, ../../component/images/{ref,def}Speckled.tif, are the speckle
-pair already in speckle_512.npz)."""
+pair already in speckle_512.npz; defSpeckledCopy.tif, the second frame of
+parallel_seed_first_step_only, is byte for byte defSpeckled.tif)."""
 import json
 import os
 import shutil
@@ -12,11 +13,13 @@ import xml.etree.ElementTree as ET
 
 SRC = "/root/reference/tests/regression"
 DST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regression_decks")
-DECKS = ["conformal_subset_def", "single_speckled_shift", "parallel_subsets"]
+DECKS = ["conformal_subset_def", "single_speckled_shift", "parallel_subsets", "parallel_subsets_seed",
+         "parallel_seed_first_step_only"]
 ENUMS = {"interpolation_method": ["BILINEAR", "BICUBIC", "KEYS_FOURTH"],
          "optimization_method": ["SIMPLEX", "GRADIENT_BASED", "GRADIENT_THEN_SEARCH", "SIMPLEX_THEN_GRADIENT_BASED",
                                  "GRADIENT_BASED_THEN_SIMPLEX"],
-         "correlation_routine": ["GENERIC_ROUTINE", "TRACKING_ROUTINE"]}
+         "correlation_routine": ["GENERIC_ROUTINE", "TRACKING_ROUTINE"],
+         "initialization_method": ["USE_FIELD_VALUES", "USE_NEIGHBOR_VALUES", "USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY"]}
 
 
 def top_params(root):
@@ -37,10 +40,12 @@ def main():
                 "reference_image": inp["image_folder"] + inp["reference_image"]}
         defs = inp_root.find("ParameterList[@name='deformed_images']")
         deck["deformed_image"] = inp["image_folder"] + defs.find("Parameter").get("name")
+        deck["frames"] = len(defs.findall("Parameter"))
+        deck["deformed_images"] = [inp["image_folder"] + q.get("name") for q in defs.findall("Parameter")]
         if "subset_file" in inp:
             shutil.copy(os.path.join(src, inp["subset_file"]), os.path.join(dst, "subsets.txt"))
             deck["subset_file"] = "subsets.txt"
-        else:
+        if "step_size" in inp:
             deck["step_size"] = int(inp["step_size"])
         for key, names in ENUMS.items():
             if key in cor:

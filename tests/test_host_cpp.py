@@ -90,6 +90,87 @@ def test_conformal_subset_generator():
         assert abs(px[off[i]:off[i + 1]].mean() - cx[i]) <= 0.5 and abs(py[off[i]:off[i + 1]].mean() - cy[i]) <= 0.5
 
 
+def seeded_grid(name):
+    """test_schema grid over the deck's regions of interest: points, neighbour ids and seeds."""
+    ddir = os.path.join(DECKS, name)
+    r = run("grid", 512, 512, 27, 10, os.path.join(ddir, "subsets.txt"))
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in r.stdout.splitlines()]
+    pts = np.array([[float(t) for t in ln] for ln in lines if ln[0] != "seed"])
+    seeds = {int(ln[1]): (float(ln[2]), float(ln[3])) for ln in lines if ln[0] == "seed"}
+    return pts, seeds
+
+
+def test_seeded_grid_matches_regression_gold():
+    """Regions of interest with seeds (tests/regression/parallel_subsets_seed): four 50 x 50 rectangles,
+    seeds nearest (25, 25), (65, 25), (25, 65), (65, 65) with displacement (0.4, 0.4).  The grid has the
+    gold's 64 centres in the gold's order: per region the seed, the serpentine to its right, then the one
+    to its left; the seed's neighbour is -1, each serpentine starts from the seed."""
+    pts, seeds = seeded_grid("parallel_subsets_seed")
+    rows = [ln.split() for ln in open(os.path.join(DECKS, "parallel_subsets_seed", "gold", "DICe_solution_0.txt"))
+            if ln[:1].isdigit()]
+    assert np.array_equal(pts[:, :2], np.array([[float(t[1]), float(t[2])] for t in rows]))
+    assert sorted(seeds) == [0, 9, 24, 39] and all(v == (0.4, 0.4) for v in seeds.values())
+    nb = pts[:, 2].astype(int)
+    assert all(nb[s] == -1 for s in seeds)
+    # region 2: seed (66, 26), right serpentine from it (seed neighbour), then left serpentine from the seed
+    assert list(pts[9:13, :2].ravel()) == [66, 26, 66, 36, 66, 46, 76, 46]
+    assert nb[10] == 9 and nb[11] == 10
+    left = [i for i in range(10, 24) if pts[i, 0] < 66]
+    assert left and nb[left[0]] == 9 and list(pts[left[0], :2]) == [56, 26]
+
+
+@pytest.mark.parametrize("name,frames", [("parallel_subsets_seed", 1), ("parallel_seed_first_step_only", 2)])
+def test_oracle_neighbor_initialization_matches_seed_golds(name, frames):
+    """USE_NEIGHBOR_VALUES (and _FIRST_STEP_ONLY) in the oracle: the subsets one after another, each from
+    its neighbour's result, on the seeded grid -- the golds of both decks (double build) to the printed
+    digits: u, v within 1e-5, sigma within 2e-6; neighbour-initialized subsets record status 3, the seeds
+    1, and the second frame of the first-step-only deck (field values) 1 everywhere."""
+    pts, seeds = seeded_grid(name)
+    d = np.load(os.path.join(GOLD, "speckle_512.npz"))
+    p = o.default_params()
+    p.initialization_method = 1 if frames == 1 else 2
+    f = o.new_fields(pts[:, 0], pts[:, 1])
+    f[o.F["NEIGHBOR_ID"]] = pts[:, 2]
+    for sid, (u, v) in seeds.items():
+        f[o.F["DISP_X"], sid], f[o.F["DISP_Y"], sid] = u, v
+    for frame in range(frames):
+        o.correlate(p, d["ref"], d["defm"], f, subset_size=27, first_frame=frame == 0, precision="double")
+        g = np.array([[float(t) for t in ln.split()] for ln in
+                      open(os.path.join(DECKS, name, "gold", "DICe_solution_%d.txt" % frame)) if ln[:1].isdigit()])
+        assert np.abs(f[o.F["DISP_X"]] - g[:, 3]).max() <= 1e-5
+        assert np.abs(f[o.F["DISP_Y"]] - g[:, 4]).max() <= 1e-5
+        assert np.abs(f[o.F["SIGMA"]] - g[:, 5]).max() <= 2e-6
+        want = np.full(len(pts), 1.0 if frame else 3.0)
+        if frame == 0:
+            want[list(seeds)] = 1.0
+        assert np.array_equal(f[o.F["STATUS"]], want)
+    # starting every subset from its own (zero) fields instead does not reproduce the chain
+    q = o.default_params()
+    f2 = o.new_fields(pts[:, 0], pts[:, 1])
+    o.correlate(q, d["ref"], d["defm"], f2, subset_size=27, precision="double")
+    assert not np.array_equal(f2[o.F["DISP_X"]], f[o.F["DISP_X"]])
+
+
+def test_serpentine_grid_helper_matches_cpp():
+    import dice_amd as dh
+    for w, h, s, step in ((512, 512, 27, 75), (300, 211, 21, 17), (128, 640, 31, 9)):
+        r = run("grid", w, h, s, step)
+        pts = np.array(r.stdout.split(), dtype=float).reshape(-1, 3)
+        cx, cy, nb = dh.serpentine_grid(w, h, s, step)
+        assert np.array_equal(pts[:, 0], cx) and np.array_equal(pts[:, 1], cy)
+        assert np.array_equal(pts[:, 2], nb)
+
+
+def test_subset_file_seed_errors(tmp_path):
+    """A seed needs a location and a displacement guess (DICe_Parser.cpp:1350-1351)."""
+    bad = tmp_path / "bad.txt"
+    bad.write_text("BEGIN REGION_OF_INTEREST\n BEGIN BOUNDARY\n BEGIN RECTANGLE\n UPPER_LEFT 0 0\n"
+                   " LOWER_RIGHT 50 50\n END\n END\n BEGIN SEED\n LOCATION 25 25\n END\nEND\n")
+    r = run("grid", 512, 512, 27, 10, bad)
+    assert r.returncode == 98 and "displacement" in r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
 def test_cpp_correlate_params(tmp_path, backend):
@@ -201,7 +282,8 @@ def test_regression_subset_file_reader():
 
 
 def test_subset_file_errors(tmp_path):
-    for body, what in [("BEGIN REGION_OF_INTEREST\nEND REGION_OF_INTEREST\n", "REGION_OF_INTEREST"),
+    for body, what in [("BEGIN MOTION_WINDOW\nEND MOTION_WINDOW\n", "MOTION_WINDOW"),
+                       ("BEGIN REGION_OF_INTEREST\nDIRICHLET_BC BOUNDARY 0 0 1 0 0.0\nEND\n", "DIRICHLET_BC"),
                        ("BEGIN CONFORMAL_SUBSET\nBEGIN BOUNDARY\nBEGIN CIRCLE\nCENTER 10 10\nRADIUS 3\nEND CIRCLE\n"
                         "END BOUNDARY\nEND CONFORMAL_SUBSET\n", "SUBSET_ID"),
                        ("BEGIN SUBSET_COORDINATES\n10\nEND SUBSET_COORDINATES\n", "not enough values")]:
@@ -281,7 +363,8 @@ def test_regular_grid_order_matches_regression_gold():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
-@pytest.mark.parametrize("name", ["conformal_subset_def", "single_speckled_shift", "parallel_subsets"])
+@pytest.mark.parametrize("name", ["conformal_subset_def", "single_speckled_shift", "parallel_subsets",
+                                  "parallel_subsets_seed", "parallel_seed_first_step_only"])
 def test_regression_deck(tmp_path, name, backend):
     """A deck of tests/regression end to end through the C++ Schema mirror -- subsets from its subset file
     (read_subset_file) or its regular grid (create_regular_grid_of_correlation_points), the correlation,
@@ -299,13 +382,15 @@ def test_regression_deck(tmp_path, name, backend):
            "def_name": deck["deformed_image"], "routine": deck.get("correlation_routine", 0)}
     if "subset_file" in deck:
         cfg["subset_file"] = os.path.join(ddir, deck["subset_file"])
-    else:
+    if "step_size" in deck:
         cfg["step_size"] = deck["step_size"]
     if "fields" in deck:
         cfg["fields"] = ",".join(deck["fields"])
-    for k, key in (("optimization_method", "optimization"), ("interpolation_method", "interpolation")):
+    for k, key in (("optimization_method", "optimization"), ("interpolation_method", "interpolation"),
+                   ("initialization_method", "initialization"), ("frames", "frames")):
         if k in deck:
             cfg[key] = deck[k]
+    cfg["def_names"] = ",".join(deck["deformed_images"])
     cfg_path = tmp_path / "deck.cfg"
     cfg_path.write_text("".join("%s %s\n" % kv for kv in cfg.items()))
     r = run("deck", cfg_path)
