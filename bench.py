@@ -29,6 +29,12 @@ under translation + shear up to 2 px, 9 x 9 subsets at step 4 (760,920
 subsets), Nelder-Mead over the 6 affine parameters from a zero initial guess,
 200-iteration cap, bilinear interpolation.  One step = one frame pair.
 
+--workload neighbors (USE_NEIGHBOR_VALUES): the c2 frame with A = 1.0 px, the
+same 41 x 41 subsets at step 5 laid out as the reference's serpentine (one chain
+of 155,236 subsets, each starting from its neighbour's result), 6-parameter
+Gauss-Newton, rounds until no guess moves by more than --settle (1e-6 px).
+Also the exact order (settle 0) on a 512^2 crop.  One step = one frame.
+
 --workload gauss (image ingest): random --size^2 float32 frames (2048 by
 default) resident in HBM, each set as the deformed image with
 gauss_filter_images on (--mask, 7 by default), no gradients.  One step = one
@@ -704,12 +710,115 @@ def bench_gauss(args):
     return result
 
 
+def bench_neighbors(args):
+    """USE_NEIGHBOR_VALUES over the serpentine grid of the headline frame: one chain of 155,236 subsets."""
+    import torch
+    import dice_amd as dh
+    from dice_amd import synth
+
+    size, S, step = args.size, args.subset, args.step
+    t0 = time.time()
+    ref, frames = synth.sinusoid_sequence(size=size, frames=10, seed=1)
+    de = frames[-1]  # A = 1.0 px
+    log("neighbors inputs generated in %.1f s" % (time.time() - t0))
+    cx, cy, nb = dh.serpentine_grid(size, size, S, step)
+    n = len(cx)
+    init = dh.new_fields(cx, cy)
+    init[dh.F["NEIGHBOR_ID"]] = nb
+    p = dh.default_params()
+    p.optimization_method = dh.GRADIENT_BASED
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    p.initialization_method = dh.USE_NEIGHBOR_VALUES
+    p.neighbor_settle_tolerance = args.settle
+    dev = torch.device("cuda:0")
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(ref)
+    ctx.set_subsets(cx, cy, S)
+    d_def = torch.from_numpy(de).to(dev)
+    d_init = torch.from_numpy(init).to(dev)
+    d_f = d_init.clone()
+    torch.cuda.synchronize()
+    ctx.set_def_image_device(d_def.data_ptr(), size, size, dh.STORAGE_F32)
+    stream = torch.cuda.ExternalStream(ctx.stream())
+
+    def frame():
+        with torch.cuda.stream(stream):
+            d_f.copy_(d_init)
+        ctx.correlate_device(d_f.data_ptr(), mode=dh.MODE_BATCHED, first_frame=True)
+        ctx.synchronize()
+
+    for _ in range(args.warmup):
+        frame()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    el = time.perf_counter() - t
+    rounds = ctx.neighbor_rounds()
+    f = d_f.cpu().numpy()
+    st = f[dh.F["STATUS"]]
+    log("neighbors: %d subsets, %d rounds, %.3f s per frame" % (n, rounds, el / args.steps))
+    # the reference's own order (settle 0) on a 512^2 crop of the frame, a chain of its own
+    crop = 512
+    ccx, ccy, cnb = dh.serpentine_grid(crop, crop, S, step)
+    cinit = dh.new_fields(ccx, ccy)
+    cinit[dh.F["NEIGHBOR_ID"]] = cnb
+    q = dh.default_params()
+    q.optimization_method = dh.GRADIENT_BASED
+    q.enable_rotation = q.enable_normal_strain = q.enable_shear_strain = 1
+    q.initialization_method = dh.USE_NEIGHBOR_VALUES
+    ctx.set_params(q)
+    ctx.set_ref_image(np.ascontiguousarray(ref[:crop, :crop]))
+    ctx.set_def_image(np.ascontiguousarray(de[:crop, :crop]))
+    ctx.set_subsets(ccx, ccy, S)
+    fc = cinit.copy()
+    t = time.perf_counter()
+    ctx.correlate(fc, mode=dh.MODE_BATCHED, first_frame=True)
+    exact_s = time.perf_counter() - t
+    exact_rounds = ctx.neighbor_rounds()
+    log("exact order on a %d^2 crop: %d subsets, %d rounds, %.3f s" % (crop, len(ccx), exact_rounds, exact_s))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as o
+        # the chain from its start, in the reference's order (one thread: each subset waits for the last)
+        k = 256
+        while True:
+            fo = init[:, :k].copy()
+            t0 = time.perf_counter()
+            op = o.default_params()
+            for name, _ in dh.Params._fields_:
+                setattr(op, name, getattr(p, name))
+            o.correlate(op, ref, de, fo, subset_size=S, precision="float", threads=1)
+            cs = time.perf_counter() - t0
+            if cs > min(args.cpu_seconds, 10.0) / 4 or k * 2 > n:
+                break
+            k *= 2
+        cpu = {"value": round(k / cs, 1), "unit": "subsets/s", "cores": 1, "kind": "port",
+               "sample": "the oracle (float build) on the first %d subsets of the chain, in order, one thread" % k}
+    result = result_line(
+        args, "subsets/sec with USE_NEIGHBOR_VALUES (settle %.0e px), one serpentine chain of %d %dx%d subsets "
+              "on %d^2" % (args.settle, n, S, S, size),
+        n * args.steps / el, "subsets/s", 1000.0 * el / args.steps, "f32 (FP64 accumulation)",
+        {"workload": "speckle %dx%d f32, sinusoidal field A = 1.0 px, %dx%d subsets step %d (%d), serpentine "
+                     "order, every subset from its neighbour's result (zero fields elsewhere), 6-parameter "
+                     "Gauss-Newton" % (size, size, S, S, step, n),
+         "rounds": rounds, "neighbor_settle_tolerance": args.settle,
+         "converged_frac": round(float(np.isin(st, dh.CONVERGED_STATUSES).mean()), 4),
+         "exact_order_crop": {"size": crop, "subsets": len(ccx), "rounds": exact_rounds,
+                              "seconds": round(exact_s, 3), "subsets_per_s": round(len(ccx) / exact_s, 1)}},
+        None, cpu_baseline=cpu)
+    ctx.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "gauss"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "gauss", "neighbors"], default="c2")
+    ap.add_argument("--settle", type=float, default=1e-6, help="neighbors: neighbor_settle_tolerance")
     ap.add_argument("--mask", type=int, default=7, help="gauss: mask size (5, 7, 9, 11, 13)")
     ap.add_argument("--size", type=int, default=2048, help="c2: image side")
     ap.add_argument("--frames", type=int, default=0, help="c2: 10, c3: 100 when 0")
@@ -740,7 +849,8 @@ def main():
         print("bench.py runs on one GPU only: the correlation of one device's subset share is the "
               "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
-    run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "gauss": bench_gauss}[args.workload]
+    run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "gauss": bench_gauss,
+           "neighbors": bench_neighbors}[args.workload]
     print(json.dumps(run(args)), flush=True)
     return 0
 
