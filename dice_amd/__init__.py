@@ -24,6 +24,8 @@ FIELDS = ["COORD_X", "COORD_Y", "DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "
           "STATUS", "ITERATIONS", "CONDITION", "NEIGHBOR_ID",
           "FIELD_1", "FIELD_2", "FIELD_3", "FIELD_4", "FIELD_5", "FIELD_6", "FIELD_7"]
 F = {name: i for i, name in enumerate(FIELDS)}
+# enum dice_vsg_field
+VSG_FIELDS = ["VSG_STRAIN_XX", "VSG_STRAIN_YY", "VSG_STRAIN_XY", "VSG_DUDX", "VSG_DUDY", "VSG_DVDX", "VSG_DVDY"]
 NUM_FIELDS = len(FIELDS)
 
 MODE_PLAIN = 0
@@ -85,7 +87,7 @@ EXPORTS = [
     "dice_hip_synchronize", "dice_hip_stream", "dice_hip_kernel_timing", "dice_hip_set_kernel_timing",
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
-    "dice_hip_neighbor_rounds",
+    "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
 ]
 
 _lib = None
@@ -144,6 +146,8 @@ def lib():
         L.dice_hip_gamma_adjoint.argtypes = [P, I, P, P, P, P]
         L.dice_hip_gauss_filter.argtypes = [I, I, I, I, P, P, I]
         L.dice_hip_neighbor_rounds.argtypes = [P]
+        L.dice_hip_vsg_strain.argtypes = [P, I, P, P]
+        L.dice_hip_vsg_strain_device.argtypes = [P, I, P, P]
         _lib = L
     return _lib
 
@@ -286,6 +290,14 @@ class Context:
                                                           ctypes.c_void_p(fields_ptr),
                                                           ctypes.c_void_p(history_ptr) if history_ptr else None,
                                                           1 if use_graph else 0), "correlate_sequence")
+
+    def vsg_strain(self, fields, window_size):
+        """dice_hip_vsg_strain: VSG strains of the current subsets from fields (MATCH set to -1 where the
+        strain fails, in place); returns a (7, n) array in VSG_FIELDS order."""
+        assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape == (NUM_FIELDS, self.n)
+        out = np.zeros((len(VSG_FIELDS), self.n))
+        self._check(self._lib.dice_hip_vsg_strain(self.h, window_size, _p(fields), _p(out)), "vsg_strain")
+        return out
 
     def neighbor_rounds(self):
         """Correlation rounds of the last frame with neighbour initialization (0 otherwise)."""

@@ -83,6 +83,7 @@ struct dice_hip_context {
   DevBuf nb_start, nb_guess[2], nb_list, nb_counts;  // USE_NEIGHBOR_VALUES: frame-start fields, guesses,
                                                      // subsets to correlate again, their count per round
   int nb_rounds = 0;                         // correlation rounds of the last neighbour-initialized frame
+  DevBuf vsg_cells, vsg_ids, vsg_out;        // VSG strain: subset grid, host-call output
   bool def_has_grads = false;
   // subsets
   int n = 0, S = 0, fov = 0;
@@ -338,7 +339,8 @@ int dice_hip_destroy(dice_hip_context *ctx) {
   for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
                     &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
-                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->fields, &ctx->work})
+                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids,
+                    &ctx->vsg_out, &ctx->fields, &ctx->work})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -650,6 +652,84 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
 }
 
 int dice_hip_neighbor_rounds(dice_hip_context *ctx) { return ctx ? ctx->nb_rounds : DICE_HIP_EINVAL; }
+
+// VSG strains: the subsets binned on the host into a uniform grid of cell side
+// max(r, 1) px (sorted by cell, then id), uploaded with the offsets, then one
+// thread per subset on the context stream.
+static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields, double *d_vsg) {
+  if (window_size <= 0)
+    return ctx->fail(DICE_HIP_EINVAL, window_size == 0 ? "strain_window_size_in_pixels must not be 0"
+                                                       : "k-nearest VSG windows (negative sizes) are not supported");
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
+  const int n = ctx->n;
+  std::vector<double> xy(2 * (size_t)n);
+  HIP_TRY(ctx, hipMemcpyAsync(xy.data(), d_fields + (size_t)DICE_F_COORD_X * n, 2 * (size_t)n * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy coordinates");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "coordinates");
+  const double *X = xy.data(), *Y = xy.data() + n;
+  double x0 = X[0], y0 = Y[0], x1 = X[0], y1 = Y[0];
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(X[i]) || !std::isfinite(Y[i])) return ctx->fail(DICE_HIP_EINVAL, "non-finite coordinates");
+    x0 = std::min(x0, X[i]); x1 = std::max(x1, X[i]);
+    y0 = std::min(y0, Y[i]); y1 = std::max(y1, Y[i]);
+  }
+  const float radius = (float)window_size / 2.0f;
+  dice_hip::VsgGrid g{};
+  g.x0 = x0;
+  g.y0 = y0;
+  // cells of side >= r, so that every neighbour lies in the 3 x 3 cells around a subset; at most
+  // 4000 cells along a side
+  g.cell = std::max({(double)radius, 1.0, (x1 - x0) / 4000.0, (y1 - y0) / 4000.0});
+  g.ncx = (int)std::floor((x1 - x0) / g.cell) + 1;
+  g.ncy = (int)std::floor((y1 - y0) / g.cell) + 1;
+  const size_t ncells = (size_t)g.ncx * g.ncy;
+  std::vector<int> start(ncells + 1, 0), ids(n), cell(n);
+  for (int i = 0; i < n; ++i) {
+    const int cx = std::min(std::max((int)std::floor((X[i] - x0) / g.cell), 0), g.ncx - 1);
+    const int cy = std::min(std::max((int)std::floor((Y[i] - y0) / g.cell), 0), g.ncy - 1);
+    cell[i] = cy * g.ncx + cx;
+    ++start[cell[i] + 1];
+  }
+  for (size_t c = 0; c < ncells; ++c) start[c + 1] += start[c];
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (int i = 0; i < n; ++i) ids[fill[cell[i]]++] = i;
+  HIP_TRY(ctx, ctx->vsg_cells.ensure((ncells + 1) * sizeof(int)), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_ids.ensure((size_t)n * sizeof(int)), "hipMalloc VSG grid");
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->vsg_cells.p, start.data(), (ncells + 1) * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream), "hipMemcpy VSG grid");
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->vsg_ids.p, ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream),
+          "hipMemcpy VSG grid");
+  g.cell_start = ctx->vsg_cells.as<int>();
+  g.cell_ids = ctx->vsg_ids.as<int>();
+  HIP_TRY(ctx, dice_hip::launch_vsg_strain(g, radius, d_fields, d_vsg, n, ctx->stream), "VSG strain kernel");
+  // the host vectors must outlive the copies
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG strain");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *device_fields, double *device_vsg) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!device_fields || !device_vsg) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  (void)hipSetDevice(ctx->device);
+  return vsg_enqueue(ctx, window_size, device_fields, device_vsg);
+}
+
+int dice_hip_vsg_strain(dice_hip_context *ctx, int window_size, double *fields, double *vsg) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!fields || !vsg) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
+  (void)hipSetDevice(ctx->device);
+  const size_t bytes = (size_t)DICE_F_COUNT * ctx->n * sizeof(double);
+  const size_t vbytes = (size_t)dice_hip::kVsgFields * ctx->n * sizeof(double);
+  HIP_TRY(ctx, ctx->vsg_out.ensure(vbytes), "hipMalloc VSG fields");
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->fields.p, fields, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy fields");
+  int rc = vsg_enqueue(ctx, window_size, ctx->fields.as<double>(), ctx->vsg_out.as<double>());
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(fields, ctx->fields.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy fields");
+  HIP_TRY(ctx, hipMemcpyAsync(vsg, ctx->vsg_out.p, vbytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy VSG");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG strain");
+  return DICE_HIP_OK;
+}
 
 static int enqueue_gamma(dice_hip_context *ctx, int mode, const dice_hip::GammaArgs &ga) {
   if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");

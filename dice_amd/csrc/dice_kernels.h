@@ -163,6 +163,23 @@ hipError_t launch_neighbor_guess(const double *start, const double *cur, double 
 hipError_t launch_reset_listed(const int *list, const int *count, const double *start, double *fields, int n,
                                hipStream_t stream);
 
+// VSG_Strain_Post_Processor::execute (DICe_PostProcessor.cpp:931-1136) for
+// every subset: the neighbours within the window radius r (squared distance
+// below r^2 + 1e-5, formed in float as the reference's float kd-tree does),
+// found through a uniform grid of cell side cell >= r (cell_start: ncx * ncy + 1
+// offsets into cell_ids, subsets sorted by cell), a least-squares plane of u and
+// v over the valid ones (sigma >= 0), its gradient and the Green-Lagrange
+// strains.  vsg: 7 x n doubles (strain xx, yy, xy, du/dx, du/dy, dv/dx, dv/dy);
+// subsets whose own sigma is < 0, that have fewer than 3 valid neighbours, or
+// whose normal matrix has rcond < 1e-12 get zeros and MATCH = -1 in fields.
+struct VsgGrid {
+  double x0, y0, cell;
+  int ncx, ncy;
+  const int *cell_start, *cell_ids;
+};
+hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream);
+constexpr int kVsgFields = 7;
+
 // Enqueue the correlation of all subsets, or of the device list items (count
 // *item_count) when items is not null.
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,

@@ -192,7 +192,129 @@ __global__ void reset_listed_kernel(const int *list, const int *count, const dou
   for (int s = 0; s < kNumFields; ++s) fields[(size_t)s * n + i] = start[(size_t)s * n + i];
 }
 
+// 3 x 3 LU with partial pivoting and the inverse from its factors, in double
+// (GETRF + GETRI).  Returns false if a pivot is zero.
+__device__ bool inverse3(const double (&a_in)[3][3], double (&inv)[3][3]) {
+  double a[3][3];
+  int piv[3] = {0, 1, 2};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = a_in[i][j];
+  for (int k = 0; k < 3; ++k) {
+    int p = k;
+    for (int i = k + 1; i < 3; ++i)
+      if (fabs(a[i][k]) > fabs(a[p][k])) p = i;
+    if (a[p][k] == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < 3; ++j) {
+        const double t = a[k][j];
+        a[k][j] = a[p][j];
+        a[p][j] = t;
+      }
+      const int t = piv[k];
+      piv[k] = piv[p];
+      piv[p] = t;
+    }
+    for (int i = k + 1; i < 3; ++i) {
+      a[i][k] /= a[k][k];
+      for (int j = k + 1; j < 3; ++j) a[i][j] -= a[i][k] * a[k][j];
+    }
+  }
+  // solve L U x = P e_c for each column c
+  for (int c = 0; c < 3; ++c) {
+    double x[3];
+    for (int i = 0; i < 3; ++i) {
+      double v = piv[i] == c ? 1.0 : 0.0;
+      for (int j = 0; j < i; ++j) v -= a[i][j] * x[j];
+      x[i] = v;
+    }
+    for (int i = 2; i >= 0; --i) {
+      double v = x[i];
+      for (int j = i + 1; j < 3; ++j) v -= a[i][j] * x[j];
+      x[i] = v / a[i][i];
+    }
+    for (int i = 0; i < 3; ++i) inv[i][c] = x[i];
+  }
+  return true;
+}
+
+__device__ double norm1_3(const double (&a)[3][3]) {
+  double m = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    const double c = fabs(a[0][j]) + fabs(a[1][j]) + fabs(a[2][j]);
+    m = c > m ? c : m;
+  }
+  return m;
+}
+
+// One subset per thread (see launch_vsg_strain).
+__global__ void vsg_strain_kernel(VsgGrid g, float radius, double *fields, double *vsg, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  auto F = [&](int s, int k) -> double & { return fields[(size_t)s * n + k]; };
+  const double xi = F(F_COORD_X, i), yi = F(F_COORD_Y, i);
+  const float xf = (float)xi, yf = (float)yi;
+  const float r2 = radius * radius + 1.0e-5f;  // the reference's rad^2 + tiny in scalar_t
+  double s1 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, su = 0, sxu = 0, syu = 0, sv = 0, sxv = 0, syv = 0;
+  int valid = 0;
+  if (F(F_SIGMA, i) >= 0.0) {
+    const int ci = min(max((int)floor((xi - g.x0) / g.cell), 0), g.ncx - 1);
+    const int cj = min(max((int)floor((yi - g.y0) / g.cell), 0), g.ncy - 1);
+    for (int cy = max(cj - 1, 0); cy <= min(cj + 1, g.ncy - 1); ++cy)
+      for (int cx = max(ci - 1, 0); cx <= min(ci + 1, g.ncx - 1); ++cx) {
+        const int c = cy * g.ncx + cx;
+        for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
+          const int j = g.cell_ids[k];
+          const float ddx = (float)F(F_COORD_X, j) - xf, ddy = (float)F(F_COORD_Y, j) - yf;
+          if (!(ddx * ddx + ddy * ddy < r2)) continue;
+          if (!(F(F_SIGMA, j) >= 0.0)) continue;
+          // the reference keeps the neighbour offsets and the displacements as scalar_t (float)
+          const double dx = ddx, dy = ddy;
+          const double u = (float)F(F_DISP_X, j), v = (float)F(F_DISP_Y, j);
+          ++valid;
+          s1 += 1.0; sx += dx; sy += dy;
+          sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
+          su += u; sxu += dx * u; syu += dy * u;
+          sv += v; sxv += dx * v; syv += dy * v;
+        }
+      }
+  }
+  double out[kVsgFields] = {0, 0, 0, 0, 0, 0, 0};
+  bool ok = valid >= 3;
+  if (ok) {
+    const double a[3][3] = {{s1, sx, sy}, {sx, sxx, sxy}, {sy, sxy, syy}};
+    double inv[3][3];
+    ok = inverse3(a, inv);
+    // the reciprocal 1-norm condition number (the reference estimates it with GECON)
+    if (ok) ok = 1.0 / (norm1_3(a) * norm1_3(inv)) >= 1.0e-12;
+    if (ok) {
+      const double bu[3] = {su, sxu, syu}, bv[3] = {sv, sxv, syv};
+      double cu[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          cu[r] += inv[r][c] * bu[c];
+          cv[r] += inv[r][c] * bv[c];
+        }
+      const double dudx = cu[1], dudy = cu[2], dvdx = cv[1], dvdy = cv[2];
+      out[0] = (float)(0.5 * (2.0 * dudx + dudx * dudx + dvdx * dvdx));
+      out[1] = (float)(0.5 * (2.0 * dvdy + dudy * dudy + dvdy * dvdy));
+      out[2] = (float)(0.5 * (dudy + dvdx + dudx * dudy + dvdx * dvdy));
+      out[3] = (float)dudx;  // stored in scalar_t fields
+      out[4] = (float)dudy;
+      out[5] = (float)dvdx;
+      out[6] = (float)dvdy;
+    }
+  }
+  for (int k = 0; k < kVsgFields; ++k) vsg[(size_t)k * n + i] = out[k];
+  if (!ok) F(F_MATCH, i) = -1.0;
+}
+
 }  // namespace
+
+hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vsg_strain_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, g, radius, fields, vsg, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_neighbor_guess(const double *start, const double *cur, double *guess, const double *prev_guess,
                                  double tol, int *list, int *count, int n, hipStream_t stream) {

@@ -31,6 +31,7 @@ struct Tiff {
   int width = 0, height = 0, bits = 8, compression = 1, photometric = 1, predictor = 1;
   int samples_per_pixel = 1, planar = 1, rows_per_strip = 0;
   std::vector<uint32_t> strip_offsets, strip_counts;
+  std::vector<uint16_t> colormap;  // palette images: 3 * 2^bits values (red, green, blue)
   bool big_endian = false;
 };
 
@@ -93,6 +94,10 @@ Tiff parse_tiff(const std::vector<unsigned char> &b, const std::string &file) {
         break;
       case 284: t.planar = (int)value(0); break;
       case 317: t.predictor = (int)value(0); break;
+      case 320:
+        t.colormap.resize(count);
+        for (uint32_t i = 0; i < count; ++i) t.colormap[i] = (uint16_t)value(i);
+        break;
       case 322: case 323: case 324: case 325: tiled = true; break;
       default: break;
     }
@@ -101,7 +106,9 @@ Tiff parse_tiff(const std::vector<unsigned char> &b, const std::string &file) {
   if (t.width < 1 || t.height < 1) fail(file, "bad dimensions");
   if (t.samples_per_pixel != 1) fail(file, "only one grey sample per pixel is supported");
   if (t.bits != 8 && t.bits != 16) fail(file, "only 8- and 16-bit samples are supported");
-  if (t.photometric != 0 && t.photometric != 1) fail(file, "only grey (WhiteIsZero / BlackIsZero) images are supported");
+  if (t.photometric != 0 && t.photometric != 1 && t.photometric != 3)
+    fail(file, "only grey (WhiteIsZero / BlackIsZero) and palette images are supported");
+  if (t.photometric == 3 && (t.bits != 8 || t.colormap.size() != 3 * 256u)) fail(file, "bad palette");
   if (t.compression != 1 && t.compression != 5 && t.compression != 32773)
     fail(file, "compression " + std::to_string(t.compression) + " is not supported");
   if (t.predictor != 1 && t.predictor != 2) fail(file, "predictor " + std::to_string(t.predictor) + " is not supported");
@@ -181,7 +188,8 @@ void packbits_decode(const unsigned char *in, size_t n, std::vector<unsigned cha
 }
 
 // The grey samples of the whole image, row-major, as stored.
-std::vector<uint16_t> tiff_samples(const std::string &file, int &width, int &height, int &bits, int &photometric) {
+std::vector<uint16_t> tiff_samples(const std::string &file, int &width, int &height, int &bits, int &photometric,
+                                   std::vector<uint16_t> *colormap = nullptr) {
   const std::vector<unsigned char> b = slurp(file);
   const Tiff t = parse_tiff(b, file);
   const size_t bps = t.bits / 8, row_bytes = (size_t)t.width * bps;
@@ -214,6 +222,7 @@ std::vector<uint16_t> tiff_samples(const std::string &file, int &width, int &hei
   height = t.height;
   bits = t.bits;
   photometric = t.photometric;
+  if (colormap) *colormap = t.colormap;
   return v;
 }
 
@@ -299,7 +308,17 @@ void read_image(const std::string &file, std::vector<float> &intensities, int &w
   }
   if (type != TIFF) fail(file, type == NO_SUCH_IMAGE_FILE_TYPE ? "unrecognized image file type" : "image file type not supported");
   int w = 0, h = 0, bits = 8, photometric = 1;
-  const std::vector<uint16_t> v = tiff_samples(file, w, h, bits, photometric);
+  std::vector<uint16_t> cmap;
+  const std::vector<uint16_t> v = tiff_samples(file, w, h, bits, photometric, &cmap);
+  // palette images: the 16-bit colour map to 8 bits (x >> 8), then to grey with the BT.601
+  // weights in 14-bit fixed point (R 4899, G 9617, B 1868, rounded); a grey colour map maps
+  // each index to its own grey level exactly
+  uint8_t grey[256] = {0};
+  if (photometric == 3)
+    for (int k = 0; k < 256; ++k) {
+      const uint32_t r = cmap[k] >> 8, gg = cmap[256 + k] >> 8, b = cmap[512 + k] >> 8;
+      grey[k] = (uint8_t)((r * 4899u + gg * 9617u + b * 1868u + 8192u) >> 14);
+    }
   if (sub_offset_x < 0 || sub_offset_y < 0 || sub_w < 0 || sub_h < 0) fail(file, "bad sub-image");
   width = sub_w == 0 ? w : sub_w;
   height = sub_h == 0 ? h : sub_h;
@@ -310,6 +329,7 @@ void read_image(const std::string &file, std::vector<float> &intensities, int &w
       uint32_t s = v[(size_t)(y + sub_offset_y) * w + x + sub_offset_x];
       if (bits == 16) s = (s * 255u + 32767u) / 65535u;
       if (photometric == 0) s = 255u - s;
+      if (photometric == 3) s = grey[s & 255u];
       intensities[(size_t)y * width + x] = (float)s;
     }
 }

@@ -29,11 +29,13 @@ void read_image_dimensions(const std::string &file_name, int &width, int &height
 // A region of a .rawi file throws, as in the reference (DICe_ImageIO.cpp:286-290);
 // .rawi files must hold 4-byte values (the reference's float build rejects
 // other sizes, DICe_Rawi.cpp:79-83).  TIFF: one 8- or 16-bit grey sample
-// per pixel (BlackIsZero or WhiteIsZero), strips, no compression, LZW
-// (with or without the horizontal predictor) or PackBits, either byte
-// order.  Intensities are 8-bit values, as OpenCV's IMREAD_GRAYSCALE
-// delivers them: 16-bit samples scale to (v * 255 + 32767) / 65535 and
-// WhiteIsZero samples are inverted.
+// per pixel (BlackIsZero or WhiteIsZero) or an 8-bit palette index, strips,
+// no compression, LZW (with or without the horizontal predictor) or
+// PackBits, either byte order.  Intensities are 8-bit values, as OpenCV's
+// IMREAD_GRAYSCALE delivers them: 16-bit samples scale to
+// (v * 255 + 32767) / 65535, WhiteIsZero samples are inverted, palette
+// entries become grey by the BT.601 weights (a grey palette, R = G = B,
+// gives its own level).
 void read_image(const std::string &file_name, std::vector<float> &intensities, int &width, int &height,
                 int sub_offset_x = 0, int sub_offset_y = 0, int sub_w = 0, int sub_h = 0);
 

@@ -446,7 +446,9 @@ const std::vector<std::pair<Field_Spec, const char *>> &field_names() {
       {NOISE_LEVEL_FS, "NOISE_LEVEL"}, {CONTRAST_LEVEL_FS, "CONTRAST_LEVEL"}, {ACTIVE_PIXELS_FS, "ACTIVE_PIXELS"},
       {STATUS_FLAG_FS, "STATUS_FLAG"}, {ITERATIONS_FS, "ITERATIONS"}, {CONDITION_NUMBER_FS, "CONDITION_NUMBER"},
       {NEIGHBOR_ID_FS, "NEIGHBOR_ID"}, {FIELD_1_FS, "FIELD_1"}, {FIELD_2_FS, "FIELD_2"}, {FIELD_3_FS, "FIELD_3"},
-      {FIELD_4_FS, "FIELD_4"}, {FIELD_5_FS, "FIELD_5"}, {FIELD_6_FS, "FIELD_6"}, {FIELD_7_FS, "FIELD_7"}};
+      {FIELD_4_FS, "FIELD_4"}, {FIELD_5_FS, "FIELD_5"}, {FIELD_6_FS, "FIELD_6"}, {FIELD_7_FS, "FIELD_7"},
+      {VSG_STRAIN_XX_FS, "VSG_STRAIN_XX"}, {VSG_STRAIN_YY_FS, "VSG_STRAIN_YY"}, {VSG_STRAIN_XY_FS, "VSG_STRAIN_XY"},
+      {VSG_DUDX_FS, "VSG_DUDX"}, {VSG_DUDY_FS, "VSG_DUDY"}, {VSG_DVDX_FS, "VSG_DVDX"}, {VSG_DVDY_FS, "VSG_DVDY"}};
   return v;
 }
 }  // namespace
@@ -762,17 +764,33 @@ int Schema::execute_correlation() {
   const int mode = backend_ == HIP_PLAIN ? DICE_MODE_PLAIN : DICE_MODE_BATCHED;
   check(dice_hip_correlate(ctx_, mode, fields_.data(), frame_id_ == first_frame_id_ ? 1 : 0),
         "execute_correlation");
+  // post_execution_tasks: the post-processors of the frame
+  if (vsg_window_ > 0) {
+    vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
+    check(dice_hip_vsg_strain(ctx_, vsg_window_, fields_.data(), vsg_.data()), "VSG strain");
+  }
   ++frame_id_;  // update_frame_id
   return 0;
 }
 
+void Schema::set_vsg_strain_window(int window_size_in_pixels) {
+  if (window_size_in_pixels < 0)
+    throw std::invalid_argument("Error, k-nearest VSG windows (negative sizes) are not supported");
+  vsg_window_ = window_size_in_pixels;
+}
+
 double &Schema::local_field_value(int local_id, Field_Spec fs) {
   if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
+  if ((int)fs >= 100) {
+    if (vsg_.empty()) vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
+    return vsg_[(size_t)((int)fs - 100) * n_ + local_id];
+  }
   return fields_[(size_t)fs * n_ + local_id];
 }
 
 double Schema::local_field_value(int local_id, Field_Spec fs) const {
   if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
+  if ((int)fs >= 100) return vsg_.empty() ? 0.0 : vsg_[(size_t)((int)fs - 100) * n_ + local_id];
   return fields_[(size_t)fs * n_ + local_id];
 }
 
@@ -837,7 +855,12 @@ void Output_Spec::write_info(std::FILE *file) const {
   std::fprintf(file, "*** Incremental correlation: false\n");
   std::fprintf(file, "*** Subset size: %i\n", s.subset_dim());
   std::fprintf(file, "*** Step size: x %i y %i (-1 implies not regular grid)\n", s.step_size_x(), s.step_size_y());
-  std::fprintf(file, "*** Strain window: N/A\n");
+  // the first strain post-processor's window (DICe_Schema.cpp:4843-4847)
+  if (s.strain_window_size() > 0)
+    std::fprintf(file, "*** Strain window size in pixels: %i (only first strain post-processor is reported)\n",
+                 s.strain_window_size());
+  else
+    std::fprintf(file, "*** Strain window: N/A\n");
   std::fprintf(file, "*** Coordinates given with (0,0) as upper left corner of image, x positive right, y positive down\n");
   std::fprintf(file, "***\n");
 }
@@ -892,7 +915,15 @@ void Schema::write_output(const std::string &output_folder, const std::string &p
     std::FILE *f = open(name, "w");
     spec.write_info(f);
     spec.write_header(f, "SUBSET_ID");
-    for (int i = 0; i < n_; ++i) spec.write_frame(f, i, i);
+    std::vector<int> order(n_);
+    for (int i = 0; i < n_; ++i) order[i] = i;
+    if (sort_txt_output_)
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const double xa = fields_[(size_t)DICE_F_COORD_X * n_ + a], xb = fields_[(size_t)DICE_F_COORD_X * n_ + b];
+        const double ya = fields_[(size_t)DICE_F_COORD_Y * n_ + a], yb = fields_[(size_t)DICE_F_COORD_Y * n_ + b];
+        return xa < xb || (xa == xb && ya < yb);
+      });
+    for (int i : order) spec.write_frame(f, i, i);
     std::fclose(f);
   }
 }

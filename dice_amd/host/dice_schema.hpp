@@ -71,7 +71,15 @@ enum Field_Spec {
   FIELD_4_FS = DICE_F_FIELD_4,
   FIELD_5_FS = DICE_F_FIELD_5,
   FIELD_6_FS = DICE_F_FIELD_6,
-  FIELD_7_FS = DICE_F_FIELD_7
+  FIELD_7_FS = DICE_F_FIELD_7,
+  // fields of the VSG strain post-processor (DICe_FieldEnums.h), held beside the correlation fields
+  VSG_STRAIN_XX_FS = 100 + DICE_VSG_STRAIN_XX,
+  VSG_STRAIN_YY_FS = 100 + DICE_VSG_STRAIN_YY,
+  VSG_STRAIN_XY_FS = 100 + DICE_VSG_STRAIN_XY,
+  VSG_DUDX_FS = 100 + DICE_VSG_DUDX,
+  VSG_DUDY_FS = 100 + DICE_VSG_DUDY,
+  VSG_DVDX_FS = 100 + DICE_VSG_DVDX,
+  VSG_DVDY_FS = 100 + DICE_VSG_DVDY
 };
 
 // Parameter list keyed by the reference's parameter names.  Values are
@@ -234,6 +242,19 @@ class Schema {
   void set_ref_image(int width, int height, const uint16_t *intensities);
   void set_def_image(int width, int height, const float *intensities);
   void set_def_image(int width, int height, const uint16_t *intensities);
+  // The post_process_vsg_strain block of the correlation parameters
+  // (strain_window_size_in_pixels, DICe_PostProcessor.cpp:877-906): after every
+  // execute_correlation the VSG strains of the frame (VSG_*_FS fields).  0: no
+  // VSG post-processor.  Throws std::invalid_argument for a negative size (the
+  // k-nearest form is not supported).
+  void set_vsg_strain_window(int window_size_in_pixels);
+  int strain_window_size() const { return vsg_window_; }
+  // sort_txt_output (DICe_Schema.cpp:4446-4466): write_output's one-file-per-frame
+  // rows in order of x, then y
+  void set_sort_txt_output(bool sort) { sort_txt_output_ = sort; }
+  // The id of the first frame (the reference's first_frame_id_, from
+  // start_image_index): frame ids and output file numbers count from it.
+  void set_first_frame_index(int index) { first_frame_id_ = frame_id_ = index; }
   // NEIGHBOR_ID_FS of every subset (Schema::initialize, DICe_Schema.cpp:1298-1303), the
   // source of the initial guess under USE_NEIGHBOR_VALUES.
   void set_neighbor_ids(const std::vector<int> &neighbor_ids);
@@ -319,6 +340,9 @@ class Schema {
   int step_x_ = -1, step_y_ = -1;
   std::string ref_name_, def_name_;
   std::vector<double> fields_;
+  int vsg_window_ = 0;
+  bool sort_txt_output_ = false;
+  std::vector<double> vsg_;  // DICE_VSG_COUNT x n
 };
 
 // Output_Spec (DICe_Schema.cpp:4673-4945, text output of a local analysis):

@@ -162,6 +162,23 @@ int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int firs
  * initialization, whose rounds wait on the stream between launches. */
 int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame);
 
+/* VSG_Strain_Post_Processor::execute (DICe_PostProcessor.cpp:877-1136): for each
+ * subset, a least-squares plane of u and v over the subsets within
+ * window_size / 2 px (window_size > 0; the reference's k-nearest form, a
+ * negative size, is not supported) whose SIGMA >= 0, and from its gradient the
+ * Green-Lagrange strains.  vsg: DICE_VSG_COUNT * n doubles, slot-major in the
+ * order of enum dice_vsg_field.  Subsets that fail (own SIGMA < 0, fewer than 3
+ * valid neighbours, a singular normal matrix) get zeros and MATCH = -1 in
+ * fields, as in the reference.  dice_hip_vsg_strain takes host arrays and is
+ * synchronous; the _device form takes device arrays on the context stream and
+ * waits for the kernel (the neighbour grid is built on the host). */
+enum dice_vsg_field {
+  DICE_VSG_STRAIN_XX = 0, DICE_VSG_STRAIN_YY, DICE_VSG_STRAIN_XY,
+  DICE_VSG_DUDX, DICE_VSG_DUDY, DICE_VSG_DVDX, DICE_VSG_DVDY, DICE_VSG_COUNT
+};
+int dice_hip_vsg_strain(dice_hip_context *ctx, int window_size, double *fields, double *vsg);
+int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *device_fields, double *device_vsg);
+
 /* Correlation rounds the last neighbour-initialized frame took (0 after a
  * USE_FIELD_VALUES frame).  No counterpart in the reference, which correlates
  * the subsets in order, once each: a diagnostic of this library's rounds. */

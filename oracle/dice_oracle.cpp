@@ -1131,6 +1131,77 @@ int oracle_gauss_filter_d(int w, int h, double *img, int mask_size) {
   return gauss_filter_t<double, double>(mask_size, w, h, img);
 }
 
+int oracle_vsg_strain(int n, double *F, int window, double *vsg) {
+  // VSG_Strain_Post_Processor: pre_execution_tasks + initialize_neighborhood (radius search,
+  // DICe_PostProcessor.cpp:140-236, 908-929) and execute (:931-1136).  Neighbours by brute
+  // force, sorted by distance as nanoflann's sorted radius search returns them (ties by id);
+  // squared distances and the radius test in float, as the float kd-tree has them.
+  if (n < 1 || !F || !vsg || window <= 0) return -1;
+  auto f = [&](int slot, int i) -> double & { return F[(size_t)slot * n + i]; };
+  const float rad = (float)window / 2.0f;
+  const float r2 = rad * rad + (float)1.0E-5;
+  for (int i = 0; i < n; ++i) {
+    const float xi = (float)f(OF_COORD_X, i), yi = (float)f(OF_COORD_Y, i);
+    std::vector<std::pair<float, int>> nb;
+    for (int j = 0; j < n; ++j) {
+      const float dx = (float)f(OF_COORD_X, j) - xi, dy = (float)f(OF_COORD_Y, j) - yi;
+      const float d2 = dx * dx + dy * dy;
+      if (d2 < r2) nb.push_back({d2, j});
+    }
+    std::sort(nb.begin(), nb.end());
+    double out[7] = {0, 0, 0, 0, 0, 0, 0};
+    int valid = 0;
+    for (const auto &e : nb) valid += f(OF_SIGMA, e.second) >= 0.0;
+    bool ok = valid >= 3 && !nb.empty() && f(OF_SIGMA, nb[0].second) >= 0.0;
+    if (ok) {
+      double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, bu[3] = {0, 0, 0}, bv[3] = {0, 0, 0};
+      for (const auto &e : nb) {
+        const int j = e.second;
+        if (!(f(OF_SIGMA, j) >= 0.0)) continue;
+        // neighbour offsets and displacements as scalar_t (float), DICe_PostProcessor.cpp:219-220,1011-1018
+        const double row[3] = {1.0, (double)((float)f(OF_COORD_X, j) - xi), (double)((float)f(OF_COORD_Y, j) - yi)};
+        const double u = (float)f(OF_DISP_X, j), v = (float)f(OF_DISP_Y, j);
+        for (int k = 0; k < 3; ++k) {
+          for (int m = 0; m < 3; ++m) A[k][m] += row[k] * row[m];
+          bu[k] += row[k] * u;
+          bv[k] += row[k] * v;
+        }
+      }
+      // GETRF + GECON + GETRI (:1047-1085); the condition number exactly in the 1-norm
+      double anorm = 0.0;
+      for (int j = 0; j < 3; ++j) anorm = std::max(anorm, std::abs(A[0][j]) + std::abs(A[1][j]) + std::abs(A[2][j]));
+      double inv[6][6];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inv[r][c] = A[r][c];
+      const int info = lu_inverse(3, inv);
+      double inorm = 0.0;
+      for (int j = 0; j < 3; ++j)
+        inorm = std::max(inorm, std::abs(inv[0][j]) + std::abs(inv[1][j]) + std::abs(inv[2][j]));
+      ok = info == 0 && 1.0 / (anorm * inorm) >= 1.0E-12;
+      if (ok) {
+        double cu[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            cu[r] += inv[r][c] * bu[c];
+            cv[r] += inv[r][c] * bv[c];
+          }
+        const double dudx = cu[1], dudy = cu[2], dvdx = cv[1], dvdy = cv[2];
+        // the Green-Lagrange strains (:1118-1124), stored in scalar_t fields
+        out[0] = (float)(0.5 * (2.0 * dudx + dudx * dudx + dvdx * dvdx));
+        out[1] = (float)(0.5 * (2.0 * dvdy + dudy * dudy + dvdy * dvdy));
+        out[2] = (float)(0.5 * (dudy + dvdx + dudx * dudy + dvdx * dvdy));
+        out[3] = (float)dudx;
+        out[4] = (float)dudy;
+        out[5] = (float)dvdx;
+        out[6] = (float)dvdy;
+      }
+    }
+    for (int k = 0; k < 7; ++k) vsg[(size_t)k * n + i] = out[k];
+    if (!ok) f(OF_MATCH, i) = -1.0;
+  }
+  return 0;
+}
+
 int oracle_lu_inverse(int n, double *a) {
   if (n < 1 || n > 6) return -1;
   double A[6][6];

@@ -127,6 +127,11 @@ int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const 
 int oracle_gauss_filter_f(int w, int h, int storage, void *img, int mask_size);
 int oracle_gauss_filter_d(int w, int h, double *img, int mask_size);
 
+/* VSG_Strain_Post_Processor over n subsets: fields as for oracle_correlate (COORD, DISP, SIGMA
+ * read; MATCH set to -1 where the strain fails); vsg: 7 * n doubles (strain xx, yy, xy, du/dx,
+ * du/dy, dv/dx, dv/dy).  window > 0 (radius window / 2).  Returns 0 or -1. */
+int oracle_vsg_strain(int n, double *fields, int window, double *vsg);
+
 /* 6x6 LU with partial pivoting + inverse (GETRF/GETRI semantics); returns INFO. */
 int oracle_lu_inverse(int n, double *a_rowmajor);
 

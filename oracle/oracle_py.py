@@ -84,6 +84,7 @@ def lib():
         _lib.oracle_lu_inverse.argtypes = [i, P]
         _lib.oracle_gauss_filter_f.argtypes = [i, i, i, P, i]
         _lib.oracle_gauss_filter_d.argtypes = [i, i, P, i]
+        _lib.oracle_vsg_strain.argtypes = [i, P, i, P]
     return _lib
 
 
@@ -229,6 +230,16 @@ def gauss_filter(img, mask_size):
         raise TypeError("gauss_filter: float32, uint16 or float64 image")
     if rc != 0:
         raise ValueError("gauss_filter: bad mask size or image too small")
+    return out
+
+
+def vsg_strain(fields, window):
+    """VSG_Strain_Post_Processor: returns (7, n) VSG fields; sets MATCH = -1 in fields where it fails."""
+    assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape[0] == NUM_FIELDS
+    n = fields.shape[1]
+    out = np.zeros((7, n))
+    if lib().oracle_vsg_strain(n, _p(fields), window, _p(out)) != 0:
+        raise ValueError("vsg_strain: bad arguments")
     return out
 
 

@@ -201,9 +201,13 @@ int subset_file_info(const char *path) {
 // separate (0|1), backend (0|1), ref_name, def_name, routine (0|1), and
 // optional fields (comma-separated output names), optimization /
 // interpolation / initialization (enum values), gauss_filter_mask_size (turns
-// the Gauss filter on) and frames (the deformed image correlated that many
+// the Gauss filter on), frames (the deformed image correlated that many
 // times, one output file per frame; def_names: the comma-separated names for
-// the header of each frame).  A subset file of regions of interest
+// the header of each frame; def_files: one image file per frame),
+// enable_rotation / enable_normal_strain / enable_shear_strain, vsg_window
+// (the VSG strain post-processor), sort, omit_row_id, delimiter,
+// first_frame_index and num_frames (the frame count of the whole sequence,
+// which sets the digits of the output file numbers).  A subset file of regions of interest
 // gives the grid (step_size) inside them, with the seeds' values and the
 // neighbour ids.  Default parameters otherwise.
 int deck(const char *config_path) {
@@ -226,6 +230,8 @@ int deck(const char *config_path) {
   if (c.count("optimization")) params.set("optimization_method", num("optimization"));
   if (c.count("interpolation")) params.set("interpolation_method", num("interpolation"));
   if (c.count("initialization")) params.set("initialization_method", num("initialization"));
+  for (const char *k : {"enable_rotation", "enable_normal_strain", "enable_shear_strain"})
+    if (c.count(k)) params.set(k, num(k));
   if (c.count("gauss_filter_mask_size")) {
     params.set("gauss_filter_images", 1);
     params.set("gauss_filter_mask_size", num("gauss_filter_mask_size"));
@@ -248,6 +254,17 @@ int deck(const char *config_path) {
     schema.set_neighbor_ids(neighbors);
     schema.set_seed_values(info);
   }
+  if (c.count("vsg_window")) schema.set_vsg_strain_window(num("vsg_window"));
+  if (c.count("sort")) schema.set_sort_txt_output(num("sort") != 0);
+  if (c.count("first_frame_index")) schema.set_first_frame_index(num("first_frame_index"));
+  const std::string delimiter = c.count("delimiter") ? c.at("delimiter") : " ";
+  const bool omit_row_id = c.count("omit_row_id") && num("omit_row_id") != 0;
+  std::vector<std::string> def_files;
+  if (c.count("def_files")) {
+    std::stringstream ss(c.at("def_files"));
+    std::string f;
+    while (std::getline(ss, f, ',')) def_files.push_back(f);
+  }
   if (files) schema.set_ref_image(c.at("ref_file"));
   else schema.set_ref_image(w, h, ref.data());
   schema.set_image_file_names(c.at("ref_name"), c.at("def_name"));
@@ -265,11 +282,13 @@ int deck(const char *config_path) {
     while (std::getline(ss, f, ',')) def_names.push_back(f);
   }
   for (int frame = 0; frame < frames; ++frame) {
-    if (files) schema.set_def_image(c.at("def_file"));
+    if (!def_files.empty()) schema.set_def_image(def_files.at(frame));
+    else if (files) schema.set_def_image(c.at("def_file"));
     else schema.set_def_image(w, h, def.data());
     schema.set_image_file_names(c.at("ref_name"), frame < (int)def_names.size() ? def_names[frame] : c.at("def_name"));
     schema.execute_correlation();
-    schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, " ", false, frames);
+    schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, delimiter, omit_row_id,
+                        c.count("num_frames") ? num("num_frames") : frames);
   }
   for (int i = 0; i < schema.local_num_subsets(); ++i)
     std::cout << "subset " << i << " x " << cx[i] << " y " << cy[i] << " u "

@@ -66,5 +66,59 @@ def main():
             json.dump(deck, f, indent=1)
 
 
+def vsg_deck():
+    """tests/regression/dic_challenge_12_sorted (a region of interest with an excluded circle, step 35,
+    USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY, normal strains, the VSG strain post-processor with a 360-px
+    window, sorted comma-separated output without row ids), cut to its first two frames: the reference
+    image oht_cfrp_00.tiff and the deformed oht_cfrp_01/02.tiff (8-bit grey palette TIFFs whose palette is
+    the identity, stored as their pixel values in images.npz) and the golds DICe_solution_01/02.txt."""
+    import numpy as np
+    from PIL import Image
+    name = "dic_challenge_12_sorted"
+    src, dst = os.path.join(SRC, name), os.path.join(DST, name)
+    os.makedirs(os.path.join(dst, "gold"), exist_ok=True)
+    inp = top_params(ET.parse(os.path.join(src, "input.xml")).getroot())
+    cor_root = ET.parse(os.path.join(src, inp["correlation_parameters_file"])).getroot()
+    cor = top_params(cor_root)
+    frames = {}
+    for k in range(3):
+        im = Image.open(os.path.join(src, "images", "oht_cfrp_%02d.tiff" % k))
+        pal = im.getpalette()
+        assert all(pal[3 * i] == pal[3 * i + 1] == pal[3 * i + 2] == i for i in range(256))
+        frames["frame_%d" % k] = np.asarray(im, dtype=np.uint8)
+    np.savez_compressed(os.path.join(dst, "images.npz"), **frames)
+    shutil.copy(os.path.join(src, inp["subset_file"]), os.path.join(dst, "subsets.txt"))
+    spec = cor_root.find("ParameterList[@name='output_spec']")
+    fields = sorted(spec.findall("Parameter"), key=lambda q: int(q.get("value")))
+    deck = {"subset_size": int(inp["subset_size"]), "step_size": int(inp["step_size"]), "separate": False,
+            "prefix": inp.get("output_prefix", "DICe_solution"),
+            "reference_image": inp["image_folder"] + "oht_cfrp_00.tiff",
+            "deformed_image": inp["image_folder"] + "oht_cfrp_01.tiff",
+            "deformed_images": [inp["image_folder"] + "oht_cfrp_%02d.tiff" % k for k in (1, 2)],
+            "frames": 2, "first_frame_index": int(inp["start_image_index"]),
+            "num_frames": int(inp["end_image_index"]) - int(inp["start_image_index"]) + 1,
+            "subset_file": "subsets.txt", "images": "images.npz",
+            "vsg_window": int(cor_root.find("ParameterList[@name='post_process_vsg_strain']")
+                              .find("Parameter").get("value")),
+            "sort": cor.get("sort_txt_output") == "true", "omit_row_id": cor.get("omit_output_row_id") == "true",
+            "delimiter": cor.get("output_delimiter", " "),
+            "fields": [q.get("name") for q in fields],
+            "gold": ["DICe_solution_01.txt", "DICe_solution_02.txt"],
+            "diff_tolerance": 0.05, "diff_floor": 5.0e-5}
+    for k in ("enable_rotation", "enable_normal_strain", "enable_shear_strain"):
+        deck[k] = 1 if cor.get(k) == "true" else 0
+    for key, names in ENUMS.items():
+        if key in cor:
+            deck[key] = names.index(cor[key].upper())
+    for g in deck["gold"]:
+        shutil.copy(os.path.join(src, "gold", g), os.path.join(dst, "gold", g))
+    for root, _, files in os.walk(dst):
+        for f in files:
+            os.chmod(os.path.join(root, f), 0o644)
+    with open(os.path.join(dst, "deck.json"), "w") as f:
+        json.dump(deck, f, indent=1)
+
+
 if __name__ == "__main__":
     main()
+    vsg_deck()

@@ -152,6 +152,79 @@ def test_oracle_neighbor_initialization_matches_seed_golds(name, frames):
     assert not np.array_equal(f2[o.F["DISP_X"]], f[o.F["DISP_X"]])
 
 
+VSG_DECK = os.path.join(GOLD, "regression_decks", "dic_challenge_12_sorted")
+
+
+def vsg_deck_frames():
+    import json
+    deck = json.load(open(os.path.join(VSG_DECK, "deck.json")))
+    im = np.load(os.path.join(VSG_DECK, deck["images"]))
+    return deck, [im["frame_%d" % k] for k in range(3)]
+
+
+def gold_csv(path):
+    return np.array([[float(t) for t in ln.split(",")] for ln in open(path) if ln[:1].isdigit() or ln[:1] == "-"])
+
+
+def test_oracle_vsg_deck_matches_gold():
+    """tests/regression/dic_challenge_12_sorted, frames 1 and 2: the grid in a region of interest with an
+    excluded circle, USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY, u, v and normal strains by Gauss-Newton, then the
+    VSG strains over a 360-px window; the rows sorted by x, then y.  The oracle (double build) against
+    the gold to its printed digits: coordinates and MATCH exactly, u and v within 5e-5 px, the VSG strains,
+    gamma within 2e-4 relative (5 significant digits are printed)."""
+    deck, frames = vsg_deck_frames()
+    h, w = frames[0].shape
+    r = run("grid", w, h, deck["subset_size"], deck["step_size"], os.path.join(VSG_DECK, deck["subset_file"]))
+    pts = np.array([[float(t) for t in ln.split()] for ln in r.stdout.splitlines() if not ln.startswith("seed")])
+    p = o.default_params()
+    p.initialization_method = deck["initialization_method"]
+    p.optimization_method = deck["optimization_method"]
+    p.enable_normal_strain = deck["enable_normal_strain"]
+    f = o.new_fields(pts[:, 0], pts[:, 1])
+    f[o.F["NEIGHBOR_ID"]] = pts[:, 2]
+    for k, gold in zip((1, 2), deck["gold"]):
+        o.correlate(p, frames[0], frames[k], f, subset_size=deck["subset_size"], first_frame=k == 1,
+                    precision="double")
+        vsg = o.vsg_strain(f, deck["vsg_window"])
+        g = gold_csv(os.path.join(VSG_DECK, "gold", gold))
+        order = np.lexsort((f[o.F["COORD_Y"]], f[o.F["COORD_X"]]))
+        ours = np.stack([f[o.F["COORD_X"]], f[o.F["COORD_Y"]], f[o.F["DISP_X"]], f[o.F["DISP_Y"]], vsg[0], vsg[1],
+                         f[o.F["GAMMA"]], f[o.F["MATCH"]]], 1)[order]
+        assert ours.shape == g.shape
+        assert np.array_equal(ours[:, [0, 1, 7]], g[:, [0, 1, 7]])
+        assert np.abs(ours[:, 2:4] - g[:, 2:4]).max() <= 5e-5
+        rel = np.abs(ours[:, 4:7] - g[:, 4:7]) / np.abs(g[:, 4:7])
+        assert rel.max() <= 2e-4
+
+
+def test_oracle_vsg_closed_forms():
+    """The VSG plane fit: a linear displacement field gives its gradient exactly (to the float storage of
+    u and v) and the Green-Lagrange strains of that gradient; subsets with sigma < 0, with fewer than 3
+    valid neighbours, or whose valid neighbours lie on one line get zeros and MATCH = -1."""
+    xs, ys = np.meshgrid(np.arange(20.0, 400.0, 10.0), np.arange(20.0, 300.0, 10.0))
+    cx, cy = xs.ravel(), ys.ravel()
+    f = o.new_fields(cx, cy)
+    a, b, c, d = 0.01, -0.002, 0.004, 0.003
+    f[o.F["DISP_X"]] = 0.3 + a * cx + b * cy
+    f[o.F["DISP_Y"]] = -0.1 + c * cx + d * cy
+    f[o.F["SIGMA"]] = 0.01
+    f[o.F["SIGMA"], 5] = -1.0
+    v = o.vsg_strain(f, 41)
+    ok = np.arange(len(cx)) != 5
+    for k, want in ((3, a), (4, b), (5, c), (6, d)):
+        assert np.abs(v[k][ok] - want).max() < 2e-8
+    assert np.allclose(v[0][ok], 0.5 * (2 * a + a * a + c * c), atol=1e-7)
+    assert np.allclose(v[1][ok], 0.5 * (2 * d + b * b + d * d), atol=1e-7)
+    assert np.allclose(v[2][ok], 0.5 * (b + c + a * b + c * d), atol=1e-7)
+    assert np.all(v[:, 5] == 0) and f[o.F["MATCH"], 5] == -1 and np.all(f[o.F["MATCH"]][ok] == 0)
+    # one row of subsets: every neighbourhood is collinear (singular normal matrix); two points: too few
+    g = o.new_fields(np.arange(10.0, 200.0, 10.0), np.full(19, 50.0))
+    g[o.F["SIGMA"]] = 0.01
+    assert np.all(o.vsg_strain(g, 41) == 0) and np.all(g[o.F["MATCH"]] == -1)
+    h = o.new_fields(np.array([10.0, 20.0]), np.array([10.0, 12.0]))
+    assert np.all(o.vsg_strain(h, 41) == 0) and np.all(h[o.F["MATCH"]] == -1)
+
+
 def test_serpentine_grid_helper_matches_cpp():
     import dice_amd as dh
     for w, h, s, step in ((512, 512, 27, 75), (300, 211, 21, 17), (128, 640, 31, 9)):
@@ -359,6 +432,59 @@ def test_regular_grid_order_matches_regression_gold():
     gold = np.array([[float(t[1]), float(t[2])] for t in rows])
     assert np.array_equal(pts[:, :2], gold)
     assert list(pts[:2, 2]) == [-1, -1] and np.array_equal(pts[2:, 2], np.arange(1, len(pts) - 1))
+
+
+def write_tiff_u8(path, img):
+    """Little-endian, uncompressed 8-bit grey TIFF, one strip (TIFF 6.0 section 2)."""
+    import struct
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    tags = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 1, 8), (259, 3, 1, 1), (262, 3, 1, 1),
+            (273, 4, 1, 8 + 2 + 12 * 8 + 4), (278, 4, 1, h), (279, 4, 1, w * h)]
+    out = bytearray(b"II" + struct.pack("<HI", 42, 8) + struct.pack("<H", len(tags)))
+    for tag, typ, cnt, val in tags:
+        out += struct.pack("<HHI", tag, typ, cnt) + (struct.pack("<HH", val, 0) if typ == 3 else struct.pack("<I", val))
+    out += struct.pack("<I", 0) + img.tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
+def test_vsg_regression_deck(tmp_path, backend):
+    """dic_challenge_12_sorted (frames 1 and 2) end to end through the Schema mirror: TIFF images read by
+    name, the region-of-interest grid, neighbour initialization on the first frame, normal strains, the
+    VSG post-processor on the GPU, sorted comma-separated output without row ids -- dice_diff against both
+    golds with the suite's options, and the columns within the oracle test's tolerances."""
+    deck, frames = vsg_deck_frames()
+    h, w = frames[0].shape
+    names = []
+    for k, img in enumerate(frames):
+        write_tiff_u8(tmp_path / ("oht_cfrp_%02d.tiff" % k), img)
+        names.append(str(tmp_path / ("oht_cfrp_%02d.tiff" % k)))
+    cfg = {"ref_file": names[0], "def_file": names[1], "def_files": ",".join(names[1:]), "width": w, "height": h,
+           "subset_size": deck["subset_size"], "step_size": deck["step_size"],
+           "subset_file": os.path.join(VSG_DECK, deck["subset_file"]), "out_dir": tmp_path,
+           "prefix": deck["prefix"], "separate": 0, "backend": backend, "ref_name": deck["reference_image"],
+           "def_name": deck["deformed_image"], "def_names": ",".join(deck["deformed_images"]), "routine": 0,
+           "fields": ",".join(deck["fields"]), "optimization": deck["optimization_method"],
+           "interpolation": deck["interpolation_method"], "initialization": deck["initialization_method"],
+           "enable_normal_strain": deck["enable_normal_strain"], "vsg_window": deck["vsg_window"],
+           "sort": int(deck["sort"]), "omit_row_id": int(deck["omit_row_id"]), "delimiter": deck["delimiter"],
+           "frames": deck["frames"], "first_frame_index": deck["first_frame_index"],
+           "num_frames": deck["num_frames"]}
+    cfg_path = tmp_path / "deck.cfg"
+    cfg_path.write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+    r = run("deck", cfg_path)
+    assert r.returncode == 0, r.stderr
+    for gname in deck["gold"]:
+        gold = os.path.join(VSG_DECK, "gold", gname)
+        code, out = dice_diff(gold, tmp_path / gname, "-t", deck["diff_tolerance"], "-f", deck["diff_floor"], "-v")
+        assert code == 0 and "TEST PASSED" in out, (gname, out)
+        g, ours = gold_csv(gold), gold_csv(tmp_path / gname)
+        assert np.array_equal(ours[:, [0, 1, 7]], g[:, [0, 1, 7]])
+        assert np.abs(ours[:, 2:4] - g[:, 2:4]).max() <= 1e-4
+        assert (np.abs(ours[:, 4:7] - g[:, 4:7]) / np.abs(g[:, 4:7])).max() <= 5e-3
 
 
 @pytest.mark.gpu
