@@ -35,6 +35,10 @@ of 155,236 subsets, each starting from its neighbour's result), 6-parameter
 Gauss-Newton, rounds until no guess moves by more than --settle (1e-6 px).
 Also the exact order (settle 0) on a 512^2 crop.  One step = one frame.
 
+--workload vsg (post-processing): the c2 frame with A = 1.0 px correlated once,
+then the VSG strain post-processor over its 155,236 subsets with a 121-px
+window (--vsg-window).  One step = one post-processor call.
+
 --workload gauss (image ingest): random --size^2 float32 frames (2048 by
 default) resident in HBM, each set as the deformed image with
 gauss_filter_images on (--mask, 7 by default), no gradients.  One step = one
@@ -812,12 +816,85 @@ def bench_neighbors(args):
     return result
 
 
+def bench_vsg(args):
+    """The VSG strain post-processor over the headline frame's 155,236 subsets."""
+    import torch
+    import dice_amd as dh
+    from dice_amd import synth
+
+    size, S, step, window = args.size, args.subset, args.step, args.vsg_window
+    ref, frames = synth.sinusoid_sequence(size=size, frames=10, seed=1)
+    cx, cy = dh.regular_grid(size, size, S, step)
+    n = len(cx)
+    p = dh.default_params()
+    p.optimization_method = dh.GRADIENT_BASED
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    ctx = dh.Context(0)
+    ctx.set_params(p)
+    ctx.set_ref_image(ref)
+    ctx.set_def_image(frames[-1])
+    ctx.set_subsets(cx, cy, S)
+    f = dh.new_fields(cx, cy)
+    ctx.correlate(f, mode=dh.MODE_BATCHED)
+    dev = torch.device("cuda:0")
+    d_f = torch.from_numpy(f).to(dev)
+    d_v = torch.zeros((len(dh.VSG_FIELDS), n), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    lib = dh.lib()
+
+    def once():
+        rc = lib.dice_hip_vsg_strain_device(ctx.h, window, ctypes_p(d_f.data_ptr()), ctypes_p(d_v.data_ptr()))
+        assert rc == 0
+
+    for _ in range(args.warmup):
+        once()
+    steps = max(args.steps, 10)
+    t = time.perf_counter()
+    for _ in range(steps):
+        once()
+    el = time.perf_counter() - t
+    v = d_v.cpu().numpy()
+    # the field's own strain (u = A sin(2 pi X / 256), v = 0.5 A cos(2 pi Y / 256), A = 1): du/dx at the centres
+    dudx_true = (2 * np.pi / 256) * np.cos(2 * np.pi * cx / 256)
+    log("vsg: %d subsets, window %d, %.3f ms per call" % (n, window, 1000 * el / steps))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as o
+        # a 256 x 256 px corner of the grid (the oracle's neighbour search is brute force)
+        sel = (cx < 256 + 2 * S) & (cy < 256 + 2 * S)
+        fs = np.ascontiguousarray(f[:, sel])
+        t0 = time.perf_counter()
+        o.vsg_strain(fs, window)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(sel.sum() / cs, 1), "unit": "subsets/s", "cores": 1, "kind": "port",
+               "sample": "the oracle's VSG (brute-force neighbour search) on the %d subsets of a corner of the "
+                         "grid" % int(sel.sum())}
+    result = result_line(
+        args, "VSG strain subsets/sec (window %d px), %d subsets on %d^2" % (window, n, size),
+        n * steps / el, "subsets/s", 1000.0 * el / steps, "f64 (float neighbour distances)",
+        {"workload": "the c2 frame with A = 1.0 px correlated once (6-parameter GN), then the VSG strain "
+                     "post-processor over all %d subsets with a %d-px window; each call includes the "
+                     "host-side neighbour grid and its upload" % (n, window),
+         "window": window, "match_failed": int((d_f[dh.F["MATCH"]] == -1).sum().item()),
+         "mean_abs_dudx_error": float(np.abs(v[3] - dudx_true).mean())},
+        None, cpu_baseline=cpu)
+    ctx.close()
+    return result
+
+
+def ctypes_p(ptr):
+    import ctypes
+    return ctypes.c_void_p(ptr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "gauss", "neighbors"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "gauss", "neighbors", "vsg"], default="c2")
+    ap.add_argument("--vsg-window", type=int, default=121, help="vsg: strain window in pixels")
     ap.add_argument("--settle", type=float, default=1e-6, help="neighbors: neighbor_settle_tolerance")
     ap.add_argument("--mask", type=int, default=7, help="gauss: mask size (5, 7, 9, 11, 13)")
     ap.add_argument("--size", type=int, default=2048, help="c2: image side")
@@ -850,7 +927,7 @@ def main():
               "measured unit; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
     run = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "gauss": bench_gauss,
-           "neighbors": bench_neighbors}[args.workload]
+           "neighbors": bench_neighbors, "vsg": bench_vsg}[args.workload]
     print(json.dumps(run(args)), flush=True)
     return 0
 

@@ -88,6 +88,7 @@ EXPORTS = [
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
     "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
+    "dice_hip_uncertainty", "dice_hip_uncertainty_device",
 ]
 
 _lib = None
@@ -148,6 +149,8 @@ def lib():
         L.dice_hip_neighbor_rounds.argtypes = [P]
         L.dice_hip_vsg_strain.argtypes = [P, I, P, P]
         L.dice_hip_vsg_strain_device.argtypes = [P, I, P, P]
+        L.dice_hip_uncertainty.argtypes = [P, P, P]
+        L.dice_hip_uncertainty_device.argtypes = [P, P, P]
         _lib = L
     return _lib
 
@@ -297,6 +300,13 @@ class Context:
         assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape == (NUM_FIELDS, self.n)
         out = np.zeros((len(VSG_FIELDS), self.n))
         self._check(self._lib.dice_hip_vsg_strain(self.h, window_size, _p(fields), _p(out)), "vsg_strain")
+        return out
+
+    def uncertainty(self, fields):
+        """dice_hip_uncertainty: (2, n) array of UNCERTAINTY and UNCERTAINTY_ANGLE from fields."""
+        assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape == (NUM_FIELDS, self.n)
+        out = np.zeros((2, self.n))
+        self._check(self._lib.dice_hip_uncertainty(self.h, _p(fields), _p(out)), "uncertainty")
         return out
 
     def neighbor_rounds(self):

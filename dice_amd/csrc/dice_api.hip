@@ -714,6 +714,31 @@ int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *d
   return vsg_enqueue(ctx, window_size, device_fields, device_vsg);
 }
 
+int dice_hip_uncertainty_device(dice_hip_context *ctx, const double *device_fields, double *device_out) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!device_fields || !device_out) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
+  (void)hipSetDevice(ctx->device);
+  HIP_TRY(ctx, dice_hip::launch_uncertainty(device_fields, device_out, ctx->n, ctx->stream), "uncertainty kernel");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_uncertainty(dice_hip_context *ctx, const double *fields, double *out) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!fields || !out) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
+  (void)hipSetDevice(ctx->device);
+  const size_t bytes = (size_t)DICE_F_COUNT * ctx->n * sizeof(double), obytes = 2 * (size_t)ctx->n * sizeof(double);
+  HIP_TRY(ctx, ctx->vsg_out.ensure(std::max(obytes, (size_t)dice_hip::kVsgFields * ctx->n * sizeof(double))),
+          "hipMalloc output");
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->fields.p, fields, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy fields");
+  HIP_TRY(ctx, dice_hip::launch_uncertainty(ctx->fields.as<double>(), ctx->vsg_out.as<double>(), ctx->n, ctx->stream),
+          "uncertainty kernel");
+  HIP_TRY(ctx, hipMemcpyAsync(out, ctx->vsg_out.p, obytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy output");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "uncertainty");
+  return DICE_HIP_OK;
+}
+
 int dice_hip_vsg_strain(dice_hip_context *ctx, int window_size, double *fields, double *vsg) {
   if (!ctx) return DICE_HIP_EINVAL;
   if (!fields || !vsg) return ctx->fail(DICE_HIP_EINVAL, "null pointer");

@@ -180,6 +180,11 @@ struct VsgGrid {
 hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream);
 constexpr int kVsgFields = 7;
 
+// Uncertainty_Post_Processor::execute (DICe_PostProcessor.cpp:1507-1537) for a
+// 2-D analysis: out[0][i] = UNCERTAINTY = SIGMA / FIELD_1 (0 when FIELD_1 is 0
+// or SIGMA < 0), out[1][i] = UNCERTAINTY_ANGLE = FIELD_1, in scalar_t (float).
+hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStream_t stream);
+
 // Enqueue the correlation of all subsets, or of the device list items (count
 // *item_count) when items is not null.
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,

@@ -308,7 +308,23 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, double *fields, doubl
   if (!ok) F(F_MATCH, i) = -1.0;
 }
 
+__global__ void uncertainty_kernel(const double *fields, double *out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float angle = (float)fields[(size_t)F_FIELD_1 * n + i];
+  const float sig = (float)fields[(size_t)F_SIGMA * n + i];
+  // 1.0 / angle * sig with the double literal of the reference, stored as scalar_t
+  out[i] = sig < 0.0f ? 0.0f : angle == 0.0f ? 0.0f : (float)(1.0 / (double)angle * (double)sig);
+  out[(size_t)n + i] = angle;
+}
+
 }  // namespace
+
+hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(uncertainty_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, out, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;

@@ -448,7 +448,8 @@ const std::vector<std::pair<Field_Spec, const char *>> &field_names() {
       {NEIGHBOR_ID_FS, "NEIGHBOR_ID"}, {FIELD_1_FS, "FIELD_1"}, {FIELD_2_FS, "FIELD_2"}, {FIELD_3_FS, "FIELD_3"},
       {FIELD_4_FS, "FIELD_4"}, {FIELD_5_FS, "FIELD_5"}, {FIELD_6_FS, "FIELD_6"}, {FIELD_7_FS, "FIELD_7"},
       {VSG_STRAIN_XX_FS, "VSG_STRAIN_XX"}, {VSG_STRAIN_YY_FS, "VSG_STRAIN_YY"}, {VSG_STRAIN_XY_FS, "VSG_STRAIN_XY"},
-      {VSG_DUDX_FS, "VSG_DUDX"}, {VSG_DUDY_FS, "VSG_DUDY"}, {VSG_DVDX_FS, "VSG_DVDX"}, {VSG_DVDY_FS, "VSG_DVDY"}};
+      {VSG_DUDX_FS, "VSG_DUDX"}, {VSG_DUDY_FS, "VSG_DUDY"}, {VSG_DVDX_FS, "VSG_DVDX"}, {VSG_DVDY_FS, "VSG_DVDY"},
+      {UNCERTAINTY_FS, "UNCERTAINTY"}, {UNCERTAINTY_ANGLE_FS, "UNCERTAINTY_ANGLE"}};
   return v;
 }
 }  // namespace
@@ -769,6 +770,10 @@ int Schema::execute_correlation() {
     vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
     check(dice_hip_vsg_strain(ctx_, vsg_window_, fields_.data(), vsg_.data()), "VSG strain");
   }
+  if (uncertainty_pp_) {
+    unc_.assign(2 * (size_t)n_, 0.0);
+    check(dice_hip_uncertainty(ctx_, fields_.data(), unc_.data()), "uncertainty");
+  }
   ++frame_id_;  // update_frame_id
   return 0;
 }
@@ -781,6 +786,10 @@ void Schema::set_vsg_strain_window(int window_size_in_pixels) {
 
 double &Schema::local_field_value(int local_id, Field_Spec fs) {
   if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
+  if ((int)fs >= 110) {
+    if (unc_.empty()) unc_.assign(2 * (size_t)n_, 0.0);
+    return unc_[(size_t)((int)fs - 110) * n_ + local_id];
+  }
   if ((int)fs >= 100) {
     if (vsg_.empty()) vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
     return vsg_[(size_t)((int)fs - 100) * n_ + local_id];
@@ -790,6 +799,7 @@ double &Schema::local_field_value(int local_id, Field_Spec fs) {
 
 double Schema::local_field_value(int local_id, Field_Spec fs) const {
   if (local_id < 0 || local_id >= n_) throw std::runtime_error("Error, subset id out of range");
+  if ((int)fs >= 110) return unc_.empty() ? 0.0 : unc_[(size_t)((int)fs - 110) * n_ + local_id];
   if ((int)fs >= 100) return vsg_.empty() ? 0.0 : vsg_[(size_t)((int)fs - 100) * n_ + local_id];
   return fields_[(size_t)fs * n_ + local_id];
 }

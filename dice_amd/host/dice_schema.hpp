@@ -79,7 +79,10 @@ enum Field_Spec {
   VSG_DUDX_FS = 100 + DICE_VSG_DUDX,
   VSG_DUDY_FS = 100 + DICE_VSG_DUDY,
   VSG_DVDX_FS = 100 + DICE_VSG_DVDX,
-  VSG_DVDY_FS = 100 + DICE_VSG_DVDY
+  VSG_DVDY_FS = 100 + DICE_VSG_DVDY,
+  // fields of the uncertainty post-processor
+  UNCERTAINTY_FS = 110,
+  UNCERTAINTY_ANGLE_FS = 111
 };
 
 // Parameter list keyed by the reference's parameter names.  Values are
@@ -249,6 +252,10 @@ class Schema {
   // k-nearest form is not supported).
   void set_vsg_strain_window(int window_size_in_pixels);
   int strain_window_size() const { return vsg_window_; }
+  // The post_process_uncertainty block (Uncertainty_Post_Processor,
+  // DICe_PostProcessor.cpp:1494-1537): UNCERTAINTY_FS and UNCERTAINTY_ANGLE_FS
+  // after every execute_correlation.
+  void set_uncertainty_post_processor(bool on) { uncertainty_pp_ = on; }
   // sort_txt_output (DICe_Schema.cpp:4446-4466): write_output's one-file-per-frame
   // rows in order of x, then y
   void set_sort_txt_output(bool sort) { sort_txt_output_ = sort; }
@@ -343,6 +350,8 @@ class Schema {
   int vsg_window_ = 0;
   bool sort_txt_output_ = false;
   std::vector<double> vsg_;  // DICE_VSG_COUNT x n
+  bool uncertainty_pp_ = false;
+  std::vector<double> unc_;  // 2 x n: UNCERTAINTY, UNCERTAINTY_ANGLE
 };
 
 // Output_Spec (DICe_Schema.cpp:4673-4945, text output of a local analysis):

@@ -179,6 +179,15 @@ enum dice_vsg_field {
 int dice_hip_vsg_strain(dice_hip_context *ctx, int window_size, double *fields, double *vsg);
 int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *device_fields, double *device_vsg);
 
+/* Uncertainty_Post_Processor::execute (DICe_PostProcessor.cpp:1494-1537), 2-D
+ * analysis: out[i] = UNCERTAINTY = SIGMA / FIELD_1 (FIELD_1 is the cosine of the
+ * angle of computeUncertaintyFields; 0 when it is 0 or when SIGMA < 0) and
+ * out[n + i] = UNCERTAINTY_ANGLE = FIELD_1, as scalar_t (float) values.
+ * dice_hip_uncertainty takes host arrays and is synchronous; the _device form is
+ * enqueued on the context stream. */
+int dice_hip_uncertainty(dice_hip_context *ctx, const double *fields, double *out);
+int dice_hip_uncertainty_device(dice_hip_context *ctx, const double *device_fields, double *device_out);
+
 /* Correlation rounds the last neighbour-initialized frame took (0 after a
  * USE_FIELD_VALUES frame).  No counterpart in the reference, which correlates
  * the subsets in order, once each: a diagnostic of this library's rounds. */

@@ -1202,6 +1202,18 @@ int oracle_vsg_strain(int n, double *F, int window, double *vsg) {
   return 0;
 }
 
+int oracle_uncertainty(int n, const double *F, double *out) {
+  // Uncertainty_Post_Processor::execute, DICe_PostProcessor.cpp:1507-1537 (2-D: max_m = 0).
+  if (n < 1 || !F || !out) return -1;
+  for (int i = 0; i < n; ++i) {
+    const float angle = (float)F[(size_t)OF_FIELD_1 * n + i];
+    const float sig = (float)F[(size_t)OF_SIGMA * n + i];
+    out[(size_t)n + i] = angle;
+    out[i] = sig < 0.0f ? 0.0f : (angle == 0.0f ? 0.0f : (float)(1.0 / (double)angle * (double)sig));
+  }
+  return 0;
+}
+
 int oracle_lu_inverse(int n, double *a) {
   if (n < 1 || n > 6) return -1;
   double A[6][6];

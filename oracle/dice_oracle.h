@@ -132,6 +132,10 @@ int oracle_gauss_filter_d(int w, int h, double *img, int mask_size);
  * du/dy, dv/dx, dv/dy).  window > 0 (radius window / 2).  Returns 0 or -1. */
 int oracle_vsg_strain(int n, double *fields, int window, double *vsg);
 
+/* Uncertainty_Post_Processor (2-D): out[i] = SIGMA / FIELD_1 (0 if FIELD_1 == 0 or SIGMA < 0),
+ * out[n + i] = FIELD_1, in float.  Returns 0 or -1. */
+int oracle_uncertainty(int n, const double *fields, double *out);
+
 /* 6x6 LU with partial pivoting + inverse (GETRF/GETRI semantics); returns INFO. */
 int oracle_lu_inverse(int n, double *a_rowmajor);
 

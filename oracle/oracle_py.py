@@ -85,6 +85,7 @@ def lib():
         _lib.oracle_gauss_filter_f.argtypes = [i, i, i, P, i]
         _lib.oracle_gauss_filter_d.argtypes = [i, i, P, i]
         _lib.oracle_vsg_strain.argtypes = [i, P, i, P]
+        _lib.oracle_uncertainty.argtypes = [i, P, P]
     return _lib
 
 
@@ -240,6 +241,16 @@ def vsg_strain(fields, window):
     out = np.zeros((7, n))
     if lib().oracle_vsg_strain(n, _p(fields), window, _p(out)) != 0:
         raise ValueError("vsg_strain: bad arguments")
+    return out
+
+
+def uncertainty(fields):
+    """Uncertainty_Post_Processor (2-D): (2, n) array of UNCERTAINTY and UNCERTAINTY_ANGLE."""
+    assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape[0] == NUM_FIELDS
+    n = fields.shape[1]
+    out = np.zeros((2, n))
+    if lib().oracle_uncertainty(n, _p(fields), _p(out)) != 0:
+        raise ValueError("uncertainty: bad arguments")
     return out
 
 

@@ -74,3 +74,20 @@ def test_vsg_errors(ctx):
     for bad in (0, -8):
         with pytest.raises(dh.DiceHipError):
             ctx.vsg_strain(f, bad)
+
+
+def test_uncertainty_post_processor_parity(ctx):
+    """dice_hip_uncertainty against the oracle, bit for bit: UNCERTAINTY = SIGMA / FIELD_1 (0 for failed
+    subsets and for FIELD_1 = 0), UNCERTAINTY_ANGLE = FIELD_1."""
+    rng = np.random.default_rng(11)
+    cx, cy = dh.regular_grid(512, 512, 21, 8)
+    ctx.set_subsets(cx, cy, 21)
+    f = dh.new_fields(cx, cy)
+    f[dh.F["SIGMA"]] = rng.uniform(0.001, 0.05, len(cx))
+    f[dh.F["SIGMA"], ::13] = -1.0
+    f[dh.F["FIELD_1"]] = rng.uniform(0.2, 1.0, len(cx)).astype(np.float32)
+    f[dh.F["FIELD_1"], ::17] = 0.0
+    ug = ctx.uncertainty(f)
+    uo = o.uncertainty(f)
+    assert np.array_equal(ug, uo)
+    assert np.all(ug[0, ::13] == 0) and np.all(ug[0, ::17] == 0)

@@ -225,6 +225,18 @@ def test_oracle_vsg_closed_forms():
     assert np.all(o.vsg_strain(h, 41) == 0) and np.all(h[o.F["MATCH"]] == -1)
 
 
+def test_oracle_uncertainty_post_processor():
+    """Uncertainty_Post_Processor (2-D): sigma over the cosine in FIELD_1, rounded as the reference's
+    1.0 / angle * sigma (double literal, stored in float); zero for failed subsets and a zero cosine."""
+    f = o.new_fields(np.arange(4.0), np.zeros(4))
+    f[o.F["SIGMA"]] = [0.02, 0.013, -1.0, 0.5]
+    f[o.F["FIELD_1"]] = np.array([0.5, 0.7, 0.9, 0.0], np.float32)
+    u = o.uncertainty(f)
+    want = np.float32(1.0 / np.float64(np.float32(0.7)) * np.float64(np.float32(0.013)))
+    assert u[0, 0] == np.float32(0.04) and u[0, 1] == want and u[0, 2] == 0 and u[0, 3] == 0
+    assert np.array_equal(u[1], np.array([0.5, 0.7, 0.9, 0.0], np.float32))
+
+
 def test_serpentine_grid_helper_matches_cpp():
     import dice_amd as dh
     for w, h, s, step in ((512, 512, 27, 75), (300, 211, 21, 17), (128, 640, 31, 9)):
