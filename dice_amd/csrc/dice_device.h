@@ -2443,6 +2443,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   Solver<T, WaveGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                           lds_per_wave);
   const int total = items.total(ss.n);
+  // more waves than items (a short list): the first `total` waves take every item between them, and
+  // the others leave without touching the counters
+  if ((int)blockIdx.x * (int)(blockDim.x / kWave) + wave >= total) return;
   int range = (int)(blockIdx.x % kWorkRanges);
 #pragma unroll 1
   for (int visited = 0; visited < kWorkRanges;) {
@@ -2485,6 +2488,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                         lds_per_wave);
   const int total = items.total(ss.n);
+  if ((int)blockIdx.x * (int)(blockDim.x / kWave) + wave >= total) return;  // as in correlate_batched_kernel
   int range = (int)(blockIdx.x % kWorkRanges);
 #pragma unroll 1
   for (int visited = 0; visited < kWorkRanges;) {
