@@ -42,6 +42,8 @@ CONVERGED_STATUSES = (1, 2, 3, 4)
 BILINEAR, BICUBIC, KEYS_FOURTH = 0, 1, 2
 SIMPLEX, GRADIENT_BASED, GRADIENT_THEN_SEARCH, SIMPLEX_THEN_GRADIENT_BASED, GRADIENT_BASED_THEN_SIMPLEX = 0, 1, 2, 3, 4
 USE_FIELD_VALUES, USE_NEIGHBOR_VALUES, USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY = 0, 1, 2
+# enum dice_objective_call
+OBJ_GAMMA, OBJ_SIGMA, OBJ_UPDATE_FAST, OBJ_UPDATE_ROBUST = 0, 1, 2, 3
 
 
 class Params(ctypes.Structure):
@@ -88,7 +90,7 @@ EXPORTS = [
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
     "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
-    "dice_hip_uncertainty", "dice_hip_uncertainty_device",
+    "dice_hip_uncertainty", "dice_hip_uncertainty_device", "dice_hip_objective",
 ]
 
 _lib = None
@@ -151,6 +153,7 @@ def lib():
         L.dice_hip_vsg_strain_device.argtypes = [P, I, P, P]
         L.dice_hip_uncertainty.argtypes = [P, P, P]
         L.dice_hip_uncertainty_device.argtypes = [P, P, P]
+        L.dice_hip_objective.argtypes = [P, I, I, P, P]
         _lib = L
     return _lib
 
@@ -308,6 +311,14 @@ class Context:
         out = np.zeros((2, self.n))
         self._check(self._lib.dice_hip_uncertainty(self.h, _p(fields), _p(out)), "uncertainty")
         return out
+
+    def objective(self, subset_id, call, params):
+        """dice_hip_objective: one Objective_ZNSSD call (OBJ_GAMMA, OBJ_SIGMA, OBJ_UPDATE_FAST,
+        OBJ_UPDATE_ROBUST) on subset_id; returns (params after the call, result[10])."""
+        p = np.ascontiguousarray(params, dtype=np.float64).copy()
+        r = np.zeros(10)
+        self._check(self._lib.dice_hip_objective(self.h, subset_id, call, _p(p), _p(r)), "objective")
+        return p, r
 
     def neighbor_rounds(self):
         """Correlation rounds of the last frame with neighbour initialization (0 otherwise)."""

@@ -84,6 +84,7 @@ struct dice_hip_context {
                                                      // subsets to correlate again, their count per round
   int nb_rounds = 0;                         // correlation rounds of the last neighbour-initialized frame
   DevBuf vsg_cells, vsg_ids, vsg_out;        // VSG strain: subset grid, host-call output
+  DevBuf obj_io;                             // dice_hip_objective: parameters and results
   bool def_has_grads = false;
   // subsets
   int n = 0, S = 0, fov = 0;
@@ -340,7 +341,7 @@ int dice_hip_destroy(dice_hip_context *ctx) {
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
                     &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
                     &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids,
-                    &ctx->vsg_out, &ctx->fields, &ctx->work})
+                    &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -712,6 +713,43 @@ int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *d
   if (!device_fields || !device_vsg) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
   (void)hipSetDevice(ctx->device);
   return vsg_enqueue(ctx, window_size, device_fields, device_vsg);
+}
+
+int dice_hip_objective(dice_hip_context *ctx, int subset_id, int call, double *params, double *result) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!params || !result) return ctx->fail(DICE_HIP_EINVAL, "null pointer");
+  if (call < DICE_OBJ_GAMMA || call > DICE_OBJ_UPDATE_ROBUST) return ctx->fail(DICE_HIP_EINVAL, "unknown call");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (subset_id < 0 || subset_id >= ctx->n) return ctx->fail(DICE_HIP_EINVAL, "subset id out of range");
+  (void)hipSetDevice(ctx->device);
+  DImage ref, def;
+  DSubsets ss;
+  fill_views(ctx, ref, def, ss);
+  const KParams kp = make_kparams(ctx->params, 0);
+  HIP_TRY(ctx, ctx->obj_io.ensure(8 * sizeof(double)), "hipMalloc objective");
+  double io[8] = {params[0], params[1], params[2], params[3], params[4], params[5], 0.0, 0.0};
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->obj_io.p, io, sizeof io, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy objective");
+  HIP_TRY(ctx, dice_hip::launch_objective(ctx->rstorage, kp, ref, def, ss, ctx->fields.as<double>(), subset_id, call,
+                                          ctx->obj_io.as<double>(), ctx->stream),
+          "objective kernel");
+  HIP_TRY(ctx, hipMemcpyAsync(io, ctx->obj_io.p, sizeof io, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy objective");
+  double extra[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // CONDITION_NUMBER, FIELD_1..7 of the subset
+  if (call == DICE_OBJ_UPDATE_FAST) {
+    const int slots[8] = {DICE_F_CONDITION, DICE_F_FIELD_1, DICE_F_FIELD_2, DICE_F_FIELD_3,
+                          DICE_F_FIELD_4,   DICE_F_FIELD_5, DICE_F_FIELD_6, DICE_F_FIELD_7};
+    for (int k = 0; k < 8; ++k)
+      HIP_TRY(ctx, hipMemcpyAsync(&extra[k], ctx->fields.as<double>() + (size_t)slots[k] * ctx->n + subset_id,
+                                  sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpy objective fields");
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "objective");
+  if (call >= DICE_OBJ_UPDATE_FAST)
+    for (int k = 0; k < 6; ++k) params[k] = io[k];
+  result[0] = io[6];
+  result[1] = io[7];
+  for (int k = 0; k < 8; ++k) result[2 + k] = extra[k];
+  return DICE_HIP_OK;
 }
 
 int dice_hip_uncertainty_device(dice_hip_context *ctx, const double *device_fields, double *device_out) {

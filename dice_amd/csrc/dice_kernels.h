@@ -185,6 +185,13 @@ constexpr int kVsgFields = 7;
 // or SIGMA < 0), out[1][i] = UNCERTAINTY_ANGLE = FIELD_1, in scalar_t (float).
 hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStream_t stream);
 
+// One Objective_ZNSSD call for subset gid (call 0 gamma, 1 sigma, 2
+// computeUpdateFast, 3 computeUpdateRobust) on one workgroup; io: 8 device
+// doubles (parameters in / out, then two results).  computeUpdateFast writes the
+// subset's CONDITION_NUMBER and uncertainty fields into fields, as the reference.
+hipError_t launch_objective(int storage, const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &ss,
+                            double *fields, int gid, int call, double *io, hipStream_t stream);
+
 // Enqueue the correlation of all subsets, or of the device list items (count
 // *item_count) when items is not null.
 hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, const DImage &ref,

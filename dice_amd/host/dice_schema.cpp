@@ -717,6 +717,78 @@ void Schema::set_def_image(int width, int height, const float *intensities) {
 void Schema::set_def_image(int width, int height, const uint16_t *intensities) {
   set_image(false, width, height, intensities, DICE_STORAGE_U16);
 }
+Affine_Shape_Function::Affine_Shape_Function(bool enable_rotation, bool enable_normal_strain,
+                                             bool enable_shear_strain) {
+  en_[0] = en_[1] = true;
+  en_[2] = enable_rotation;
+  en_[3] = en_[4] = enable_normal_strain;
+  en_[5] = enable_shear_strain;
+}
+
+void Affine_Shape_Function::set(int index, double value) {
+  if (index < 0 || index > 5) throw std::invalid_argument("Error, shape function parameter index out of range");
+  p_[index] = en_[index] ? value : 0.0;
+}
+
+void Affine_Shape_Function::initialize_parameters_from_fields(const Schema *schema, int subset_gid) {
+  const Field_Spec fs[6] = {SUBSET_DISPLACEMENT_X_FS, SUBSET_DISPLACEMENT_Y_FS, ROTATION_Z_FS,
+                            NORMAL_STRETCH_XX_FS,     NORMAL_STRETCH_YY_FS,     SHEAR_STRETCH_XY_FS};
+  for (int k = 0; k < 6; ++k) set(k, schema->local_field_value(subset_gid, fs[k]));
+}
+
+void Affine_Shape_Function::save_fields(Schema *schema, int subset_gid) const {
+  const Field_Spec fs[6] = {SUBSET_DISPLACEMENT_X_FS, SUBSET_DISPLACEMENT_Y_FS, ROTATION_Z_FS,
+                            NORMAL_STRETCH_XX_FS,     NORMAL_STRETCH_YY_FS,     SHEAR_STRETCH_XY_FS};
+  for (int k = 0; k < 6; ++k)
+    if (en_[k]) schema->local_field_value(subset_gid, fs[k]) = p_[k];
+}
+
+Objective_ZNSSD::Objective_ZNSSD(Schema *schema, int correlation_point_global_id)
+    : schema_(schema), gid_(correlation_point_global_id) {
+  if (!schema) throw std::invalid_argument("Error, null schema");
+  if (gid_ < 0 || gid_ >= schema->local_num_subsets()) throw std::runtime_error("Error, subset id out of range");
+}
+
+double Objective_ZNSSD::call(int which, const Affine_Shape_Function &sf, double *params, double *result) const {
+  for (int k = 0; k < 6; ++k) params[k] = sf.p_[k];
+  const int rc = dice_hip_objective(schema_->context(), gid_, which, params, result);
+  if (rc != DICE_HIP_OK)
+    throw std::runtime_error(std::string("Error, objective call failed: ") + dice_hip_last_error(schema_->context()));
+  return result[0];
+}
+
+Status_Flag Objective_ZNSSD::computeUpdateFast(Affine_Shape_Function &sf, int &num_iterations) {
+  double params[6], result[10];
+  const double status = call(DICE_OBJ_UPDATE_FAST, sf, params, result);
+  for (int k = 0; k < 6; ++k) sf.set(k, params[k]);
+  num_iterations = (int)result[1];
+  schema_->local_field_value(gid_, CONDITION_NUMBER_FS) = result[2];
+  const Field_Spec unc[7] = {FIELD_1_FS, FIELD_2_FS, FIELD_3_FS, FIELD_4_FS, FIELD_5_FS, FIELD_6_FS, FIELD_7_FS};
+  if (status == CORRELATION_SUCCESSFUL)
+    for (int k = 0; k < 7; ++k) schema_->local_field_value(gid_, unc[k]) = result[3 + k];
+  return (Status_Flag)(int)status;
+}
+
+Status_Flag Objective_ZNSSD::computeUpdateRobust(Affine_Shape_Function &sf, int &num_iterations) {
+  double params[6], result[10];
+  const double status = call(DICE_OBJ_UPDATE_ROBUST, sf, params, result);
+  for (int k = 0; k < 6; ++k) sf.set(k, params[k]);
+  num_iterations = (int)result[1];
+  return (Status_Flag)(int)status;
+}
+
+double Objective_ZNSSD::gamma(const Affine_Shape_Function &sf) const {
+  double params[6], result[10];
+  return call(DICE_OBJ_GAMMA, sf, params, result);
+}
+
+double Objective_ZNSSD::sigma(const Affine_Shape_Function &sf, double &noise_level) const {
+  double params[6], result[10];
+  const double s = call(DICE_OBJ_SIGMA, sf, params, result);
+  noise_level = result[1];
+  return s;
+}
+
 void Schema::set_neighbor_ids(const std::vector<int> &neighbor_ids) {
   if ((int)neighbor_ids.size() != n_) throw std::invalid_argument("Error, one neighbor id per subset is needed");
   for (int i = 0; i < n_; ++i) {

@@ -354,6 +354,63 @@ class Schema {
   std::vector<double> unc_;  // 2 x n: UNCERTAINTY, UNCERTAINTY_ANGLE
 };
 
+// Status_Flag values of DICe.h:609-677 that the correlation records.
+enum Status_Flag {
+  CORRELATION_SUCCESSFUL = 0,
+  INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL = 1,
+  INITIALIZE_USING_NEIGHBOR_VALUE_SUCCESSFUL = 3,
+  INITIALIZE_SUCCESSFUL = 4,
+  INITIALIZE_FAILED = 5,
+  CORRELATION_FAILED = 8,
+  MAX_ITERATIONS_REACHED = 11,
+  INITIALIZE_FAILED_BY_EXCEPTION = 12,
+  JUMP_TOLERANCE_EXCEEDED = 16
+};
+
+// Affine_Shape_Function (DICe_LocalShapeFunction.h:266-398): the six parameters
+// u, v, theta, ex, ey, gxy, with the disabled ones held at 0.
+class Affine_Shape_Function {
+ public:
+  Affine_Shape_Function(bool enable_rotation = true, bool enable_normal_strain = true,
+                        bool enable_shear_strain = true);
+  double u() const { return p_[0]; }
+  double v() const { return p_[1]; }
+  double theta() const { return p_[2]; }
+  double ex() const { return p_[3]; }
+  double ey() const { return p_[4]; }
+  double gxy() const { return p_[5]; }
+  // sets a parameter (index 0..5 in the order above); a disabled one stays 0
+  void set(int index, double value);
+  const double *parameters() const { return p_; }
+  // initialize_parameters_from_fields / save_fields (DICe_LocalShapeFunction.cpp:251-312, 65-74)
+  void initialize_parameters_from_fields(const Schema *schema, int subset_gid);
+  void save_fields(Schema *schema, int subset_gid) const;
+
+ private:
+  friend class Objective_ZNSSD;
+  double p_[6] = {0, 0, 0, 0, 0, 0};
+  bool en_[6];
+};
+
+// Objective_ZNSSD (DICe_Objective.h:82-242) for one subset of a Schema whose
+// images and parameters are set: the calls run on the GPU, one workgroup each
+// (dice_hip_objective).  computeUpdateFast stores the subset's CONDITION_NUMBER and
+// uncertainty fields (FIELD_1..7) in the Schema, as the reference does.
+class Objective_ZNSSD {
+ public:
+  Objective_ZNSSD(Schema *schema, int correlation_point_global_id);
+  int correlation_point_global_id() const { return gid_; }
+  Status_Flag computeUpdateFast(Affine_Shape_Function &shape_function, int &num_iterations);
+  Status_Flag computeUpdateRobust(Affine_Shape_Function &shape_function, int &num_iterations);
+  double gamma(const Affine_Shape_Function &shape_function) const;
+  double sigma(const Affine_Shape_Function &shape_function, double &noise_level) const;
+
+ private:
+  double call(int which, const Affine_Shape_Function &sf, double *params, double *result) const;
+  Schema *schema_;
+  int gid_;
+};
+
 // Output_Spec (DICe_Schema.cpp:4673-4945, text output of a local analysis):
 // the header block, the field-name row and the %4.4E rows.
 class Output_Spec {

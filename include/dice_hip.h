@@ -179,6 +179,23 @@ enum dice_vsg_field {
 int dice_hip_vsg_strain(dice_hip_context *ctx, int window_size, double *fields, double *vsg);
 int dice_hip_vsg_strain_device(dice_hip_context *ctx, int window_size, double *device_fields, double *device_vsg);
 
+/* Objective_ZNSSD(schema, subset_id) calls for one subset (DICe_Objective.h:82-242) on
+ * the context's images and subsets:
+ *   DICE_OBJ_GAMMA          Objective::gamma(sf) (DICe_Objective.cpp:58-76): result[0]
+ *   DICE_OBJ_SIGMA          Objective::sigma(sf, noise) (:150-190): result[0] sigma,
+ *                           result[1] noise
+ *   DICE_OBJ_UPDATE_FAST    computeUpdateFast(sf, iters) (:430-657): params updated,
+ *                           result[0] Status_Flag, result[1] iterations, result[2]
+ *                           CONDITION_NUMBER and result[3..9] FIELD_1..7 of the subset
+ *                           (the fields it writes in the reference)
+ *   DICE_OBJ_UPDATE_ROBUST  computeUpdateRobust(sf, iters) (:369-388): params updated,
+ *                           result[0] Status_Flag, result[1] iterations
+ * params: 6 doubles (u, v, theta, ex, ey, gxy; disabled ones read as 0).  result: 10
+ * doubles.  Synchronous; one workgroup, for callers that drive subsets one by one as
+ * the reference's routines do (dice_hip_correlate runs whole frames). */
+enum dice_objective_call { DICE_OBJ_GAMMA = 0, DICE_OBJ_SIGMA, DICE_OBJ_UPDATE_FAST, DICE_OBJ_UPDATE_ROBUST };
+int dice_hip_objective(dice_hip_context *ctx, int subset_id, int call, double *params, double *result);
+
 /* Uncertainty_Post_Processor::execute (DICe_PostProcessor.cpp:1494-1537), 2-D
  * analysis: out[i] = UNCERTAINTY = SIGMA / FIELD_1 (FIELD_1 is the cosine of the
  * angle of computeUncertaintyFields; 0 when it is 0 or when SIGMA < 0) and

@@ -942,6 +942,68 @@ int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def
   return 0;
 }
 
+// One Objective_ZNSSD call on one square subset (DICe_Objective.h:82-242): 0 gamma, 1 sigma,
+// 2 computeUpdateFast, 3 computeUpdateRobust.  params: 6 (u, v, theta, ex, ey, gxy; disabled
+// ones ignored), updated by the updates; result: 10 (value | sigma | status, noise | iterations,
+// then CONDITION_NUMBER and FIELD_1..7 after computeUpdateFast).
+template <typename S, typename T>
+int objective_t(const oracle_params *P, int w, int h, const T *ref, const T *def, int subset_size, int cx, int cy,
+                int call, double *params, double *result) {
+  if (!P || !ref || !def || !params || !result || subset_size < 1 || call < 0 || call > 3) return -1;
+  Image<S, T> refi, defi;
+  refi.w = defi.w = w;
+  refi.h = defi.h = h;
+  refi.I = ref;
+  defi.I = def;
+  std::vector<T> ref_f, def_f;
+  if (P->gauss_filter_images) {
+    ref_f.assign(ref, ref + (size_t)w * h);
+    def_f.assign(def, def + (size_t)w * h);
+    if (gauss_filter_t<S, T>(P->gauss_filter_mask_size, w, h, ref_f.data()) ||
+        gauss_filter_t<S, T>(P->gauss_filter_mask_size, w, h, def_f.data()))
+      return -1;
+    refi.I = ref_f.data();
+    defi.I = def_f.data();
+  }
+  refi.compute_gradients_fd();
+  if (P->gradient_method == 1) refi.smooth_gradients();
+  if (P->compute_def_gradients) {
+    defi.compute_gradients_fd();
+    if (P->gradient_method == 1) defi.smooth_gradients();
+  }
+  Subset<S> sub;
+  sub.make_square(cx, cy, subset_size);
+  sub.init_ref(refi);
+  double F[OF_COUNT] = {0};
+  F[OF_COORD_X] = cx;
+  F[OF_COORD_Y] = cy;
+  Objective<S, T> obj{P, &refi, &defi, &sub, F, 1, 0};
+  Affine<S> sf;
+  sf.init(P->enable_rotation != 0, P->enable_normal_strain != 0, P->enable_shear_strain != 0,
+          (S)P->robust_delta_disp, (S)P->robust_delta_theta);
+  const int idx[6] = {sf.iu, sf.iv, sf.it, sf.iex, sf.iey, sf.ig};
+  for (int k = 0; k < 6; ++k)
+    if (idx[k] >= 0) sf.p[idx[k]] = (S)params[k];
+  for (int k = 0; k < 10; ++k) result[k] = 0.0;
+  if (call == 0) {
+    result[0] = obj.gamma(sf);
+  } else if (call == 1) {
+    S noise = 0.0;
+    result[0] = obj.sigma(sf, noise);
+    result[1] = noise;
+  } else {
+    int iters = -1;
+    result[0] = call == 2 ? obj.update_fast(sf, iters) : obj.update_robust(sf, iters);
+    result[1] = iters;
+    for (int k = 0; k < 6; ++k) params[k] = idx[k] >= 0 ? (double)sf.p[idx[k]] : 0.0;
+    if (call == 2) {
+      result[2] = F[OF_CONDITION];
+      for (int k = 0; k < 7; ++k) result[3 + k] = F[OF_FIELD_1 + k];
+    }
+  }
+  return 0;
+}
+
 // storage: 0 = float32 pixels, 1 = uint16 pixels, 2 = float64 pixels.  With
 // S = double, float32 pixels are widened to double storage (the
 // DICE_USE_DOUBLE build stores intensities as scalar_t).
@@ -1200,6 +1262,27 @@ int oracle_vsg_strain(int n, double *F, int window, double *vsg) {
     if (!ok) f(OF_MATCH, i) = -1.0;
   }
   return 0;
+}
+
+int oracle_objective_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int subset_size, int cx, int cy, int call, double *params, double *result) {
+  if (storage == 0)
+    return objective_t<float, float>(p, w, h, (const float *)ref, (const float *)def, subset_size, cx, cy, call,
+                                     params, result);
+  if (storage == 1)
+    return objective_t<float, uint16_t>(p, w, h, (const uint16_t *)ref, (const uint16_t *)def, subset_size, cx, cy,
+                                        call, params, result);
+  return -1;
+}
+int oracle_objective_d(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int subset_size, int cx, int cy, int call, double *params, double *result) {
+  if (storage == 2)
+    return objective_t<double, double>(p, w, h, (const double *)ref, (const double *)def, subset_size, cx, cy, call,
+                                       params, result);
+  if (storage == 1)
+    return objective_t<double, uint16_t>(p, w, h, (const uint16_t *)ref, (const uint16_t *)def, subset_size, cx, cy,
+                                         call, params, result);
+  return -1;
 }
 
 int oracle_uncertainty(int n, const double *F, double *out) {

@@ -132,6 +132,17 @@ int oracle_gauss_filter_d(int w, int h, double *img, int mask_size);
  * du/dy, dv/dx, dv/dy).  window > 0 (radius window / 2).  Returns 0 or -1. */
 int oracle_vsg_strain(int n, double *fields, int window, double *vsg);
 
+/* One Objective_ZNSSD call on the square subset of side subset_size centred at (cx, cy):
+ * call 0 gamma, 1 sigma, 2 computeUpdateFast, 3 computeUpdateRobust.  params: 6 doubles
+ * (u, v, theta, ex, ey, gxy), updated by the updates.  result: 10 doubles (gamma | sigma |
+ * status, noise | iterations, then CONDITION_NUMBER and FIELD_1..7 after computeUpdateFast).
+ * _f: float arithmetic on float32 (storage 0) or uint16 (1) pixels; _d: double arithmetic on
+ * float64 (storage 2) or uint16 pixels.  Returns 0 or -1. */
+int oracle_objective_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int subset_size, int cx, int cy, int call, double *params, double *result);
+int oracle_objective_d(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                       int subset_size, int cx, int cy, int call, double *params, double *result);
+
 /* Uncertainty_Post_Processor (2-D): out[i] = SIGMA / FIELD_1 (0 if FIELD_1 == 0 or SIGMA < 0),
  * out[n + i] = FIELD_1, in float.  Returns 0 or -1. */
 int oracle_uncertainty(int n, const double *fields, double *out);

@@ -86,6 +86,8 @@ def lib():
         _lib.oracle_gauss_filter_d.argtypes = [i, i, P, i]
         _lib.oracle_vsg_strain.argtypes = [i, P, i, P]
         _lib.oracle_uncertainty.argtypes = [i, P, P]
+        _lib.oracle_objective_f.argtypes = [P, i, i, i, P, P, i, i, i, i, P, P]
+        _lib.oracle_objective_d.argtypes = [P, i, i, i, P, P, i, i, i, i, P, P]
     return _lib
 
 
@@ -242,6 +244,25 @@ def vsg_strain(fields, window):
     if lib().oracle_vsg_strain(n, _p(fields), window, _p(out)) != 0:
         raise ValueError("vsg_strain: bad arguments")
     return out
+
+
+def objective(params, ref, defm, subset_size, cx, cy, call, p, precision="double"):
+    """One Objective_ZNSSD call (0 gamma, 1 sigma, 2 computeUpdateFast, 3 computeUpdateRobust) on the square
+    subset at (cx, cy); returns (parameters after the call, result[10])."""
+    h, w = ref.shape
+    if precision == "double":
+        ref, defm = [np.ascontiguousarray(a, dtype=np.float64) if a.dtype != np.uint16 else a for a in (ref, defm)]
+        storage = 1 if ref.dtype == np.uint16 else 2
+        fn = lib().oracle_objective_d
+    else:
+        ref, defm = [np.ascontiguousarray(a, dtype=np.float32) if a.dtype != np.uint16 else a for a in (ref, defm)]
+        storage = 1 if ref.dtype == np.uint16 else 0
+        fn = lib().oracle_objective_f
+    q = np.ascontiguousarray(p, dtype=np.float64).copy()
+    r = np.zeros(10)
+    if fn(ctypes.byref(params), w, h, storage, _p(ref), _p(defm), subset_size, int(cx), int(cy), call, _p(q), _p(r)):
+        raise ValueError("objective: bad arguments")
+    return q, r
 
 
 def uncertainty(fields):

@@ -11,6 +11,8 @@
 //   test_schema grid <w> <h> <subset size> <step> [subset file]   (prints "x y neighbor" of the
 //                                  regular grid, then "seed id u v" for each seed)
 //   test_schema errors             (error behaviour that needs no GPU)
+//   test_schema objective <ref.f32> <def.f32> <w> <h> <S> <cx> <cy> <u> <v>   (Objective_ZNSSD calls on
+//                                  one subset, affine: gamma, sigma, computeUpdateFast, computeUpdateRobust)
 //   test_schema read_image <image file> <out.rawi> [ox oy w h]   (read_image, then write_rawi_image)
 //   test_schema tiff_samples <tiff file> <out>   (uint32 w, h, bits, then the uint16 samples)
 //
@@ -335,6 +337,36 @@ int main(int argc, char **argv) {
     if (cmd == "errors") return errors_without_gpu();
     if (cmd == "subset_file_info" && argc >= 3) return subset_file_info(argv[2]);
     if (cmd == "deck" && argc >= 3) return deck(argv[2]);
+    if (cmd == "objective" && argc >= 11) {
+      const int w = std::atoi(argv[4]), h = std::atoi(argv[5]), S = std::atoi(argv[6]);
+      const std::vector<float> ref = read_raw(argv[2], w, h), def = read_raw(argv[3], w, h);
+      ParameterList params;
+      dice_default_params(&params);
+      params.set("enable_rotation", 1);
+      params.set("enable_normal_strain", 1);
+      params.set("enable_shear_strain", 1);
+      Schema schema({std::atof(argv[7])}, {std::atof(argv[8])}, S, nullptr, &params);
+      schema.set_ref_image(w, h, ref.data());
+      schema.set_def_image(w, h, def.data());
+      Objective_ZNSSD obj(&schema, 0);
+      Affine_Shape_Function sf(true, true, true);
+      sf.set(0, std::atof(argv[9]));
+      sf.set(1, std::atof(argv[10]));
+      double noise = 0.0;
+      std::printf("gamma %.9g\n", obj.gamma(sf));
+      const double sig = obj.sigma(sf, noise);
+      std::printf("sigma %.9g %.9g\n", sig, noise);
+      Affine_Shape_Function fast = sf, robust = sf;
+      int it_fast = 0, it_robust = 0;
+      const int st_fast = obj.computeUpdateFast(fast, it_fast);
+      const int st_robust = obj.computeUpdateRobust(robust, it_robust);
+      std::printf("fast %d %d", st_fast, it_fast);
+      for (int k = 0; k < 6; ++k) std::printf(" %.9g", fast.parameters()[k]);
+      std::printf(" %.9g\nrobust %d %d", schema.local_field_value(0, CONDITION_NUMBER_FS), st_robust, it_robust);
+      for (int k = 0; k < 6; ++k) std::printf(" %.9g", robust.parameters()[k]);
+      std::printf("\n");
+      return 0;
+    }
     if (cmd == "read_image" && argc >= 4) {
       std::vector<float> pix;
       int w = 0, h = 0;
