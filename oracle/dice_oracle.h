@@ -29,8 +29,17 @@
  *   statuses ............ flat subsets, field of view, failed initial guess, max iterations + fallback
  *   polygon_pixels ...... conformal(_d).rawi golden raster (DICe_TestSubset.cpp:248-309)
  *   lu_inverse .......... numpy inverse; exactly singular input leaves the LU factors
- * parity unpinned: the uncertainty fields FIELD_1..7 (computeUncertaintyFields), Nelder-Mead
- * iteration counts and which stop rule fired, and sigma/noise beyond the seven gold subsets; they
+ * and in the other test files:
+ *   gauss_filter ........ the ten filtered-image goldens outFilter{5..13}(_d).rawi bit for bit
+ *                         (tests/test_image_io.py)
+ *   correlate (neighbour  parallel_subsets_seed and parallel_seed_first_step_only golds
+ *   initialization) ..... (tests/test_host_cpp.py)
+ *   vsg_strain .......... dic_challenge_12_sorted gold (VSG_STRAIN_XX/YY), closed forms on linear fields
+ *                         (tests/test_host_cpp.py)
+ *   objective, FIELD_1,   bitmap_star gold: u, v, sigma, gamma and UNCERTAINTY = sigma / FIELD_1
+ *   uncertainty ......... (tests/test_uncertainty_gold.py)
+ * parity unpinned: the uncertainty fields FIELD_2..7 (computeUncertaintyFields), Nelder-Mead
+ * iteration counts and which stop rule fired, and the noise level beyond what sigma pins; they
  * follow the cited lines by review only.
  */
 #ifndef DICE_ORACLE_H
