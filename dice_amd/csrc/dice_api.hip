@@ -658,9 +658,10 @@ int dice_hip_neighbor_rounds(dice_hip_context *ctx) { return ctx ? ctx->nb_round
 // max(r, 1) px (sorted by cell, then id), uploaded with the offsets, then one
 // thread per subset on the context stream.
 static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields, double *d_vsg) {
-  if (window_size <= 0)
-    return ctx->fail(DICE_HIP_EINVAL, window_size == 0 ? "strain_window_size_in_pixels must not be 0"
-                                                       : "k-nearest VSG windows (negative sizes) are not supported");
+  if (window_size == 0) return ctx->fail(DICE_HIP_EINVAL, "strain_window_size_in_pixels must not be 0");
+  if (window_size < -dice_hip::kVsgMaxK)
+    return ctx->fail(DICE_HIP_EINVAL, "k-nearest VSG windows are limited to 64 neighbours");
+  const int knn = window_size < 0 ? -window_size : 0;
   if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
   const int n = ctx->n;
   std::vector<double> xy(2 * (size_t)n);
@@ -674,13 +675,14 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
     x0 = std::min(x0, X[i]); x1 = std::max(x1, X[i]);
     y0 = std::min(y0, Y[i]); y1 = std::max(y1, Y[i]);
   }
-  const float radius = (float)window_size / 2.0f;
+  const float radius = knn ? 0.0f : (float)window_size / 2.0f;
   dice_hip::VsgGrid g{};
   g.x0 = x0;
   g.y0 = y0;
-  // cells of side >= r, so that every neighbour lies in the 3 x 3 cells around a subset; at most
-  // 4000 cells along a side
-  g.cell = std::max({(double)radius, 1.0, (x1 - x0) / 4000.0, (y1 - y0) / 4000.0});
+  // radius windows: cells of side >= r, so that every neighbour lies in the 3 x 3 cells around a
+  // subset; k-nearest: cells holding about k subsets; at most 4000 cells along a side
+  const double side = knn ? std::sqrt((x1 - x0 + 1.0) * (y1 - y0 + 1.0) * knn / n) : (double)radius;
+  g.cell = std::max({side, 1.0, (x1 - x0) / 4000.0, (y1 - y0) / 4000.0});
   g.ncx = (int)std::floor((x1 - x0) / g.cell) + 1;
   g.ncy = (int)std::floor((y1 - y0) / g.cell) + 1;
   const size_t ncells = (size_t)g.ncx * g.ncy;
@@ -702,7 +704,7 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
           "hipMemcpy VSG grid");
   g.cell_start = ctx->vsg_cells.as<int>();
   g.cell_ids = ctx->vsg_ids.as<int>();
-  HIP_TRY(ctx, dice_hip::launch_vsg_strain(g, radius, d_fields, d_vsg, n, ctx->stream), "VSG strain kernel");
+  HIP_TRY(ctx, dice_hip::launch_vsg_strain(g, radius, knn, d_fields, d_vsg, n, ctx->stream), "VSG strain kernel");
   // the host vectors must outlive the copies
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG strain");
   return DICE_HIP_OK;

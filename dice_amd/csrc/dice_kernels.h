@@ -177,8 +177,12 @@ struct VsgGrid {
   int ncx, ncy;
   const int *cell_start, *cell_ids;
 };
-hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream);
+// knn > 0 (at most kVsgMaxK): the knn nearest subsets instead (this one included, ties in
+// squared float distance broken by subset id), the grid of any cell side.
+hipError_t launch_vsg_strain(const VsgGrid &g, float radius, int knn, double *fields, double *vsg, int n,
+                             hipStream_t stream);
 constexpr int kVsgFields = 7;
+constexpr int kVsgMaxK = 64;
 
 // Uncertainty_Post_Processor::execute (DICe_PostProcessor.cpp:1507-1537) for a
 // 2-D analysis: out[0][i] = UNCERTAINTY = SIGMA / FIELD_1 (0 when FIELD_1 is 0

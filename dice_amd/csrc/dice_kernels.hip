@@ -247,7 +247,7 @@ __device__ double norm1_3(const double (&a)[3][3]) {
 }
 
 // One subset per thread (see launch_vsg_strain).
-__global__ void vsg_strain_kernel(VsgGrid g, float radius, double *fields, double *vsg, int n) {
+__global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fields, double *vsg, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   auto F = [&](int s, int k) -> double & { return fields[(size_t)s * n + k]; };
@@ -256,27 +256,65 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, double *fields, doubl
   const float r2 = radius * radius + 1.0e-5f;  // the reference's rad^2 + tiny in scalar_t
   double s1 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, su = 0, sxu = 0, syu = 0, sv = 0, sxv = 0, syv = 0;
   int valid = 0;
-  if (F(F_SIGMA, i) >= 0.0) {
-    const int ci = min(max((int)floor((xi - g.x0) / g.cell), 0), g.ncx - 1);
-    const int cj = min(max((int)floor((yi - g.y0) / g.cell), 0), g.ncy - 1);
+  auto add = [&](int j, float ddx, float ddy) {
+    if (!(F(F_SIGMA, j) >= 0.0)) return;
+    // the reference keeps the neighbour offsets and the displacements as scalar_t (float)
+    const double dx = ddx, dy = ddy;
+    const double u = (float)F(F_DISP_X, j), v = (float)F(F_DISP_Y, j);
+    ++valid;
+    s1 += 1.0; sx += dx; sy += dy;
+    sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
+    su += u; sxu += dx * u; syu += dy * u;
+    sv += v; sxv += dx * v; syv += dy * v;
+  };
+  const int ci = min(max((int)floor((xi - g.x0) / g.cell), 0), g.ncx - 1);
+  const int cj = min(max((int)floor((yi - g.y0) / g.cell), 0), g.ncy - 1);
+  if (F(F_SIGMA, i) >= 0.0 && knn == 0) {
     for (int cy = max(cj - 1, 0); cy <= min(cj + 1, g.ncy - 1); ++cy)
       for (int cx = max(ci - 1, 0); cx <= min(ci + 1, g.ncx - 1); ++cx) {
         const int c = cy * g.ncx + cx;
         for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
           const int j = g.cell_ids[k];
           const float ddx = (float)F(F_COORD_X, j) - xf, ddy = (float)F(F_COORD_Y, j) - yf;
-          if (!(ddx * ddx + ddy * ddy < r2)) continue;
-          if (!(F(F_SIGMA, j) >= 0.0)) continue;
-          // the reference keeps the neighbour offsets and the displacements as scalar_t (float)
-          const double dx = ddx, dy = ddy;
-          const double u = (float)F(F_DISP_X, j), v = (float)F(F_DISP_Y, j);
-          ++valid;
-          s1 += 1.0; sx += dx; sy += dy;
-          sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
-          su += u; sxu += dx * u; syu += dy * u;
-          sv += v; sxv += dx * v; syv += dy * v;
+          if (ddx * ddx + ddy * ddy < r2) add(j, ddx, ddy);
         }
       }
+  } else if (F(F_SIGMA, i) >= 0.0) {
+    // the knn nearest subsets (this one included) by squared float distance, then id: rings of
+    // cells around this one until no unvisited cell can hold a closer subset
+    float bd[kVsgMaxK];
+    int bj[kVsgMaxK];
+    int have = 0;
+    const int rings = max(g.ncx, g.ncy);
+    for (int r = 0; r <= rings; ++r) {
+      for (int cy = cj - r; cy <= cj + r; ++cy) {
+        if (cy < 0 || cy >= g.ncy) continue;
+        const int step = (cy == cj - r || cy == cj + r) ? 1 : 2 * r;
+        for (int cx = ci - r; cx <= ci + r; cx += max(step, 1)) {
+          if (cx < 0 || cx >= g.ncx) continue;
+          const int c = cy * g.ncx + cx;
+          for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
+            const int j = g.cell_ids[k];
+            const float ddx = (float)F(F_COORD_X, j) - xf, ddy = (float)F(F_COORD_Y, j) - yf;
+            const float d2 = ddx * ddx + ddy * ddy;
+            if (have == knn && !(d2 < bd[knn - 1] || (d2 == bd[knn - 1] && j < bj[knn - 1]))) continue;
+            int pos = have < knn ? have++ : knn - 1;
+            while (pos > 0 && (d2 < bd[pos - 1] || (d2 == bd[pos - 1] && j < bj[pos - 1]))) {
+              bd[pos] = bd[pos - 1];
+              bj[pos] = bj[pos - 1];
+              --pos;
+            }
+            bd[pos] = d2;
+            bj[pos] = j;
+          }
+        }
+      }
+      const float reach = (float)(r * g.cell);
+      if (have == knn && bd[knn - 1] < reach * reach) break;
+    }
+    if (have > 0 && !(F(F_SIGMA, bj[0]) >= 0.0)) have = 0;  // the nearest must be valid
+    for (int k = 0; k < have; ++k)
+      add(bj[k], (float)F(F_COORD_X, bj[k]) - xf, (float)F(F_COORD_Y, bj[k]) - yf);
   }
   double out[kVsgFields] = {0, 0, 0, 0, 0, 0, 0};
   bool ok = valid >= 3;
@@ -326,9 +364,11 @@ hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_vsg_strain(const VsgGrid &g, float radius, double *fields, double *vsg, int n, hipStream_t stream) {
+hipError_t launch_vsg_strain(const VsgGrid &g, float radius, int knn, double *fields, double *vsg, int n,
+                             hipStream_t stream) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(vsg_strain_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, g, radius, fields, vsg, n);
+  if (knn < 0 || knn > kVsgMaxK) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(vsg_strain_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, g, radius, knn, fields, vsg, n);
   return hipGetLastError();
 }
 

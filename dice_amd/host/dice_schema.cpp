@@ -838,7 +838,7 @@ int Schema::execute_correlation() {
   check(dice_hip_correlate(ctx_, mode, fields_.data(), frame_id_ == first_frame_id_ ? 1 : 0),
         "execute_correlation");
   // post_execution_tasks: the post-processors of the frame
-  if (vsg_window_ > 0) {
+  if (vsg_window_ != 0) {
     vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
     check(dice_hip_vsg_strain(ctx_, vsg_window_, fields_.data(), vsg_.data()), "VSG strain");
   }
@@ -851,8 +851,8 @@ int Schema::execute_correlation() {
 }
 
 void Schema::set_vsg_strain_window(int window_size_in_pixels) {
-  if (window_size_in_pixels < 0)
-    throw std::invalid_argument("Error, k-nearest VSG windows (negative sizes) are not supported");
+  if (window_size_in_pixels < -64)
+    throw std::invalid_argument("Error, k-nearest VSG windows are limited to 64 neighbours");
   vsg_window_ = window_size_in_pixels;
 }
 
@@ -938,7 +938,7 @@ void Output_Spec::write_info(std::FILE *file) const {
   std::fprintf(file, "*** Subset size: %i\n", s.subset_dim());
   std::fprintf(file, "*** Step size: x %i y %i (-1 implies not regular grid)\n", s.step_size_x(), s.step_size_y());
   // the first strain post-processor's window (DICe_Schema.cpp:4843-4847)
-  if (s.strain_window_size() > 0)
+  if (s.strain_window_size() != 0 && s.strain_window_size() != -1)
     std::fprintf(file, "*** Strain window size in pixels: %i (only first strain post-processor is reported)\n",
                  s.strain_window_size());
   else

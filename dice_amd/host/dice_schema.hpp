@@ -248,8 +248,8 @@ class Schema {
   // The post_process_vsg_strain block of the correlation parameters
   // (strain_window_size_in_pixels, DICe_PostProcessor.cpp:877-906): after every
   // execute_correlation the VSG strains of the frame (VSG_*_FS fields).  0: no
-  // VSG post-processor.  Throws std::invalid_argument for a negative size (the
-  // k-nearest form is not supported).
+  // VSG post-processor; a negative size -k: the k nearest subsets (k <= 64,
+  // std::invalid_argument beyond).
   void set_vsg_strain_window(int window_size_in_pixels);
   int strain_window_size() const { return vsg_window_; }
   // The post_process_uncertainty block (Uncertainty_Post_Processor,

@@ -164,9 +164,10 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
 
 /* VSG_Strain_Post_Processor::execute (DICe_PostProcessor.cpp:877-1136): for each
  * subset, a least-squares plane of u and v over the subsets within
- * window_size / 2 px (window_size > 0; the reference's k-nearest form, a
- * negative size, is not supported) whose SIGMA >= 0, and from its gradient the
- * Green-Lagrange strains.  vsg: DICE_VSG_COUNT * n doubles, slot-major in the
+ * window_size / 2 px (window_size > 0), or over its -window_size nearest subsets
+ * (window_size < 0, at most 64; ties in distance go to the lower subset id,
+ * where the reference's kd-tree has an order of its own), whose SIGMA >= 0, and
+ * from its gradient the Green-Lagrange strains.  vsg: DICE_VSG_COUNT * n doubles, slot-major in the
  * order of enum dice_vsg_field.  Subsets that fail (own SIGMA < 0, fewer than 3
  * valid neighbours, a singular normal matrix) get zeros and MATCH = -1 in
  * fields, as in the reference.  dice_hip_vsg_strain takes host arrays and is

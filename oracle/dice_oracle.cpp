@@ -1198,19 +1198,23 @@ int oracle_vsg_strain(int n, double *F, int window, double *vsg) {
   // DICe_PostProcessor.cpp:140-236, 908-929) and execute (:931-1136).  Neighbours by brute
   // force, sorted by distance as nanoflann's sorted radius search returns them (ties by id);
   // squared distances and the radius test in float, as the float kd-tree has them.
-  if (n < 1 || !F || !vsg || window <= 0) return -1;
+  // window < 0: the -window nearest subsets (knnSearch, :186-206), ties by id; nanoflann's own
+  // order among equidistant points depends on its tree, so only untied cases are pinned
+  if (n < 1 || !F || !vsg || window == 0) return -1;
   auto f = [&](int slot, int i) -> double & { return F[(size_t)slot * n + i]; };
   const float rad = (float)window / 2.0f;
   const float r2 = rad * rad + (float)1.0E-5;
+  const size_t knn = window < 0 ? (size_t)(-window) : 0;
   for (int i = 0; i < n; ++i) {
     const float xi = (float)f(OF_COORD_X, i), yi = (float)f(OF_COORD_Y, i);
     std::vector<std::pair<float, int>> nb;
     for (int j = 0; j < n; ++j) {
       const float dx = (float)f(OF_COORD_X, j) - xi, dy = (float)f(OF_COORD_Y, j) - yi;
       const float d2 = dx * dx + dy * dy;
-      if (d2 < r2) nb.push_back({d2, j});
+      if (knn || d2 < r2) nb.push_back({d2, j});
     }
     std::sort(nb.begin(), nb.end());
+    if (knn && nb.size() > knn) nb.resize(knn);
     double out[7] = {0, 0, 0, 0, 0, 0, 0};
     int valid = 0;
     for (const auto &e : nb) valid += f(OF_SIGMA, e.second) >= 0.0;

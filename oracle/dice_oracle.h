@@ -138,7 +138,8 @@ int oracle_gauss_filter_d(int w, int h, double *img, int mask_size);
 
 /* VSG_Strain_Post_Processor over n subsets: fields as for oracle_correlate (COORD, DISP, SIGMA
  * read; MATCH set to -1 where the strain fails); vsg: 7 * n doubles (strain xx, yy, xy, du/dx,
- * du/dy, dv/dx, dv/dy).  window > 0 (radius window / 2).  Returns 0 or -1. */
+ * du/dy, dv/dx, dv/dy).  window > 0: radius window / 2; window < 0: the -window nearest subsets
+ * (ties by id).  Returns 0 or -1. */
 int oracle_vsg_strain(int n, double *fields, int window, double *vsg);
 
 /* One Objective_ZNSSD call on the square subset of side subset_size centred at (cx, cy):

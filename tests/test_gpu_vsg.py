@@ -71,7 +71,7 @@ def test_vsg_errors(ctx):
     cx, cy = dh.regular_grid(256, 256, 21, 20)
     ctx.set_subsets(cx, cy, 21)
     f = dh.new_fields(cx, cy)
-    for bad in (0, -8):
+    for bad in (0, -65):
         with pytest.raises(dh.DiceHipError):
             ctx.vsg_strain(f, bad)
 
@@ -91,3 +91,19 @@ def test_uncertainty_post_processor_parity(ctx):
     uo = o.uncertainty(f)
     assert np.array_equal(ug, uo)
     assert np.all(ug[0, ::13] == 0) and np.all(ug[0, ::17] == 0)
+
+
+@pytest.mark.parametrize("k", [9, 13, 20])
+def test_vsg_k_nearest_parity(ctx, k):
+    """k-nearest windows (negative sizes): scattered centres, so that no two neighbours tie at the k-th
+    distance (the reference's kd-tree orders ties its own way, the GPU and the oracle by id)."""
+    rng = np.random.default_rng(k)
+    pts = sorted({(int(x), int(y)) for x, y in rng.uniform(20, 1500, size=(4000, 2))})
+    cx = np.array([p[0] for p in pts], np.float64)
+    cy = np.array([p[1] for p in pts], np.float64)
+    f = dh.new_fields(cx, cy)
+    f[dh.F["DISP_X"]] = (0.4 * np.sin(cx / 200) + 0.002 * cy).astype(np.float32)
+    f[dh.F["DISP_Y"]] = (0.3 * np.cos(cy / 150)).astype(np.float32)
+    f[dh.F["SIGMA"]] = 0.01
+    f[dh.F["SIGMA"], rng.random(len(cx)) < 0.03] = -1.0
+    check(ctx, cx, cy, f, -k)

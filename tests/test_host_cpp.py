@@ -225,6 +225,20 @@ def test_oracle_vsg_closed_forms():
     assert np.all(o.vsg_strain(h, 41) == 0) and np.all(h[o.F["MATCH"]] == -1)
 
 
+def test_oracle_vsg_k_nearest_matches_radius_on_complete_shells():
+    """On a regular grid the 9 nearest subsets are the 3 x 3 block, the same set as a 30-px window (radius
+    15 > 10 sqrt 2), away from the grid's edge: both forms give the same strains bit for bit."""
+    xs, ys = np.meshgrid(np.arange(20.0, 300.0, 10.0), np.arange(20.0, 300.0, 10.0))
+    cx, cy = xs.ravel(), ys.ravel()
+    f = o.new_fields(cx, cy)
+    f[o.F["DISP_X"]] = (0.01 * cx + 0.002 * cy + 1e-5 * cx * cy).astype(np.float32)
+    f[o.F["DISP_Y"]] = (0.003 * cx - 2e-5 * cy * cy).astype(np.float32)
+    f[o.F["SIGMA"]] = 0.01
+    knn, rad = o.vsg_strain(f.copy(), -9), o.vsg_strain(f.copy(), 30)
+    inner = (cx > 20) & (cx < 290) & (cy > 20) & (cy < 290)
+    assert np.array_equal(knn[:, inner], rad[:, inner]) and np.any(knn[0, inner] != 0)
+
+
 def test_oracle_uncertainty_post_processor():
     """Uncertainty_Post_Processor (2-D): sigma over the cosine in FIELD_1, rounded as the reference's
     1.0 / angle * sigma (double literal, stored in float); zero for failed subsets and a zero cosine."""
