@@ -875,7 +875,8 @@ def bench_vsg(args):
         n * steps / el, "subsets/s", 1000.0 * el / steps, "f64 (float neighbour distances)",
         {"workload": "the c2 frame with A = 1.0 px correlated once (6-parameter GN), then the VSG strain "
                      "post-processor over all %d subsets with a %d-px window; each call includes the "
-                     "host-side neighbour grid and its upload" % (n, window),
+                     "neighbour grid built on the device (bounds, cell counts, prefix sum, radix sort of "
+                     "(cell, id), packed records) and one 40-byte read-back of the bounds" % (n, window),
          "window": window, "match_failed": int((d_f[dh.F["MATCH"]] == -1).sum().item()),
          "mean_abs_dudx_error": float(np.abs(v[3] - dudx_true).mean())},
         None, cpu_baseline=cpu)

@@ -84,6 +84,7 @@ struct dice_hip_context {
                                                      // subsets to correlate again, their count per round
   int nb_rounds = 0;                         // correlation rounds of the last neighbour-initialized frame
   DevBuf vsg_cells, vsg_ids, vsg_out;        // VSG strain: subset grid, host-call output
+  DevBuf vsg_keys, vsg_keys2, vsg_ids2, vsg_counts, vsg_rec, vsg_temp, vsg_bounds;  // VSG grid build
   DevBuf obj_io;                             // dice_hip_objective: parameters and results
   bool def_has_grads = false;
   // subsets
@@ -340,7 +341,8 @@ int dice_hip_destroy(dice_hip_context *ctx) {
   for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
                     &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
-                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids,
+                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2, &ctx->vsg_counts,
+                    &ctx->vsg_rec, &ctx->vsg_temp, &ctx->vsg_bounds,
                     &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
@@ -664,17 +666,14 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
   const int knn = window_size < 0 ? -window_size : 0;
   if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
   const int n = ctx->n;
-  std::vector<double> xy(2 * (size_t)n);
-  HIP_TRY(ctx, hipMemcpyAsync(xy.data(), d_fields + (size_t)DICE_F_COORD_X * n, 2 * (size_t)n * sizeof(double),
-                              hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy coordinates");
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "coordinates");
-  const double *X = xy.data(), *Y = xy.data() + n;
-  double x0 = X[0], y0 = Y[0], x1 = X[0], y1 = Y[0];
-  for (int i = 0; i < n; ++i) {
-    if (!std::isfinite(X[i]) || !std::isfinite(Y[i])) return ctx->fail(DICE_HIP_EINVAL, "non-finite coordinates");
-    x0 = std::min(x0, X[i]); x1 = std::max(x1, X[i]);
-    y0 = std::min(y0, Y[i]); y1 = std::max(y1, Y[i]);
-  }
+  // bounds of the centres on the device; only these five numbers come to the host
+  HIP_TRY(ctx, ctx->vsg_bounds.ensure(5 * sizeof(double)), "hipMalloc VSG grid");
+  HIP_TRY(ctx, dice_hip::launch_vsg_bounds(d_fields, n, ctx->vsg_bounds.as<double>(), ctx->stream), "VSG bounds");
+  double b[5];
+  HIP_TRY(ctx, hipMemcpyAsync(b, ctx->vsg_bounds.p, sizeof b, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy VSG bounds");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG bounds");
+  if (b[4] != 0.0) return ctx->fail(DICE_HIP_EINVAL, "non-finite coordinates");
+  const double x0 = b[0], y0 = b[1], x1 = b[2], y1 = b[3];
   const float radius = knn ? 0.0f : (float)window_size / 2.0f;
   dice_hip::VsgGrid g{};
   g.x0 = x0;
@@ -685,27 +684,23 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
   g.cell = std::max({side, 1.0, (x1 - x0) / 4000.0, (y1 - y0) / 4000.0});
   g.ncx = (int)std::floor((x1 - x0) / g.cell) + 1;
   g.ncy = (int)std::floor((y1 - y0) / g.cell) + 1;
-  const size_t ncells = (size_t)g.ncx * g.ncy;
-  std::vector<int> start(ncells + 1, 0), ids(n), cell(n);
-  for (int i = 0; i < n; ++i) {
-    const int cx = std::min(std::max((int)std::floor((X[i] - x0) / g.cell), 0), g.ncx - 1);
-    const int cy = std::min(std::max((int)std::floor((Y[i] - y0) / g.cell), 0), g.ncy - 1);
-    cell[i] = cy * g.ncx + cx;
-    ++start[cell[i] + 1];
-  }
-  for (size_t c = 0; c < ncells; ++c) start[c + 1] += start[c];
-  std::vector<int> fill(start.begin(), start.end() - 1);
-  for (int i = 0; i < n; ++i) ids[fill[cell[i]]++] = i;
-  HIP_TRY(ctx, ctx->vsg_cells.ensure((ncells + 1) * sizeof(int)), "hipMalloc VSG grid");
-  HIP_TRY(ctx, ctx->vsg_ids.ensure((size_t)n * sizeof(int)), "hipMalloc VSG grid");
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->vsg_cells.p, start.data(), (ncells + 1) * sizeof(int), hipMemcpyHostToDevice,
-                              ctx->stream), "hipMemcpy VSG grid");
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->vsg_ids.p, ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream),
-          "hipMemcpy VSG grid");
-  g.cell_start = ctx->vsg_cells.as<int>();
-  g.cell_ids = ctx->vsg_ids.as<int>();
+  const long long ncells = (long long)g.ncx * g.ncy;
+  size_t temp_bytes = 0;
+  HIP_TRY(ctx, dice_hip::vsg_grid_temp_bytes(n, ncells, &temp_bytes), "VSG grid scratch size");
+  const size_t nb = (size_t)n * sizeof(int), cb = (size_t)(ncells + 1) * sizeof(int);
+  HIP_TRY(ctx, ctx->vsg_cells.ensure(cb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_counts.ensure(cb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_ids.ensure(nb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_ids2.ensure(nb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_keys.ensure(nb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_keys2.ensure(nb), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_rec.ensure((size_t)n * sizeof(float4)), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_temp.ensure(temp_bytes), "hipMalloc VSG grid");
+  const dice_hip::VsgWork w{ctx->vsg_keys.as<int>(), ctx->vsg_ids.as<int>(), ctx->vsg_keys2.as<int>(),
+                            ctx->vsg_ids2.as<int>(),  ctx->vsg_counts.as<int>(), ctx->vsg_cells.as<int>(),
+                            ctx->vsg_rec.as<float4>(), ctx->vsg_temp.p, temp_bytes};
+  HIP_TRY(ctx, dice_hip::launch_vsg_grid(d_fields, n, g, w, ctx->stream), "VSG grid");
   HIP_TRY(ctx, dice_hip::launch_vsg_strain(g, radius, knn, d_fields, d_vsg, n, ctx->stream), "VSG strain kernel");
-  // the host vectors must outlive the copies
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG strain");
   return DICE_HIP_OK;
 }

@@ -175,8 +175,27 @@ hipError_t launch_reset_listed(const int *list, const int *count, const double *
 struct VsgGrid {
   double x0, y0, cell;
   int ncx, ncy;
-  const int *cell_start, *cell_ids;
+  const int *cell_start;  // ncx * ncy + 1 offsets into the cell-ordered list
+  const int *cell_ids;    // subset id per entry, ~id for failed subsets (SIGMA < 0)
+  const float4 *rec;      // (x, y, u, v) of the entry's subset as float
 };
+// Scratch of the device-side grid build (n entries each unless noted).
+struct VsgWork {
+  int *keys, *ids, *keys_sorted, *ids_sorted;
+  int *counts;      // ncells + 1
+  int *cell_start;  // ncells + 1
+  float4 *rec;
+  void *temp;       // rocPRIM scratch of vsg_grid_temp_bytes
+  size_t temp_bytes;
+};
+// Bounds of the finite centres: bounds[0..4] = min x, min y, max x, max y,
+// count of non-finite centres (5 device doubles).
+hipError_t launch_vsg_bounds(const double *fields, int n, double *bounds, hipStream_t stream);
+hipError_t vsg_grid_temp_bytes(int n, long long ncells, size_t *bytes);
+// Bin the centres into g's cells (g.x0, y0, cell, ncx, ncy set) on the device:
+// counts, prefix sum, a stable radix sort of (cell, id), packed records; sets
+// g.cell_start, g.cell_ids and g.rec to w's buffers.
+hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWork &w, hipStream_t stream);
 // knn > 0 (at most kVsgMaxK): the knn nearest subsets instead (this one included, ties in
 // squared float distance broken by subset id), the grid of any cell side.
 hipError_t launch_vsg_strain(const VsgGrid &g, float radius, int knn, double *fields, double *vsg, int n,

@@ -2,6 +2,11 @@
 // the correlation passes (the solver kernels live in dice_k_*.hip).
 #include "dice_device.h"
 
+#include <algorithm>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 namespace dice_hip {
 
 typedef hipError_t (*PassLauncher)(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
@@ -246,6 +251,76 @@ __device__ double norm1_3(const double (&a)[3][3]) {
   return m;
 }
 
+// Bounds of the subset centres for the VSG grid: one workgroup of 1024
+// threads; out = min x, min y, max x, max y over the finite centres, then the
+// number of non-finite ones.
+__global__ __launch_bounds__(1024) void vsg_bounds_kernel(const double *fields, int n, double *out) {
+  __shared__ double red[4][1024];
+  __shared__ int bad_total;
+  const double *X = fields + (size_t)F_COORD_X * n, *Y = fields + (size_t)F_COORD_Y * n;
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = X[i], y = Y[i];
+    if (!isfinite(x) || !isfinite(y)) {
+      ++bad;
+      continue;
+    }
+    x0 = fmin(x0, x);
+    y0 = fmin(y0, y);
+    x1 = fmax(x1, x);
+    y1 = fmax(y1, y);
+  }
+  const int t = threadIdx.x;
+  if (t == 0) bad_total = 0;
+  red[0][t] = x0;
+  red[1][t] = y0;
+  red[2][t] = x1;
+  red[3][t] = y1;
+  __syncthreads();
+  if (bad) atomicAdd(&bad_total, bad);
+  for (int s = 512; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = fmin(red[0][t], red[0][t + s]);
+      red[1][t] = fmin(red[1][t], red[1][t + s]);
+      red[2][t] = fmax(red[2][t], red[2][t + s]);
+      red[3][t] = fmax(red[3][t], red[3][t + s]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int k = 0; k < 4; ++k) out[k] = red[k][0];
+    out[4] = (double)bad_total;
+  }
+}
+
+// Cell of every subset centre (the same arithmetic as vsg_strain_kernel's own
+// cell), its id, and the per-cell counts (counts zeroed beforehand).
+__global__ void vsg_cells_kernel(const double *fields, int n, VsgGrid g, int *keys, int *ids, int *counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = fields[(size_t)F_COORD_X * n + i], yi = fields[(size_t)F_COORD_Y * n + i];
+  const int ci = min(max((int)floor((xi - g.x0) / g.cell), 0), g.ncx - 1);
+  const int cj = min(max((int)floor((yi - g.y0) / g.cell), 0), g.ncy - 1);
+  const int c = cj * g.ncx + ci;
+  keys[i] = c;
+  ids[i] = i;
+  atomicAdd(counts + c, 1);
+}
+
+// Entry k of the cell-ordered list: the centre and displacement of subset
+// ids[k] as float (the precision the strain uses), and ids[k] replaced by ~id
+// when the subset failed (SIGMA < 0), so that the neighbour loop reads one
+// 16-byte record and one word per candidate.
+__global__ void vsg_pack_kernel(const double *fields, int n, int *ids, float4 *rec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int j = ids[k];
+  auto F = [&](int s) { return fields[(size_t)s * n + j]; };
+  rec[k] = make_float4((float)F(F_COORD_X), (float)F(F_COORD_Y), (float)F(F_DISP_X), (float)F(F_DISP_Y));
+  if (!(F(F_SIGMA) >= 0.0)) ids[k] = ~j;
+}
+
 // One subset per thread (see launch_vsg_strain).
 __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fields, double *vsg, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,11 +331,10 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
   const float r2 = radius * radius + 1.0e-5f;  // the reference's rad^2 + tiny in scalar_t
   double s1 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, su = 0, sxu = 0, syu = 0, sv = 0, sxv = 0, syv = 0;
   int valid = 0;
-  auto add = [&](int j, float ddx, float ddy) {
-    if (!(F(F_SIGMA, j) >= 0.0)) return;
-    // the reference keeps the neighbour offsets and the displacements as scalar_t (float)
+  // the reference keeps the neighbour offsets and the displacements as scalar_t (float)
+  auto add = [&](float uf, float vf, float ddx, float ddy) {
     const double dx = ddx, dy = ddy;
-    const double u = (float)F(F_DISP_X, j), v = (float)F(F_DISP_Y, j);
+    const double u = uf, v = vf;
     ++valid;
     s1 += 1.0; sx += dx; sy += dy;
     sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
@@ -274,9 +348,9 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
       for (int cx = max(ci - 1, 0); cx <= min(ci + 1, g.ncx - 1); ++cx) {
         const int c = cy * g.ncx + cx;
         for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
-          const int j = g.cell_ids[k];
-          const float ddx = (float)F(F_COORD_X, j) - xf, ddy = (float)F(F_COORD_Y, j) - yf;
-          if (ddx * ddx + ddy * ddy < r2) add(j, ddx, ddy);
+          const float4 q = g.rec[k];
+          const float ddx = q.x - xf, ddy = q.y - yf;
+          if (ddx * ddx + ddy * ddy < r2 && g.cell_ids[k] >= 0) add(q.z, q.w, ddx, ddy);
         }
       }
   } else if (F(F_SIGMA, i) >= 0.0) {
@@ -294,8 +368,9 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
           if (cx < 0 || cx >= g.ncx) continue;
           const int c = cy * g.ncx + cx;
           for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
-            const int j = g.cell_ids[k];
-            const float ddx = (float)F(F_COORD_X, j) - xf, ddy = (float)F(F_COORD_Y, j) - yf;
+            const int j = g.cell_ids[k] < 0 ? ~g.cell_ids[k] : g.cell_ids[k];
+            const float4 q = g.rec[k];
+            const float ddx = q.x - xf, ddy = q.y - yf;
             const float d2 = ddx * ddx + ddy * ddy;
             if (have == knn && !(d2 < bd[knn - 1] || (d2 == bd[knn - 1] && j < bj[knn - 1]))) continue;
             int pos = have < knn ? have++ : knn - 1;
@@ -313,8 +388,11 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
       if (have == knn && bd[knn - 1] < reach * reach) break;
     }
     if (have > 0 && !(F(F_SIGMA, bj[0]) >= 0.0)) have = 0;  // the nearest must be valid
-    for (int k = 0; k < have; ++k)
-      add(bj[k], (float)F(F_COORD_X, bj[k]) - xf, (float)F(F_COORD_Y, bj[k]) - yf);
+    for (int k = 0; k < have; ++k) {
+      const int j = bj[k];
+      if (F(F_SIGMA, j) >= 0.0)
+        add((float)F(F_DISP_X, j), (float)F(F_DISP_Y, j), (float)F(F_COORD_X, j) - xf, (float)F(F_COORD_Y, j) - yf);
+    }
   }
   double out[kVsgFields] = {0, 0, 0, 0, 0, 0, 0};
   bool ok = valid >= 3;
@@ -361,6 +439,56 @@ __global__ void uncertainty_kernel(const double *fields, double *out, int n) {
 hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(uncertainty_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_vsg_bounds(const double *fields, int n, double *bounds, hipStream_t stream) {
+  if (n <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(vsg_bounds_kernel, dim3(1), dim3(1024), 0, stream, fields, n, bounds);
+  return hipGetLastError();
+}
+
+static int vsg_key_bits(long long ncells) {
+  int bits = 1;
+  while ((1LL << bits) < ncells) ++bits;
+  return bits;
+}
+
+hipError_t vsg_grid_temp_bytes(int n, long long ncells, size_t *bytes) {
+  size_t sort_bytes = 0, scan_bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, sort_bytes, (const int *)nullptr, (int *)nullptr,
+                                           (const int *)nullptr, (int *)nullptr, (size_t)n, 0, vsg_key_bits(ncells));
+  if (e != hipSuccess) return e;
+  e = rocprim::exclusive_scan(nullptr, scan_bytes, (const int *)nullptr, (int *)nullptr, 0, (size_t)ncells + 1,
+                              rocprim::plus<int>());
+  if (e != hipSuccess) return e;
+  *bytes = std::max(sort_bytes, scan_bytes);
+  return hipSuccess;
+}
+
+hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWork &w, hipStream_t stream) {
+  if (n <= 0) return hipErrorInvalidValue;
+  const long long ncells = (long long)g.ncx * g.ncy;
+  const int nc1 = (int)(ncells + 1);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((nc1 + 255) / 256), dim3(256), 0, stream, w.counts, nc1);
+  hipLaunchKernelGGL(vsg_cells_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, n, g, w.keys, w.ids,
+                     w.counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = w.temp_bytes;
+  // cell_start = exclusive prefix sum of the counts (the last entry is n)
+  e = rocprim::exclusive_scan(w.temp, bytes, (const int *)w.counts, w.cell_start, 0, (size_t)nc1,
+                              rocprim::plus<int>(), stream);
+  if (e != hipSuccess) return e;
+  // ids ordered by cell; the radix sort is stable, so ids ascend within a cell
+  bytes = w.temp_bytes;
+  e = rocprim::radix_sort_pairs(w.temp, bytes, (const int *)w.keys, w.keys_sorted, (const int *)w.ids, w.ids_sorted,
+                                (size_t)n, 0, vsg_key_bits(ncells), stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vsg_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, n, w.ids_sorted, w.rec);
+  g.cell_start = w.cell_start;
+  g.cell_ids = w.ids_sorted;
+  g.rec = w.rec;
   return hipGetLastError();
 }
 
