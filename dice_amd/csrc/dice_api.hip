@@ -84,7 +84,7 @@ struct dice_hip_context {
                                                      // subsets to correlate again, their count per round
   int nb_rounds = 0;                         // correlation rounds of the last neighbour-initialized frame
   DevBuf vsg_cells, vsg_ids, vsg_out;        // VSG strain: subset grid, host-call output
-  DevBuf vsg_keys, vsg_keys2, vsg_ids2, vsg_counts, vsg_rec, vsg_temp, vsg_bounds;  // VSG grid build
+  DevBuf vsg_keys, vsg_keys2, vsg_ids2, vsg_rec, vsg_temp, vsg_bounds;  // VSG grid build
   DevBuf obj_io;                             // dice_hip_objective: parameters and results
   bool def_has_grads = false;
   // subsets
@@ -341,7 +341,7 @@ int dice_hip_destroy(dice_hip_context *ctx) {
   for (DevBuf *b : {&ctx->ref_pix, &ctx->ref_gx, &ctx->ref_gy, &ctx->def_pix, &ctx->def_gx, &ctx->def_gy, &ctx->tmp_x,
                     &ctx->tmp_y, &ctx->cx, &ctx->cy, &ctx->off, &ctx->px, &ctx->py, &ctx->act, &ctx->bbox, &ctx->nact,
                     &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
-                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2, &ctx->vsg_counts,
+                    &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2,
                     &ctx->vsg_rec, &ctx->vsg_temp, &ctx->vsg_bounds,
                     &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work})
     b->release();
@@ -667,7 +667,7 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
   if (ctx->n <= 0) return ctx->fail(DICE_HIP_EINVAL, "no subsets");
   const int n = ctx->n;
   // bounds of the centres on the device; only these five numbers come to the host
-  HIP_TRY(ctx, ctx->vsg_bounds.ensure(5 * sizeof(double)), "hipMalloc VSG grid");
+  HIP_TRY(ctx, ctx->vsg_bounds.ensure(dice_hip::kVsgBoundsDoubles * sizeof(double)), "hipMalloc VSG grid");
   HIP_TRY(ctx, dice_hip::launch_vsg_bounds(d_fields, n, ctx->vsg_bounds.as<double>(), ctx->stream), "VSG bounds");
   double b[5];
   HIP_TRY(ctx, hipMemcpyAsync(b, ctx->vsg_bounds.p, sizeof b, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy VSG bounds");
@@ -687,9 +687,8 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
   const long long ncells = (long long)g.ncx * g.ncy;
   size_t temp_bytes = 0;
   HIP_TRY(ctx, dice_hip::vsg_grid_temp_bytes(n, ncells, &temp_bytes), "VSG grid scratch size");
-  const size_t nb = (size_t)n * sizeof(int), cb = (size_t)(ncells + 1) * sizeof(int);
-  HIP_TRY(ctx, ctx->vsg_cells.ensure(cb), "hipMalloc VSG grid");
-  HIP_TRY(ctx, ctx->vsg_counts.ensure(cb), "hipMalloc VSG grid");
+  const size_t nb = (size_t)n * sizeof(int);
+  HIP_TRY(ctx, ctx->vsg_cells.ensure(2 * (size_t)ncells * sizeof(int)), "hipMalloc VSG grid");
   HIP_TRY(ctx, ctx->vsg_ids.ensure(nb), "hipMalloc VSG grid");
   HIP_TRY(ctx, ctx->vsg_ids2.ensure(nb), "hipMalloc VSG grid");
   HIP_TRY(ctx, ctx->vsg_keys.ensure(nb), "hipMalloc VSG grid");
@@ -697,8 +696,8 @@ static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields,
   HIP_TRY(ctx, ctx->vsg_rec.ensure((size_t)n * sizeof(float4)), "hipMalloc VSG grid");
   HIP_TRY(ctx, ctx->vsg_temp.ensure(temp_bytes), "hipMalloc VSG grid");
   const dice_hip::VsgWork w{ctx->vsg_keys.as<int>(), ctx->vsg_ids.as<int>(), ctx->vsg_keys2.as<int>(),
-                            ctx->vsg_ids2.as<int>(),  ctx->vsg_counts.as<int>(), ctx->vsg_cells.as<int>(),
-                            ctx->vsg_rec.as<float4>(), ctx->vsg_temp.p, temp_bytes};
+                            ctx->vsg_ids2.as<int>(),  ctx->vsg_cells.as<int>(),   ctx->vsg_rec.as<float4>(),
+                            ctx->vsg_temp.p,          temp_bytes};
   HIP_TRY(ctx, dice_hip::launch_vsg_grid(d_fields, n, g, w, ctx->stream), "VSG grid");
   HIP_TRY(ctx, dice_hip::launch_vsg_strain(g, radius, knn, d_fields, d_vsg, n, ctx->stream), "VSG strain kernel");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "VSG strain");

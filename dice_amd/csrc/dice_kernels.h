@@ -166,8 +166,8 @@ hipError_t launch_reset_listed(const int *list, const int *count, const double *
 // VSG_Strain_Post_Processor::execute (DICe_PostProcessor.cpp:931-1136) for
 // every subset: the neighbours within the window radius r (squared distance
 // below r^2 + 1e-5, formed in float as the reference's float kd-tree does),
-// found through a uniform grid of cell side cell >= r (cell_start: ncx * ncy + 1
-// offsets into cell_ids, subsets sorted by cell), a least-squares plane of u and
+// found through a uniform grid of cell side cell >= r (subsets sorted by cell,
+// each cell a range of the sorted list), a least-squares plane of u and
 // v over the valid ones (sigma >= 0), its gradient and the Green-Lagrange
 // strains.  vsg: 7 x n doubles (strain xx, yy, xy, du/dx, du/dy, dv/dx, dv/dy);
 // subsets whose own sigma is < 0, that have fewer than 3 valid neighbours, or
@@ -175,26 +175,27 @@ hipError_t launch_reset_listed(const int *list, const int *count, const double *
 struct VsgGrid {
   double x0, y0, cell;
   int ncx, ncy;
-  const int *cell_start;  // ncx * ncy + 1 offsets into the cell-ordered list
+  const int *cell_begin, *cell_end;  // per cell: its range of the cell-ordered list
   const int *cell_ids;    // subset id per entry, ~id for failed subsets (SIGMA < 0)
   const float4 *rec;      // (x, y, u, v) of the entry's subset as float
 };
 // Scratch of the device-side grid build (n entries each unless noted).
 struct VsgWork {
   int *keys, *ids, *keys_sorted, *ids_sorted;
-  int *counts;      // ncells + 1
-  int *cell_start;  // ncells + 1
+  int *cell_range;  // 2 * ncells: begin, then end, of each cell
   float4 *rec;
   void *temp;       // rocPRIM scratch of vsg_grid_temp_bytes
   size_t temp_bytes;
 };
 // Bounds of the finite centres: bounds[0..4] = min x, min y, max x, max y,
-// count of non-finite centres (5 device doubles).
+// count of non-finite centres; bounds holds kVsgBoundsDoubles device doubles
+// (the rest are partials).
+constexpr int kVsgBoundsDoubles = 5 + 5 * 256;
 hipError_t launch_vsg_bounds(const double *fields, int n, double *bounds, hipStream_t stream);
 hipError_t vsg_grid_temp_bytes(int n, long long ncells, size_t *bytes);
 // Bin the centres into g's cells (g.x0, y0, cell, ncx, ncy set) on the device:
-// counts, prefix sum, a stable radix sort of (cell, id), packed records; sets
-// g.cell_start, g.cell_ids and g.rec to w's buffers.
+// keys, a stable radix sort of (cell, id), cell ranges, packed records; sets
+// g.cell_begin, g.cell_end, g.cell_ids and g.rec to w's buffers.
 hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWork &w, hipStream_t stream);
 // knn > 0 (at most kVsgMaxK): the knn nearest subsets instead (this one included, ties in
 // squared float distance broken by subset id), the grid of any cell side.

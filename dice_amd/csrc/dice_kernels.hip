@@ -5,7 +5,6 @@
 #include <algorithm>
 
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 namespace dice_hip {
 
@@ -251,19 +250,42 @@ __device__ double norm1_3(const double (&a)[3][3]) {
   return m;
 }
 
-// Bounds of the subset centres for the VSG grid: one workgroup of 1024
-// threads; out = min x, min y, max x, max y over the finite centres, then the
-// number of non-finite ones.
-__global__ __launch_bounds__(1024) void vsg_bounds_kernel(const double *fields, int n, double *out) {
-  __shared__ double red[4][1024];
-  __shared__ int bad_total;
+// Bounds of the subset centres for the VSG grid, in two launches: each of up
+// to kVsgBoundBlocks workgroups reduces a grid-stride share into 5 partials
+// (min x, min y, max x, max y over the finite centres, count of non-finite
+// ones), then one workgroup folds the partials into out[0..4].
+constexpr int kVsgBoundBlocks = 256;
+
+__device__ void vsg_bounds_fold(double (&red)[5][256], double x0, double y0, double x1, double y1, double bad,
+                                double *out) {
+  const int t = threadIdx.x;
+  red[0][t] = x0;
+  red[1][t] = y0;
+  red[2][t] = x1;
+  red[3][t] = y1;
+  red[4][t] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = fmin(red[0][t], red[0][t + s]);
+      red[1][t] = fmin(red[1][t], red[1][t + s]);
+      red[2][t] = fmax(red[2][t], red[2][t + s]);
+      red[3][t] = fmax(red[3][t], red[3][t + s]);
+      red[4][t] += red[4][t + s];
+    }
+    __syncthreads();
+  }
+  if (t < 5) out[t] = red[t][0];
+}
+
+__global__ __launch_bounds__(256) void vsg_bounds_partial_kernel(const double *fields, int n, double *partial) {
+  __shared__ double red[5][256];
   const double *X = fields + (size_t)F_COORD_X * n, *Y = fields + (size_t)F_COORD_Y * n;
-  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-  int bad = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, bad = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double x = X[i], y = Y[i];
     if (!isfinite(x) || !isfinite(y)) {
-      ++bad;
+      bad += 1.0;
       continue;
     }
     x0 = fmin(x0, x);
@@ -271,41 +293,45 @@ __global__ __launch_bounds__(1024) void vsg_bounds_kernel(const double *fields, 
     x1 = fmax(x1, x);
     y1 = fmax(y1, y);
   }
+  vsg_bounds_fold(red, x0, y0, x1, y1, bad, partial + 5 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void vsg_bounds_final_kernel(const double *partial, int blocks, double *out) {
+  __shared__ double red[5][256];
   const int t = threadIdx.x;
-  if (t == 0) bad_total = 0;
-  red[0][t] = x0;
-  red[1][t] = y0;
-  red[2][t] = x1;
-  red[3][t] = y1;
-  __syncthreads();
-  if (bad) atomicAdd(&bad_total, bad);
-  for (int s = 512; s > 0; s >>= 1) {
-    if (t < s) {
-      red[0][t] = fmin(red[0][t], red[0][t + s]);
-      red[1][t] = fmin(red[1][t], red[1][t + s]);
-      red[2][t] = fmax(red[2][t], red[2][t + s]);
-      red[3][t] = fmax(red[3][t], red[3][t + s]);
-    }
-    __syncthreads();
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, bad = 0.0;
+  if (t < blocks) {
+    const double *q = partial + 5 * t;
+    x0 = q[0];
+    y0 = q[1];
+    x1 = q[2];
+    y1 = q[3];
+    bad = q[4];
   }
-  if (t == 0) {
-    for (int k = 0; k < 4; ++k) out[k] = red[k][0];
-    out[4] = (double)bad_total;
-  }
+  vsg_bounds_fold(red, x0, y0, x1, y1, bad, out);
 }
 
 // Cell of every subset centre (the same arithmetic as vsg_strain_kernel's own
-// cell), its id, and the per-cell counts (counts zeroed beforehand).
-__global__ void vsg_cells_kernel(const double *fields, int n, VsgGrid g, int *keys, int *ids, int *counts) {
+// cell) and its id, the keys of the sort.
+__global__ void vsg_cells_kernel(const double *fields, int n, VsgGrid g, int *keys, int *ids) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double xi = fields[(size_t)F_COORD_X * n + i], yi = fields[(size_t)F_COORD_Y * n + i];
   const int ci = min(max((int)floor((xi - g.x0) / g.cell), 0), g.ncx - 1);
   const int cj = min(max((int)floor((yi - g.y0) / g.cell), 0), g.ncy - 1);
-  const int c = cj * g.ncx + ci;
-  keys[i] = c;
+  keys[i] = cj * g.ncx + ci;
   ids[i] = i;
-  atomicAdd(counts + c, 1);
+}
+
+// Cell ranges from the sorted keys: entry k opens its cell when the previous
+// key differs and closes it when the next one does (cells without entries keep
+// the empty range [0, 0) of the zero fill).
+__global__ void vsg_ranges_kernel(const int *keys, int n, int *begin, int *end) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int c = keys[k];
+  if (k == 0 || keys[k - 1] != c) begin[c] = k;
+  if (k == n - 1 || keys[k + 1] != c) end[c] = k + 1;
 }
 
 // Entry k of the cell-ordered list: the centre and displacement of subset
@@ -347,7 +373,7 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
     for (int cy = max(cj - 1, 0); cy <= min(cj + 1, g.ncy - 1); ++cy)
       for (int cx = max(ci - 1, 0); cx <= min(ci + 1, g.ncx - 1); ++cx) {
         const int c = cy * g.ncx + cx;
-        for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
+        for (int k = g.cell_begin[c]; k < g.cell_end[c]; ++k) {
           const float4 q = g.rec[k];
           const float ddx = q.x - xf, ddy = q.y - yf;
           if (ddx * ddx + ddy * ddy < r2 && g.cell_ids[k] >= 0) add(q.z, q.w, ddx, ddy);
@@ -367,7 +393,7 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
         for (int cx = ci - r; cx <= ci + r; cx += max(step, 1)) {
           if (cx < 0 || cx >= g.ncx) continue;
           const int c = cy * g.ncx + cx;
-          for (int k = g.cell_start[c]; k < g.cell_start[c + 1]; ++k) {
+          for (int k = g.cell_begin[c]; k < g.cell_end[c]; ++k) {
             const int j = g.cell_ids[k] < 0 ? ~g.cell_ids[k] : g.cell_ids[k];
             const float4 q = g.rec[k];
             const float ddx = q.x - xf, ddy = q.y - yf;
@@ -444,7 +470,10 @@ hipError_t launch_uncertainty(const double *fields, double *out, int n, hipStrea
 
 hipError_t launch_vsg_bounds(const double *fields, int n, double *bounds, hipStream_t stream) {
   if (n <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(vsg_bounds_kernel, dim3(1), dim3(1024), 0, stream, fields, n, bounds);
+  // bounds holds 5 results, then 5 partials per workgroup
+  const int blocks = std::min(kVsgBoundBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(vsg_bounds_partial_kernel, dim3(blocks), dim3(256), 0, stream, fields, n, bounds + 5);
+  hipLaunchKernelGGL(vsg_bounds_final_kernel, dim3(1), dim3(256), 0, stream, bounds + 5, blocks, bounds);
   return hipGetLastError();
 }
 
@@ -455,38 +484,28 @@ static int vsg_key_bits(long long ncells) {
 }
 
 hipError_t vsg_grid_temp_bytes(int n, long long ncells, size_t *bytes) {
-  size_t sort_bytes = 0, scan_bytes = 0;
-  hipError_t e = rocprim::radix_sort_pairs(nullptr, sort_bytes, (const int *)nullptr, (int *)nullptr,
-                                           (const int *)nullptr, (int *)nullptr, (size_t)n, 0, vsg_key_bits(ncells));
-  if (e != hipSuccess) return e;
-  e = rocprim::exclusive_scan(nullptr, scan_bytes, (const int *)nullptr, (int *)nullptr, 0, (size_t)ncells + 1,
-                              rocprim::plus<int>());
-  if (e != hipSuccess) return e;
-  *bytes = std::max(sort_bytes, scan_bytes);
-  return hipSuccess;
+  return rocprim::radix_sort_pairs(nullptr, *bytes, (const int *)nullptr, (int *)nullptr, (const int *)nullptr,
+                                   (int *)nullptr, (size_t)n, 0, vsg_key_bits(ncells));
 }
 
 hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWork &w, hipStream_t stream) {
   if (n <= 0) return hipErrorInvalidValue;
   const long long ncells = (long long)g.ncx * g.ncy;
-  const int nc1 = (int)(ncells + 1);
-  hipLaunchKernelGGL(zero_ints_kernel, dim3((nc1 + 255) / 256), dim3(256), 0, stream, w.counts, nc1);
-  hipLaunchKernelGGL(vsg_cells_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, n, g, w.keys, w.ids,
-                     w.counts);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((int)((2 * ncells + 255) / 256)), dim3(256), 0, stream, w.cell_range,
+                     (int)(2 * ncells));
+  hipLaunchKernelGGL(vsg_cells_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, n, g, w.keys, w.ids);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t bytes = w.temp_bytes;
-  // cell_start = exclusive prefix sum of the counts (the last entry is n)
-  e = rocprim::exclusive_scan(w.temp, bytes, (const int *)w.counts, w.cell_start, 0, (size_t)nc1,
-                              rocprim::plus<int>(), stream);
-  if (e != hipSuccess) return e;
   // ids ordered by cell; the radix sort is stable, so ids ascend within a cell
-  bytes = w.temp_bytes;
+  size_t bytes = w.temp_bytes;
   e = rocprim::radix_sort_pairs(w.temp, bytes, (const int *)w.keys, w.keys_sorted, (const int *)w.ids, w.ids_sorted,
                                 (size_t)n, 0, vsg_key_bits(ncells), stream);
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vsg_ranges_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, w.keys_sorted, n, w.cell_range,
+                     w.cell_range + ncells);
   hipLaunchKernelGGL(vsg_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fields, n, w.ids_sorted, w.rec);
-  g.cell_start = w.cell_start;
+  g.cell_begin = w.cell_range;
+  g.cell_end = w.cell_range + ncells;
   g.cell_ids = w.ids_sorted;
   g.rec = w.rec;
   return hipGetLastError();

@@ -172,7 +172,8 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
  * valid neighbours, a singular normal matrix) get zeros and MATCH = -1 in
  * fields, as in the reference.  dice_hip_vsg_strain takes host arrays and is
  * synchronous; the _device form takes device arrays on the context stream and
- * waits for the kernel (the neighbour grid is built on the host). */
+ * waits for the kernel.  The neighbour grid is built on the device; a non-finite
+ * centre coordinate is an error (DICE_HIP_EINVAL). */
 enum dice_vsg_field {
   DICE_VSG_STRAIN_XX = 0, DICE_VSG_STRAIN_YY, DICE_VSG_STRAIN_XY,
   DICE_VSG_DUDX, DICE_VSG_DUDY, DICE_VSG_DVDX, DICE_VSG_DVDY, DICE_VSG_COUNT
