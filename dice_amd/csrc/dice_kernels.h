@@ -178,6 +178,7 @@ struct VsgGrid {
   const int *cell_begin, *cell_end;  // per cell: its range of the cell-ordered list
   const int *cell_ids;    // subset id per entry, ~id for failed subsets (SIGMA < 0)
   const float4 *rec;      // (x, y, u, v) of the entry's subset as float
+  const int *cell_keys;   // the entry's cell (ascending)
 };
 // Scratch of the device-side grid build (n entries each unless noted).
 struct VsgWork {
@@ -195,7 +196,7 @@ hipError_t launch_vsg_bounds(const double *fields, int n, double *bounds, hipStr
 hipError_t vsg_grid_temp_bytes(int n, long long ncells, size_t *bytes);
 // Bin the centres into g's cells (g.x0, y0, cell, ncx, ncy set) on the device:
 // keys, a stable radix sort of (cell, id), cell ranges, packed records; sets
-// g.cell_begin, g.cell_end, g.cell_ids and g.rec to w's buffers.
+// g.cell_begin, g.cell_end, g.cell_ids, g.rec and g.cell_keys to w's buffers.
 hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWork &w, hipStream_t stream);
 // knn > 0 (at most kVsgMaxK): the knn nearest subsets instead (this one included, ties in
 // squared float distance broken by subset id), the grid of any cell side.

@@ -348,9 +348,35 @@ __global__ void vsg_pack_kernel(const double *fields, int n, int *ids, float4 *r
 }
 
 // One subset per thread (see launch_vsg_strain).
-__global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fields, double *vsg, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Candidate records of the radius scan: straight from the cell-ordered list,
+// or from a workgroup's LDS copy of the three cell-row ranges it needs.
+struct VsgGlobalReader {
+  const float4 *rec;
+  const int *ids;
+  __device__ float4 at(int, int k) const { return rec[k]; }
+  __device__ int id(int, int k) const { return ids[k]; }
+};
+struct VsgTileReader {
+  const float4 *rec;  // LDS
+  const int *ids;     // LDS
+  int row0;           // cell row of range 0
+  int glo[3], loff[3];
+  __device__ int slot(int cy, int k) const {
+    const int r = cy - row0;
+    return loff[r] + (k - glo[r]);
+  }
+  __device__ float4 at(int cy, int k) const { return rec[slot(cy, k)]; }
+  __device__ int id(int cy, int k) const { return ids[slot(cy, k)]; }
+};
+
+// Strain of subset i (see launch_vsg_strain); radius candidates through R.
+// L lanes of one wave per subset (L = 1, 2, 4, ...; lane `part` of them): the
+// radius scan deals the candidates of each cell out round-robin and the lanes'
+// sums are folded by butterfly shuffles; part 0 finishes.  Every lane of a
+// group must reach the fold.
+template <int L, class Reader>
+__device__ void vsg_subset(const VsgGrid &g, float radius, int knn, double *fields, double *vsg, int n, int i,
+                           int part, const Reader &R) {
   auto F = [&](int s, int k) -> double & { return fields[(size_t)s * n + k]; };
   const double xi = F(F_COORD_X, i), yi = F(F_COORD_Y, i);
   const float xf = (float)xi, yf = (float)yi;
@@ -373,10 +399,10 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
     for (int cy = max(cj - 1, 0); cy <= min(cj + 1, g.ncy - 1); ++cy)
       for (int cx = max(ci - 1, 0); cx <= min(ci + 1, g.ncx - 1); ++cx) {
         const int c = cy * g.ncx + cx;
-        for (int k = g.cell_begin[c]; k < g.cell_end[c]; ++k) {
-          const float4 q = g.rec[k];
+        for (int k = g.cell_begin[c] + part; k < g.cell_end[c]; k += L) {
+          const float4 q = R.at(cy, k);
           const float ddx = q.x - xf, ddy = q.y - yf;
-          if (ddx * ddx + ddy * ddy < r2 && g.cell_ids[k] >= 0) add(q.z, q.w, ddx, ddy);
+          if (ddx * ddx + ddy * ddy < r2 && R.id(cy, k) >= 0) add(q.z, q.w, ddx, ddy);
         }
       }
   } else if (F(F_SIGMA, i) >= 0.0) {
@@ -420,6 +446,17 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
         add((float)F(F_DISP_X, j), (float)F(F_DISP_Y, j), (float)F(F_COORD_X, j) - xf, (float)F(F_COORD_Y, j) - yf);
     }
   }
+  if constexpr (L > 1) {
+    auto fold = [](double &v) {
+#pragma unroll
+      for (int o = L / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    };
+    fold(s1); fold(sx); fold(sy); fold(sxx); fold(sxy); fold(syy);
+    fold(su); fold(sxu); fold(syu); fold(sv); fold(sxv); fold(syv);
+#pragma unroll
+    for (int o = L / 2; o >= 1; o >>= 1) valid += __shfl_xor(valid, o);
+    if (part != 0) return;
+  }
   double out[kVsgFields] = {0, 0, 0, 0, 0, 0, 0};
   bool ok = valid >= 3;
   if (ok) {
@@ -448,6 +485,85 @@ __global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fiel
   }
   for (int k = 0; k < kVsgFields; ++k) vsg[(size_t)k * n + i] = out[k];
   if (!ok) F(F_MATCH, i) = -1.0;
+}
+
+// k-nearest windows: one subset per thread, in subset order.
+__global__ void vsg_strain_kernel(VsgGrid g, float radius, int knn, double *fields, double *vsg, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vsg_subset<1>(g, radius, knn, fields, vsg, n, i, 0, VsgGlobalReader{g.rec, g.cell_ids});
+}
+
+__device__ int lower_bound_int(const int *a, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (a[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// Radius windows: kVsgLanes lanes per subset; workgroup b takes entries
+// [64 b, 64 b + 64) of the cell-ordered list.  When they lie in one row of cells, every candidate they
+// need lies in three contiguous ranges of the list (the cell rows above, at and
+// below, from one column left of the first cell to one right of the last),
+// which the workgroup copies to LDS once; otherwise, or when the ranges exceed
+// kVsgTileEntries, its threads read the list directly.  Both give the same
+// sums in the same order.
+constexpr int kVsgTileEntries = 2048;
+#ifndef DICE_VSG_LANES
+#define DICE_VSG_LANES 4
+#endif
+constexpr int kVsgLanes = DICE_VSG_LANES;
+constexpr int kVsgGroupSubsets = 256 / kVsgLanes;
+__global__ __launch_bounds__(256) void vsg_strain_tiled_kernel(VsgGrid g, float radius, double *fields, double *vsg,
+                                                               int n) {
+  __shared__ float4 srec[kVsgTileEntries];
+  __shared__ int sid[kVsgTileEntries];
+  __shared__ int sh[6];  // start and length of the three ranges
+  const int k0 = blockIdx.x * kVsgGroupSubsets;
+  const int k1 = min(k0 + kVsgGroupSubsets, n);
+  const int cfirst = g.cell_keys[k0], clast = g.cell_keys[k1 - 1];
+  const int cj = cfirst / g.ncx;
+  bool tiled = false;
+  if (clast / g.ncx == cj) {  // uniform over the workgroup
+    if (threadIdx.x < 3) {
+      const int cy = cj - 1 + (int)threadIdx.x;
+      int lo = 0, hi = 0;
+      if (cy >= 0 && cy < g.ncy) {
+        const int a = cy * g.ncx + max(cfirst - cj * g.ncx - 1, 0);
+        const int b = cy * g.ncx + min(clast - cj * g.ncx + 1, g.ncx - 1) + 1;
+        lo = lower_bound_int(g.cell_keys, n, a);
+        hi = lower_bound_int(g.cell_keys, n, b);
+      }
+      sh[threadIdx.x] = lo;
+      sh[3 + threadIdx.x] = hi - lo;
+    }
+    __syncthreads();
+    tiled = sh[3] + sh[4] + sh[5] <= kVsgTileEntries;
+    if (tiled) {
+      int off = 0;
+      for (int r = 0; r < 3; ++r) {
+        for (int t = threadIdx.x; t < sh[3 + r]; t += blockDim.x) {
+          srec[off + t] = g.rec[sh[r] + t];
+          sid[off + t] = g.cell_ids[sh[r] + t];
+        }
+        off += sh[3 + r];
+      }
+    }
+    __syncthreads();
+  }
+  const int k = k0 + (int)threadIdx.x / kVsgLanes, part = (int)threadIdx.x % kVsgLanes;
+  if (k >= n) return;  // whole groups of lanes
+  const int e = g.cell_ids[k];
+  const int i = e < 0 ? ~e : e;
+  if (tiled) {
+    const VsgTileReader R{srec, sid, cj - 1, {sh[0], sh[1], sh[2]}, {0, sh[3], sh[3] + sh[4]}};
+    vsg_subset<kVsgLanes>(g, radius, 0, fields, vsg, n, i, part, R);
+  } else {
+    vsg_subset<kVsgLanes>(g, radius, 0, fields, vsg, n, i, part, VsgGlobalReader{g.rec, g.cell_ids});
+  }
 }
 
 __global__ void uncertainty_kernel(const double *fields, double *out, int n) {
@@ -507,6 +623,7 @@ hipError_t launch_vsg_grid(const double *fields, int n, VsgGrid &g, const VsgWor
   g.cell_begin = w.cell_range;
   g.cell_end = w.cell_range + ncells;
   g.cell_ids = w.ids_sorted;
+  g.cell_keys = w.keys_sorted;
   g.rec = w.rec;
   return hipGetLastError();
 }
@@ -515,7 +632,12 @@ hipError_t launch_vsg_strain(const VsgGrid &g, float radius, int knn, double *fi
                              hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   if (knn < 0 || knn > kVsgMaxK) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(vsg_strain_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, g, radius, knn, fields, vsg, n);
+  if (knn == 0)
+    hipLaunchKernelGGL(vsg_strain_tiled_kernel, dim3((n + kVsgGroupSubsets - 1) / kVsgGroupSubsets), dim3(256), 0,
+                       stream, g, radius, fields, vsg, n);
+  else
+    hipLaunchKernelGGL(vsg_strain_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, g, radius, knn, fields, vsg,
+                       n);
   return hipGetLastError();
 }
 
