@@ -119,6 +119,51 @@ def vsg_deck():
         json.dump(deck, f, indent=1)
 
 
+def unicode_deck():
+    """tests/regression/unicode_filename: the image ../../component/images/\u00e6b\u00e4rn\u0243 0.tif (8-bit grey,
+    uncompressed) correlated against itself on the regular grid of 15-px subsets at step 50 (GRADIENT_BASED,
+    all six parameters, USE_FIELD_VALUES), comma-separated output without row ids; the image is stored as its
+    pixel values in images.npz and the test writes it back under its non-ASCII name."""
+    import unicodedata
+
+    import numpy as np
+    from PIL import Image
+    name = "unicode_filename"
+    src, dst = os.path.join(SRC, name), os.path.join(DST, name)
+    os.makedirs(os.path.join(dst, "gold"), exist_ok=True)
+    inp = top_params(ET.parse(os.path.join(src, "input.xml")).getroot())
+    cor_root = ET.parse(os.path.join(src, inp["correlation_parameters_file"])).getroot()
+    cor = top_params(cor_root)
+    image = inp["image_file_prefix"] + inp["reference_image_index"] + inp["image_file_extension"]
+    # input.xml spells the prefix in decomposed form (a + U+0308), as the gold's header prints it; the file on
+    # disk has the composed form
+    im = Image.open(os.path.join(src, inp["image_folder"], unicodedata.normalize("NFC", image)))
+    assert im.mode == "L"
+    np.savez_compressed(os.path.join(dst, "images.npz"), frame_0=np.asarray(im, dtype=np.uint8))
+    spec = cor_root.find("ParameterList[@name='output_spec']")
+    fields = sorted(spec.findall("Parameter"), key=lambda q: int(q.get("value")))
+    deck = {"subset_size": int(inp["subset_size"]), "step_size": int(inp["step_size"]), "separate": False,
+            "prefix": inp.get("output_prefix", "DICe_solution"), "image_file": image,
+            "reference_image": inp["image_folder"] + image, "deformed_image": inp["image_folder"] + image,
+            "deformed_images": [inp["image_folder"] + image], "frames": 1, "images": "images.npz",
+            "omit_row_id": cor.get("omit_output_row_id") == "true", "delimiter": cor.get("output_delimiter", " "),
+            "fields": [q.get("name") for q in fields], "gold": ["DICe_solution_0.txt"],
+            "diff_tolerance": 0.05, "diff_floor": 5.0e-5}
+    for k in ("enable_rotation", "enable_normal_strain", "enable_shear_strain"):
+        deck[k] = 1 if cor.get(k) == "true" else 0
+    for key, names in ENUMS.items():
+        if key in cor:
+            deck[key] = names.index(cor[key].upper())
+    for g in deck["gold"]:
+        shutil.copy(os.path.join(src, "gold", g), os.path.join(dst, "gold", g))
+    for root, _, files in os.walk(dst):
+        for f in files:
+            os.chmod(os.path.join(root, f), 0o644)
+    with open(os.path.join(dst, "deck.json"), "w", encoding="utf-8") as f:
+        json.dump(deck, f, indent=1, ensure_ascii=False)
+
+
 if __name__ == "__main__":
     main()
     vsg_deck()
+    unicode_deck()

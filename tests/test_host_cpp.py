@@ -566,3 +566,41 @@ def test_regression_deck(tmp_path, name, backend):
                 if col in header:
                     k = header.index(col)
                     assert float(orow[k]) == float(gr[k]), (g, col)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
+def test_unicode_filename_deck(tmp_path, backend):
+    """tests/regression/unicode_filename end to end through the Schema mirror: the image read by its
+    non-ASCII file name (written back from the fixture under the name input.xml spells, which the gold's
+    header prints) and correlated against itself on the regular grid of 15-px subsets at step 50, six
+    parameters, comma-separated output without row ids -- dice_diff against the gold with the suite's options;
+    the displacements, which the gold holds at the 1e-14 level, within 1e-6 px of zero."""
+    import json
+    ddir = os.path.join(DECKS, "unicode_filename")
+    deck = json.load(open(os.path.join(ddir, "deck.json"), encoding="utf-8"))
+    img = np.load(os.path.join(ddir, deck["images"]))["frame_0"]
+    h, w = img.shape
+    path = tmp_path / deck["image_file"]
+    write_tiff_u8(path, img)
+    cfg = {"ref_file": str(path), "def_file": str(path), "def_files": str(path), "width": w, "height": h,
+           "subset_size": deck["subset_size"], "step_size": deck["step_size"], "out_dir": tmp_path,
+           "prefix": deck["prefix"], "separate": 0, "backend": backend, "ref_name": deck["reference_image"],
+           "def_name": deck["deformed_image"], "def_names": ",".join(deck["deformed_images"]), "routine": 0,
+           "fields": ",".join(deck["fields"]), "optimization": deck["optimization_method"],
+           "interpolation": deck["interpolation_method"], "initialization": deck["initialization_method"],
+           "enable_rotation": deck["enable_rotation"], "enable_normal_strain": deck["enable_normal_strain"],
+           "enable_shear_strain": deck["enable_shear_strain"], "omit_row_id": int(deck["omit_row_id"]),
+           "delimiter": deck["delimiter"], "frames": deck["frames"]}
+    cfg_path = tmp_path / "deck.cfg"
+    cfg_path.write_text("".join("%s %s\n" % kv for kv in cfg.items()), encoding="utf-8")
+    r = run("deck", cfg_path)
+    assert r.returncode == 0, r.stderr
+    for gname in deck["gold"]:
+        gold = os.path.join(ddir, "gold", gname)
+        code, out = dice_diff(gold, tmp_path / gname, "-t", deck["diff_tolerance"], "-f", deck["diff_floor"], "-v")
+        assert code == 0 and "TEST PASSED" in out, (gname, out)
+        g, ours = gold_csv(gold), gold_csv(tmp_path / gname)
+        assert g.shape == ours.shape == (100, 4)
+        assert np.array_equal(ours[:, :2], g[:, :2])
+        assert np.abs(ours[:, 2:4]).max() <= 1e-6
