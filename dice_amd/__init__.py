@@ -90,7 +90,7 @@ EXPORTS = [
     "dice_hip_autotune", "dice_hip_get_launch_shape", "dice_hip_set_launch_shape",
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
     "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
-    "dice_hip_uncertainty", "dice_hip_uncertainty_device", "dice_hip_objective",
+    "dice_hip_uncertainty", "dice_hip_uncertainty_device", "dice_hip_objective", "dice_hip_pass_counts",
 ]
 
 _lib = None
@@ -149,6 +149,7 @@ def lib():
         L.dice_hip_gamma_adjoint.argtypes = [P, I, P, P, P, P]
         L.dice_hip_gauss_filter.argtypes = [I, I, I, I, P, P, I]
         L.dice_hip_neighbor_rounds.argtypes = [P]
+        L.dice_hip_pass_counts.argtypes = [P, P]
         L.dice_hip_vsg_strain.argtypes = [P, I, P, P]
         L.dice_hip_vsg_strain_device.argtypes = [P, I, P, P]
         L.dice_hip_uncertainty.argtypes = [P, P, P]
@@ -323,6 +324,13 @@ class Context:
     def neighbor_rounds(self):
         """Correlation rounds of the last frame with neighbour initialization (0 otherwise)."""
         return self._lib.dice_hip_neighbor_rounds(self.h)
+
+    def pass_counts(self):
+        """dice_hip_pass_counts: (fallback list, overflow of the primary pass, overflow of the fallback pass)
+        of the last correlation launch."""
+        c = np.zeros(3, dtype=np.int32)
+        self._check(self._lib.dice_hip_pass_counts(self.h, _p(c)), "pass_counts")
+        return tuple(int(v) for v in c)
 
     def synchronize(self):
         self._check(self._lib.dice_hip_synchronize(self.h), "synchronize")

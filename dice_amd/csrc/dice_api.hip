@@ -97,6 +97,7 @@ struct dice_hip_context {
   bool packed = false;  // conformal pixels also packed into pxy
   // fields + work
   DevBuf fields, work;
+  int *last_work = nullptr;     // work area of the last correlation launch
   // launch shape of the batched mode
   Launch batched{1, 4, 0, kDefaultSlotFloats};
   std::map<std::tuple<int, int, int, int, int>, Launch> tuned;
@@ -251,7 +252,8 @@ hipEvent_t *next_event_pair(dice_hip_context *ctx) {
 }
 
 int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int first_frame, bool capturing,
-                      const double *guess = nullptr, const int *items = nullptr, const int *item_count = nullptr) {
+                      const double *guess = nullptr, const int *items = nullptr, const int *item_count = nullptr,
+                      int *work = nullptr) {
   DImage ref, def;
   DSubsets ss;
   fill_views(ctx, ref, def, ss);
@@ -260,7 +262,7 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
   const Launch L = current_launch(ctx, mode);
   // per-launch timing events on the context stream (eager launches only; a captured
   // graph is timed as a whole by the caller)
-  const bool timed = ctx->timing && !capturing;
+  const bool timed = ctx->timing && !capturing && !work;
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if (timed) {
     hipEvent_t *p = next_event_pair(ctx);
@@ -268,7 +270,8 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
     ev = ctx->events[ctx->events_used - 1];
     HIP_TRY(ctx, hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
   }
-  HIP_TRY(ctx, dice_hip::launch_correlate(L, ctx->rstorage, kp, ref, def, ss, d_fields, ctx->work.as<int>(), ctx->stream,
+  ctx->last_work = work ? work : ctx->work.as<int>();
+  HIP_TRY(ctx, dice_hip::launch_correlate(L, ctx->rstorage, kp, ref, def, ss, d_fields, ctx->last_work, ctx->stream,
                                           items, item_count),
           "correlation launch");
   if (timed) HIP_TRY(ctx, hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
@@ -570,6 +573,7 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     ctx->S = 0;
   }
   ctx->n = n;
+  ctx->last_work = nullptr;  // the work area may move
   HIP_TRY(ctx, ctx->work.ensure(dice_hip::work_ints(n) * sizeof(int)), "hipMalloc work");
   HIP_TRY(ctx, ctx->fields.ensure((size_t)DICE_F_COUNT * n * sizeof(double)), "hipMalloc fields");
   return DICE_HIP_OK;
@@ -655,6 +659,21 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
 }
 
 int dice_hip_neighbor_rounds(dice_hip_context *ctx) { return ctx ? ctx->nb_rounds : DICE_HIP_EINVAL; }
+
+int dice_hip_pass_counts(dice_hip_context *ctx, int *counts) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (!counts) return ctx->fail(DICE_HIP_EINVAL, "null counts");
+  counts[0] = counts[1] = counts[2] = 0;
+  if (!ctx->last_work) return DICE_HIP_OK;
+  (void)hipSetDevice(ctx->device);
+  int h[dice_hip::kWorkHeader];
+  HIP_TRY(ctx, hipMemcpyAsync(h, ctx->last_work, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy counts");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "pass counts");
+  counts[0] = h[dice_hip::kWorkFbCount];
+  counts[1] = h[dice_hip::kWorkOv0Count];
+  counts[2] = h[dice_hip::kWorkOv1Count];
+  return DICE_HIP_OK;
+}
 
 // VSG strains: the subsets binned on the host into a uniform grid of cell side
 // max(r, 1) px (sorted by cell, then id), uploaded with the offsets, then one

@@ -212,6 +212,14 @@ int dice_hip_uncertainty_device(dice_hip_context *ctx, const double *device_fiel
  * the subsets in order, once each: a diagnostic of this library's rounds. */
 int dice_hip_neighbor_rounds(dice_hip_context *ctx);
 
+/* Work-list sizes of the last correlation launch of the context (waits for the
+ * stream): counts[0] subsets handed to the fallback optimizer pass
+ * (GRADIENT_BASED_THEN_SIMPLEX / SIMPLEX_THEN_GRADIENT_BASED), counts[1] and
+ * counts[2] subsets whose deformed window outgrew the BATCHED wave's LDS slot in
+ * the primary and the fallback pass and were solved by the PLAIN kernel.  No
+ * counterpart in the reference: a diagnostic of this library's launches. */
+int dice_hip_pass_counts(dice_hip_context *ctx, int *counts);
+
 /* A multi-frame sequence: the CLI frame loop of execute_correlation calls
  * (DICe_Main.cpp:324-374) for n_frames deformed frames already in device
  * memory, each initialised from the previous frame's fields (USE_FIELD_VALUES
