@@ -490,6 +490,111 @@ int lu_inverse(int n, double A[6][6]) {
 }
 
 // ------------------------------------------------------------- Objective ---
+// Stop rules of nelder_mead (bits of *stop).
+enum { NM_EARLY = 1, NM_RTOL = 2, NM_RTOL_DIFF = 4, NM_REPEATS = 8, NM_MAX_ITERATIONS = 16 };
+
+// Simplex::minimize, DICe_Simplex.cpp:60-277, over nd variables with a generic
+// objective f(variables).  As in the reference the variables array is the one
+// the objective reads: every evaluation first writes its point into it, so on
+// exit it holds the LAST evaluated point, not necessarily the best vertex.
+// stop (may be null) receives the NM_* bits of the rules that ended the loop.
+template <typename S, typename F>
+int nelder_mead(int nd, S *var, const S *deltas, F &&f, int max_iterations, double tolerance, S threshold,
+                int &num_iterations, int *stop) {
+  const S tiny = 1.0E-10;
+  if (stop) *stop = 0;
+  S init[6], pts[7][6], gv[7], sums[6], ptry[6];
+  auto eval = [&](const S *pt) {
+    for (int j = 0; j < nd; ++j) var[j] = pt[j];
+    return f(var);
+  };
+  for (int i = 0; i < nd; ++i) init[i] = var[i];
+  const int mpts = nd + 1;
+  for (int i = 0; i < mpts; ++i) {
+    for (int j = 0; j < nd; ++j) pts[i][j] = var[j];
+    if (i > 0) pts[i][i - 1] += deltas[i - 1];
+    gv[i] = eval(pts[i]);
+    if (i == 0 && gv[i] < threshold && gv[i] >= 0.0) {
+      num_iterations = 0;
+      if (stop) *stop = NM_EARLY;
+      return CORRELATION_SUCCESSFUL;
+    }
+    for (int j = 0; j < nd; ++j) var[j] = init[j];
+  }
+  for (int j = 0; j < nd; ++j) {
+    S s = 0.0;
+    for (int i = 0; i < mpts; ++i) s += pts[i][j];
+    sums[j] = s;
+  }
+  S old_rtol = 0.0, gamma_new = 0.0, gamma_old = -1.0;
+  int repeats = 0;
+  const int repeats_allowed = 15;
+  const S gamma_tol = 1.0E-8;
+  // the trial point sums * (1 - fac) / nd - pts[ihi] * ((1 - fac) / nd - fac), evaluated;
+  // it replaces the worst vertex when better
+  auto trial = [&](int ihi, S fac) {
+    const S fac1 = (1.0 - fac) / nd, fac2 = fac1 - fac;
+    for (int j = 0; j < nd; j++) ptry[j] = sums[j] * fac1 - pts[ihi][j] * fac2;
+    const S ytry = eval(ptry);
+    if (ytry < gv[ihi]) {
+      gv[ihi] = ytry;
+      for (int j = 0; j < nd; j++) { sums[j] += ptry[j] - pts[ihi][j]; pts[ihi][j] = ptry[j]; }
+    }
+    return ytry;
+  };
+  int iteration = 0;
+  for (iteration = 0; iteration < max_iterations; iteration++) {
+    if (iteration >= max_iterations - 1) {
+      num_iterations = iteration;
+      if (stop) *stop = NM_MAX_ITERATIONS;
+      return MAX_ITERATIONS_REACHED;
+    }
+    int inhi;
+    int ilo = 0;
+    int ihi = gv[0] > gv[1] ? (inhi = 1, 0) : (inhi = 0, 1);
+    for (int i = 0; i < mpts; i++) {
+      if (gv[i] <= gv[ilo]) ilo = i;
+      if (gv[i] > gv[ihi]) { inhi = ihi; ihi = i; }
+      else if (gv[i] > gv[inhi] && i != ihi) inhi = i;
+    }
+    const S rtol = 2.0 * std::abs(gv[ihi] - gv[ilo]) / (std::abs(gv[ihi]) + std::abs(gv[ilo]) + tiny);
+    const S rtol_diff = std::abs(rtol - old_rtol);
+    old_rtol = rtol;
+    if (std::abs(gamma_new - gamma_old) < gamma_tol) repeats++;
+    gamma_old = gamma_new;
+    if (rtol < tolerance || rtol_diff < tolerance || repeats > repeats_allowed) {
+      if (stop) *stop = (rtol < tolerance ? NM_RTOL : 0) | (rtol_diff < tolerance ? NM_RTOL_DIFF : 0) |
+                        (repeats > repeats_allowed ? NM_REPEATS : 0);
+      std::swap(gv[0], gv[ilo]);
+      for (int i = 0; i < nd; i++) std::swap(pts[0][i], pts[ilo][i]);
+      break;
+    }
+    S ytry = trial(ihi, -1.0);       // reflect
+    if (ytry <= gv[ilo]) {
+      trial(ihi, 2.0);               // expand
+    } else if (ytry >= gv[inhi]) {
+      const S ysave = gv[ihi];
+      ytry = trial(ihi, 0.5);        // contract
+      if (ytry >= ysave) {           // shrink toward the best vertex
+        for (int i = 0; i < mpts; i++) {
+          if (i != ilo) {
+            for (int j = 0; j < nd; j++) pts[i][j] = sums[j] = 0.5 * (pts[i][j] + pts[ilo][j]);
+            gv[i] = eval(sums);
+          }
+        }
+        for (int j = 0; j < nd; j++) {
+          S s = 0.0;
+          for (int i = 0; i < mpts; i++) s += pts[i][j];
+          sums[j] = s;
+        }
+      }
+    }
+    gamma_new = f(var);  // the extra evaluation at the last point (DICe_Simplex.cpp:264)
+  }
+  num_iterations = iteration;
+  return CORRELATION_SUCCESSFUL;
+}
+
 template <typename S, typename T>
 struct Objective {
   const oracle_params *P;
@@ -633,99 +738,9 @@ struct Objective {
   // The variables are the shape-function parameters themselves, so on exit
   // they hold the LAST evaluated point, not necessarily the best vertex.
   int update_robust(Affine<S> &sf, int &num_iterations) {
-    const int nd = sf.n;
-    const int max_iterations = P->max_solver_iterations_robust;
-    const double tolerance = P->robust_solver_tolerance;
-    const S tiny = 1.0E-10;
-    const S threshold = P->skip_solve_gamma_threshold;
-    S init[6], pts[7][6], gv[7], sums[6], ptry[6];
-    for (int i = 0; i < nd; ++i) init[i] = sf.p[i];
-    const int mpts = nd + 1;
-    for (int i = 0; i < mpts; ++i) {
-      for (int j = 0; j < nd; ++j) pts[i][j] = sf.p[j];
-      if (i > 0) pts[i][i - 1] += sf.deltas[i - 1];
-      for (int j = 0; j < nd; ++j) sf.p[j] = pts[i][j];
-      gv[i] = gamma(sf);
-      if (i == 0 && gv[i] < threshold && gv[i] >= 0.0) { num_iterations = 0; return CORRELATION_SUCCESSFUL; }
-      for (int j = 0; j < nd; ++j) sf.p[j] = init[j];
-    }
-    for (int j = 0; j < nd; ++j) {
-      S s = 0.0;
-      for (int i = 0; i < mpts; ++i) s += pts[i][j];
-      sums[j] = s;
-    }
-    S old_rtol = 0.0, gamma_new = 0.0, gamma_old = -1.0;
-    int repeats = 0;
-    const int repeats_allowed = 15;
-    const S gamma_tol = 1.0E-8;
-    int iteration = 0;
-    for (iteration = 0; iteration < max_iterations; iteration++) {
-      if (iteration >= max_iterations - 1) { num_iterations = iteration; return MAX_ITERATIONS_REACHED; }
-      int inhi;
-      int ilo = 0;
-      int ihi = gv[0] > gv[1] ? (inhi = 1, 0) : (inhi = 0, 1);
-      for (int i = 0; i < mpts; i++) {
-        if (gv[i] <= gv[ilo]) ilo = i;
-        if (gv[i] > gv[ihi]) { inhi = ihi; ihi = i; }
-        else if (gv[i] > gv[inhi] && i != ihi) inhi = i;
-      }
-      const S rtol = 2.0 * std::abs(gv[ihi] - gv[ilo]) / (std::abs(gv[ihi]) + std::abs(gv[ilo]) + tiny);
-      const S rtol_diff = std::abs(rtol - old_rtol);
-      old_rtol = rtol;
-      if (std::abs(gamma_new - gamma_old) < gamma_tol) repeats++;
-      gamma_old = gamma_new;
-      if (rtol < tolerance || rtol_diff < tolerance || repeats > repeats_allowed) {
-        std::swap(gv[0], gv[ilo]);
-        for (int i = 0; i < nd; i++) std::swap(pts[0][i], pts[ilo][i]);
-        break;
-      }
-      S ytry, fac, fac1, fac2;
-      fac = -1.0; fac1 = (1.0 - fac) / nd; fac2 = fac1 - fac;
-      for (int j = 0; j < nd; j++) ptry[j] = sums[j] * fac1 - pts[ihi][j] * fac2;
-      for (int j = 0; j < nd; ++j) sf.p[j] = ptry[j];
-      ytry = gamma(sf);
-      if (ytry < gv[ihi]) {
-        gv[ihi] = ytry;
-        for (int j = 0; j < nd; j++) { sums[j] += ptry[j] - pts[ihi][j]; pts[ihi][j] = ptry[j]; }
-      }
-      if (ytry <= gv[ilo]) {
-        fac = 2.0; fac1 = (1.0 - fac) / nd; fac2 = fac1 - fac;
-        for (int j = 0; j < nd; j++) ptry[j] = sums[j] * fac1 - pts[ihi][j] * fac2;
-        for (int j = 0; j < nd; ++j) sf.p[j] = ptry[j];
-        ytry = gamma(sf);
-        if (ytry < gv[ihi]) {
-          gv[ihi] = ytry;
-          for (int j = 0; j < nd; j++) { sums[j] += ptry[j] - pts[ihi][j]; pts[ihi][j] = ptry[j]; }
-        }
-      } else if (ytry >= gv[inhi]) {
-        const S ysave = gv[ihi];
-        fac = 0.5; fac1 = (1.0 - fac) / nd; fac2 = fac1 - fac;
-        for (int j = 0; j < nd; j++) ptry[j] = sums[j] * fac1 - pts[ihi][j] * fac2;
-        for (int j = 0; j < nd; ++j) sf.p[j] = ptry[j];
-        ytry = gamma(sf);
-        if (ytry < gv[ihi]) {
-          gv[ihi] = ytry;
-          for (int j = 0; j < nd; j++) { sums[j] += ptry[j] - pts[ihi][j]; pts[ihi][j] = ptry[j]; }
-        }
-        if (ytry >= ysave) {
-          for (int i = 0; i < mpts; i++) {
-            if (i != ilo) {
-              for (int j = 0; j < nd; j++) pts[i][j] = sums[j] = 0.5 * (pts[i][j] + pts[ilo][j]);
-              for (int j = 0; j < nd; ++j) sf.p[j] = sums[j];
-              gv[i] = gamma(sf);
-            }
-          }
-          for (int j = 0; j < nd; j++) {
-            S s = 0.0;
-            for (int i = 0; i < mpts; i++) s += pts[i][j];
-            sums[j] = s;
-          }
-        }
-      }
-      gamma_new = gamma(sf);
-    }
-    num_iterations = iteration;
-    return CORRELATION_SUCCESSFUL;
+    auto objective = [&](const S *) { return gamma(sf); };
+    return nelder_mead<S>(sf.n, sf.p, sf.deltas, objective, P->max_solver_iterations_robust,
+                          P->robust_solver_tolerance, (S)P->skip_solve_gamma_threshold, num_iterations, nullptr);
   }
 };
 
@@ -1299,6 +1314,31 @@ int oracle_uncertainty(int n, const double *F, double *out) {
     out[i] = sig < 0.0f ? 0.0f : (angle == 0.0f ? 0.0f : (float)(1.0 / (double)angle * (double)sig));
   }
   return 0;
+}
+
+int oracle_nelder_mead(int kind, int nd, double *p, const double *deltas, const double *coef, int max_iterations,
+                       double tolerance, double threshold, int cap, double *trace, int *n_evals, int *num_iterations,
+                       int *stop) {
+  if (nd < 1 || nd > 6 || !p || !deltas || !n_evals || !num_iterations || (kind == 0 && !coef) || kind < 0 ||
+      kind > 1)
+    return -1;
+  int evals = 0;
+  auto f = [&](const double *v) {
+    double r = 0.0;
+    for (int i = 0; i < nd; ++i) {
+      const double d = kind == 0 ? v[i] - coef[nd + i] : v[i] * v[i] - 1.0;
+      r += (kind == 0 ? coef[i] : 1.0) * d * d;
+    }
+    if (trace && evals < cap) {
+      for (int i = 0; i < nd; ++i) trace[(size_t)evals * (nd + 1) + i] = v[i];
+      trace[(size_t)evals * (nd + 1) + nd] = r;
+    }
+    ++evals;
+    return r;
+  };
+  const int status = nelder_mead<double>(nd, p, deltas, f, max_iterations, tolerance, threshold, *num_iterations, stop);
+  *n_evals = evals;
+  return status;
 }
 
 int oracle_lu_inverse(int n, double *a) {

@@ -157,6 +157,17 @@ int oracle_objective_d(const oracle_params *p, int w, int h, int storage, const 
  * out[n + i] = FIELD_1, in float.  Returns 0 or -1. */
 int oracle_uncertainty(int n, const double *fields, double *out);
 
+/* Simplex::minimize (DICe_Simplex.cpp:60-277) on a test objective of nd <= 6 variables:
+ * kind 0: sum_i coef[i] (p_i - coef[nd + i])^2 (a quadratic bowl), kind 1: sum_i (p_i^2 - 1)^2
+ * (a double well).  p: start point in, the last evaluated point out.  trace (may be null): up to
+ * cap evaluations as (point, value) rows of nd + 1 doubles, in evaluation order; n_evals: all
+ * evaluations; num_iterations; stop: bits of the rules that ended it (1 gamma(p0) below the
+ * threshold, 2 rtol, 4 rtol difference, 8 repeated gamma, 16 iteration cap).  Returns the
+ * Status_Flag (0 success, 11 max iterations) or -1 on a bad argument. */
+int oracle_nelder_mead(int kind, int nd, double *p, const double *deltas, const double *coef, int max_iterations,
+                       double tolerance, double threshold, int cap, double *trace, int *n_evals, int *num_iterations,
+                       int *stop);
+
 /* 6x6 LU with partial pivoting + inverse (GETRF/GETRI semantics); returns INFO. */
 int oracle_lu_inverse(int n, double *a_rowmajor);
 

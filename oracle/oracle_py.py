@@ -82,6 +82,8 @@ def lib():
         _lib.oracle_interp_d.argtypes = [i, i, i, P, i, P, P, P]
         _lib.oracle_polygon_pixels.argtypes = [i, P, P, i, P, P]
         _lib.oracle_lu_inverse.argtypes = [i, P]
+        _lib.oracle_smooth_gradients_f.argtypes = [i, i, P, P]
+        _lib.oracle_nelder_mead.argtypes = [i, i, P, P, P, i, ctypes.c_double, ctypes.c_double, i, P, P, P, P]
         _lib.oracle_gauss_filter_f.argtypes = [i, i, i, P, i]
         _lib.oracle_gauss_filter_d.argtypes = [i, i, P, i]
         _lib.oracle_vsg_strain.argtypes = [i, P, i, P]
@@ -279,3 +281,30 @@ def lu_inverse(a):
     a = np.ascontiguousarray(a, dtype=np.float64).copy()
     info = lib().oracle_lu_inverse(a.shape[0], _p(a))
     return a, info
+
+
+def smooth_gradients(gx, gy):
+    """smooth_gradients_convolution_5_point on float32 copies of gx, gy."""
+    gx = np.ascontiguousarray(gx, dtype=np.float32).copy()
+    gy = np.ascontiguousarray(gy, dtype=np.float32).copy()
+    h, w = gx.shape
+    lib().oracle_smooth_gradients_f(w, h, _p(gx), _p(gy))
+    return gx, gy
+
+
+def nelder_mead(kind, p0, deltas, coef=None, max_iterations=1000, tolerance=1e-6, threshold=1e-10, cap=256):
+    """Simplex::minimize on a test objective (kind 0: quadratic bowl sum coef[i] (p_i - coef[nd + i])^2,
+    kind 1: double well sum (p_i^2 - 1)^2).  Returns dict(status, p (last evaluated point), iterations,
+    evals, stop (bits: 1 early, 2 rtol, 4 rtol difference, 8 repeats, 16 cap), trace (evals x nd + 1))."""
+    p = np.ascontiguousarray(p0, dtype=np.float64).copy()
+    nd = len(p)
+    d = np.ascontiguousarray(deltas, dtype=np.float64)
+    c = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    trace = np.zeros((cap, nd + 1))
+    ne, it, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    status = lib().oracle_nelder_mead(kind, nd, _p(p), _p(d), _p(c), max_iterations, tolerance, threshold, cap,
+                                      _p(trace), ctypes.byref(ne), ctypes.byref(it), ctypes.byref(st))
+    if status < 0:
+        raise ValueError("nelder_mead: bad arguments")
+    return {"status": status, "p": p, "iterations": it.value, "evals": ne.value, "stop": st.value,
+            "trace": trace[:min(ne.value, cap)]}
