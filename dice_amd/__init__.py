@@ -91,6 +91,7 @@ EXPORTS = [
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
     "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
     "dice_hip_uncertainty", "dice_hip_uncertainty_device", "dice_hip_objective", "dice_hip_pass_counts",
+    "dice_hip_correlate_device_noalloc",
 ]
 
 _lib = None
@@ -135,6 +136,7 @@ def lib():
         L.dice_hip_set_subsets.argtypes = [P, I, P, P, I, P, P, P, P]
         L.dice_hip_correlate.argtypes = [P, I, P, I]
         L.dice_hip_correlate_device.argtypes = [P, I, P, I]
+        L.dice_hip_correlate_device_noalloc.argtypes = [P, I, P, I]
         L.dice_hip_correlate_sequence.argtypes = [P, I, I, P, P, P, I]
         L.dice_hip_synchronize.argtypes = [P]
         L.dice_hip_stream.argtypes = [P]
@@ -290,6 +292,11 @@ class Context:
     def correlate_device(self, fields_ptr, mode=MODE_BATCHED, first_frame=True):
         self._check(self._lib.dice_hip_correlate_device(self.h, mode, ctypes.c_void_p(fields_ptr),
                                                         1 if first_frame else 0), "correlate_device")
+
+    def correlate_device_noalloc(self, fields_ptr, mode=MODE_BATCHED, first_frame=True):
+        """dice_hip_correlate_device_noalloc: as correlate_device, on the library's static work area."""
+        self._check(self._lib.dice_hip_correlate_device_noalloc(self.h, mode, ctypes.c_void_p(fields_ptr),
+                                                                1 if first_frame else 0), "correlate_device_noalloc")
 
     def correlate_sequence(self, frame_ptrs, fields_ptr, history_ptr=0, mode=MODE_BATCHED, use_graph=True):
         arr = (ctypes.c_void_p * len(frame_ptrs))(*frame_ptrs)

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -23,6 +24,16 @@ using dice_hip::DImage;
 using dice_hip::DSubsets;
 using dice_hip::KParams;
 using dice_hip::Launch;
+
+// Work area of dice_hip_correlate_device_noalloc: a file-scope device array of
+// work_ints(kNoAllocMaxSubsets) ints (11.5 MB), held by every GPU the library
+// runs on until the process ends.  Host code never uses the array's name as an
+// address: dice_hip_create resolves each device's copy once with
+// hipGetSymbolAddress (that device current) and the entry point passes that
+// pointer to the kernels.
+static constexpr int kNoAllocMaxSubsets = 1 << 18;
+static constexpr size_t kNoAllocWorkInts = dice_hip::kWorkHeader + (3 + dice_hip::kNmStride) * (size_t)kNoAllocMaxSubsets;
+static __device__ __align__(256) int scratch[kNoAllocWorkInts];
 
 namespace {
 
@@ -51,6 +62,24 @@ struct DevBuf {
   }
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), resolved once per device.
+hipError_t scratch_address(int device, int **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<int *>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 struct GraphKey {
   int mode, n_frames, n;
@@ -97,6 +126,7 @@ struct dice_hip_context {
   bool packed = false;  // conformal pixels also packed into pxy
   // fields + work
   DevBuf fields, work;
+  int *scratch_work = nullptr;  // this device's copy of the static work area (no-allocation entry point)
   int *last_work = nullptr;     // work area of the last correlation launch
   // launch shape of the batched mode
   Launch batched{1, 4, 0, kDefaultSlotFloats};
@@ -326,6 +356,11 @@ int dice_hip_create(dice_hip_context **out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cus = prop.multiProcessorCount;
+  if (scratch_address(device, &ctx->scratch_work) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DICE_HIP_EHIP;
+  }
   dice_hip_default_params(&ctx->params);
   ctx->have_params = true;
   *out = ctx;
@@ -656,6 +691,24 @@ int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fi
   if (neighbor_mode(ctx, first_frame)) return correlate_neighbors(ctx, mode, device_fields, first_frame);
   ctx->nb_rounds = 0;
   return enqueue_correlate(ctx, mode, device_fields, first_frame, false);
+}
+
+int dice_hip_correlate_device_noalloc(dice_hip_context *ctx, int mode, double *device_fields, int first_frame) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (mode != DICE_MODE_PLAIN && mode != DICE_MODE_BATCHED) return ctx->fail(DICE_HIP_EINVAL, "unknown mode");
+  if (!device_fields) return ctx->fail(DICE_HIP_EINVAL, "null fields");
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n > kNoAllocMaxSubsets)
+    return ctx->fail(DICE_HIP_EINVAL, "dice_hip_correlate_device_noalloc: more subsets than its static work area "
+                                      "holds (262144)");
+  if (neighbor_mode(ctx, first_frame))
+    return ctx->fail(DICE_HIP_EINVAL, "dice_hip_correlate_device_noalloc: USE_NEIGHBOR_VALUES needs the buffers of "
+                                      "dice_hip_correlate_device");
+  if (!ctx->scratch_work) return ctx->fail(DICE_HIP_ESTATE, "static work area not resolved");
+  (void)hipSetDevice(ctx->device);
+  ctx->nb_rounds = 0;
+  return enqueue_correlate(ctx, mode, device_fields, first_frame, false, nullptr, nullptr, nullptr, ctx->scratch_work);
 }
 
 int dice_hip_neighbor_rounds(dice_hip_context *ctx) { return ctx ? ctx->nb_rounds : DICE_HIP_EINVAL; }

@@ -162,6 +162,20 @@ int dice_hip_correlate(dice_hip_context *ctx, int mode, double *fields, int firs
  * initialization, whose rounds wait on the stream between launches. */
 int dice_hip_correlate_device(dice_hip_context *ctx, int mode, double *device_fields, int first_frame);
 
+/* dice_hip_correlate_device without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory at this call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The correlation passes' work area (work-stealing counters,
+ * fallback and overflow lists, Nelder-Mead results) is a static device array
+ * of the library, one per GPU, sized for at most 262,144 subsets (11.5 MB, held
+ * until the process ends); the call makes no hipMalloc / hipFree.  The counters
+ * are cleared by a kernel at each call.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (several contexts or streams) race.
+ * More subsets, or USE_NEIGHBOR_VALUES initialization (which needs the
+ * context's own buffers), give DICE_HIP_EINVAL.  Enqueued on the context
+ * stream; results are those of dice_hip_correlate_device. */
+int dice_hip_correlate_device_noalloc(dice_hip_context *ctx, int mode, double *device_fields, int first_frame);
+
 /* VSG_Strain_Post_Processor::execute (DICe_PostProcessor.cpp:877-1136): for each
  * subset, a least-squares plane of u and v over the subsets within
  * window_size / 2 px (window_size > 0), or over its -window_size nearest subsets
