@@ -91,7 +91,8 @@ EXPORTS = [
     "dice_hip_correlate_points", "dice_hip_gamma", "dice_hip_gamma_adjoint", "dice_hip_gauss_filter",
     "dice_hip_neighbor_rounds", "dice_hip_vsg_strain", "dice_hip_vsg_strain_device",
     "dice_hip_uncertainty", "dice_hip_uncertainty_device", "dice_hip_objective", "dice_hip_pass_counts",
-    "dice_hip_correlate_device_noalloc",
+    "dice_hip_correlate_device_noalloc", "dice_hip_set_force_simplex", "dice_hip_set_subset_masks",
+    "dice_hip_set_image_offsets",
 ]
 
 _lib = None
@@ -152,6 +153,9 @@ def lib():
         L.dice_hip_gauss_filter.argtypes = [I, I, I, I, P, P, I]
         L.dice_hip_neighbor_rounds.argtypes = [P]
         L.dice_hip_pass_counts.argtypes = [P, P]
+        L.dice_hip_set_force_simplex.argtypes = [P, P]
+        L.dice_hip_set_subset_masks.argtypes = [P, P, P, P]
+        L.dice_hip_set_image_offsets.argtypes = [P, I, I, I, I]
         L.dice_hip_vsg_strain.argtypes = [P, I, P, P]
         L.dice_hip_vsg_strain_device.argtypes = [P, I, P, P]
         L.dice_hip_uncertainty.argtypes = [P, P, P]
@@ -283,6 +287,25 @@ class Context:
         self._check(self._lib.dice_hip_set_subsets(self.h, len(cx), _p(cx), _p(cy), subset_size, _p(co), _p(px),
                                                    _p(py), _p(pa)), "set_subsets")
         self.n = len(cx)
+
+    def set_force_simplex(self, flags):
+        """dice_hip_set_force_simplex: per-subset Nelder-Mead override (None clears)."""
+        f = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f is None or f.shape == (self.n,)
+        self._check(self._lib.dice_hip_set_force_simplex(self.h, _p(f)), "set_force_simplex")
+
+    def set_subset_masks(self, masks):
+        """dice_hip_set_subset_masks: masks = (offsets int32[n+1], x, y) of blocked image pixels, or None."""
+        if masks is None:
+            self._check(self._lib.dice_hip_set_subset_masks(self.h, None, None, None), "set_subset_masks")
+            return
+        mo, mx, my = [np.ascontiguousarray(a, dtype=np.int32) for a in masks]
+        self._check(self._lib.dice_hip_set_subset_masks(self.h, _p(mo), _p(mx), _p(my)), "set_subset_masks")
+
+    def set_image_offsets(self, ref_ox, ref_oy, def_ox, def_oy):
+        """dice_hip_set_image_offsets: global coordinates of the images' (0, 0) pixels."""
+        self._check(self._lib.dice_hip_set_image_offsets(self.h, ref_ox, ref_oy, def_ox, def_oy),
+                    "set_image_offsets")
 
     def correlate(self, fields, mode=MODE_BATCHED, first_frame=True):
         assert fields.dtype == np.float64 and fields.flags.c_contiguous and fields.shape == (NUM_FIELDS, self.n)

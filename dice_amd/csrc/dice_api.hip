@@ -5,6 +5,7 @@
 // correlation passes, one HIP stream and (for multi-frame sequences) a
 // hipGraph.  All entry points return a code and never throw.
 #include "dice_hip.h"
+#include "dice_api.h"
 
 #include <hip/hip_runtime.h>
 
@@ -124,6 +125,11 @@ struct dice_hip_context {
   int max_count = 0;  // most pixels in one subset
   DevBuf cx, cy, off, px, py, act, bbox, nact, pxy, abox;
   bool packed = false;  // conformal pixels also packed into pxy
+  // per-subset options: force simplex flags, blocked-pixel masks (box, word offset, bits)
+  DevBuf force, mbox, moff, mbits;
+  bool has_force = false, has_masks = false;
+  // global coordinates of the images' (0, 0) pixels (Image_::offset_x / offset_y)
+  int rox = 0, roy = 0, dox = 0, doy = 0;
   // fields + work
   DevBuf fields, work;
   int *scratch_work = nullptr;  // this device's copy of the static work area (no-allocation entry point)
@@ -230,17 +236,19 @@ int check_ready(dice_hip_context *ctx) {
   if (!ctx->ref_pix.p) return ctx->fail(DICE_HIP_ESTATE, "reference image not set");
   if (!ctx->def_ptr) return ctx->fail(DICE_HIP_ESTATE, "deformed image not set");
   if (ctx->n <= 0) return ctx->fail(DICE_HIP_ESTATE, "subsets not set");
-  if (ctx->rw != ctx->dw || ctx->rh != ctx->dh || ctx->rstorage != ctx->dstorage)
-    return ctx->fail(DICE_HIP_EINVAL, "reference and deformed images differ in size or storage");
+  // with sub-images (dice_hip_set_image_offsets) the two images may cover different parts of the frame
+  if (ctx->rstorage != ctx->dstorage) return ctx->fail(DICE_HIP_EINVAL, "reference and deformed images differ in storage");
+  if ((ctx->rw != ctx->dw || ctx->rh != ctx->dh) && ctx->rox == 0 && ctx->roy == 0 && ctx->dox == 0 && ctx->doy == 0)
+    return ctx->fail(DICE_HIP_EINVAL, "reference and deformed images differ in size");
   if (ctx->params.compute_def_gradients && !ctx->def_has_grads)
     return ctx->fail(DICE_HIP_ESTATE, "compute_def_gradients set after the deformed image; set the image again");
   return DICE_HIP_OK;
 }
 
 void fill_views(dice_hip_context *ctx, DImage &ref, DImage &def, DSubsets &ss) {
-  ref = DImage{ctx->ref_pix.p, ctx->ref_gx.as<float>(), ctx->ref_gy.as<float>(), ctx->rw, ctx->rh, 0, 0};
+  ref = DImage{ctx->ref_pix.p, ctx->ref_gx.as<float>(), ctx->ref_gy.as<float>(), ctx->rw, ctx->rh, ctx->rox, ctx->roy};
   def = DImage{ctx->def_ptr, ctx->def_has_grads ? ctx->def_gx.as<float>() : nullptr,
-               ctx->def_has_grads ? ctx->def_gy.as<float>() : nullptr, ctx->dw, ctx->dh, 0, 0};
+               ctx->def_has_grads ? ctx->def_gy.as<float>() : nullptr, ctx->dw, ctx->dh, ctx->dox, ctx->doy};
   ss = DSubsets{ctx->n,
                 ctx->S,
                 ctx->fov,
@@ -253,7 +261,11 @@ void fill_views(dice_hip_context *ctx, DImage &ref, DImage &def, DSubsets &ss) {
                 ctx->S > 0 ? nullptr : ctx->bbox.as<int4>(),
                 ctx->S > 0 ? nullptr : ctx->nact.as<int>(),
                 ctx->S > 0 || !ctx->packed ? nullptr : ctx->pxy.as<uint32_t>(),
-                ctx->S > 0 ? nullptr : ctx->abox.as<int4>()};
+                ctx->S > 0 ? nullptr : ctx->abox.as<int4>(),
+                ctx->has_force ? ctx->force.as<uint8_t>() : nullptr,
+                ctx->has_masks ? ctx->mbox.as<int4>() : nullptr,
+                ctx->has_masks ? ctx->moff.as<int>() : nullptr,
+                ctx->has_masks ? ctx->mbits.as<uint32_t>() : nullptr};
 }
 
 Launch current_launch(dice_hip_context *ctx, int mode) {
@@ -289,6 +301,7 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
   fill_views(ctx, ref, def, ss);
   KParams kp = make_kparams(ctx->params, first_frame);
   kp.guess = guess;
+  kp.has_forced = ctx->has_force ? 1 : 0;
   const Launch L = current_launch(ctx, mode);
   // per-launch timing events on the context stream (eager launches only; a captured
   // graph is timed as a whole by the caller)
@@ -381,7 +394,8 @@ int dice_hip_destroy(dice_hip_context *ctx) {
                     &ctx->pxy, &ctx->abox, &ctx->ref_raw, &ctx->def_filt, &ctx->nb_start, &ctx->nb_guess[0],
                     &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2,
                     &ctx->vsg_rec, &ctx->vsg_temp, &ctx->vsg_bounds,
-                    &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work})
+                    &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work, &ctx->force, &ctx->mbox, &ctx->moff,
+                    &ctx->mbits})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -608,9 +622,90 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     ctx->S = 0;
   }
   ctx->n = n;
+  ctx->has_force = ctx->has_masks = false;  // per-subset options belong to the old subset set
   ctx->last_work = nullptr;  // the work area may move
   HIP_TRY(ctx, ctx->work.ensure(dice_hip::work_ints(n) * sizeof(int)), "hipMalloc work");
   HIP_TRY(ctx, ctx->fields.ensure((size_t)DICE_F_COUNT * n * sizeof(double)), "hipMalloc fields");
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_force_simplex(dice_hip_context *ctx, const uint8_t *force) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_ESTATE, "subsets not set");
+  (void)hipSetDevice(ctx->device);
+  ctx->has_force = false;
+  if (!force) return DICE_HIP_OK;
+  bool any = false;
+  for (int i = 0; i < ctx->n && !any; ++i) any = force[i] != 0;
+  if (!any) return DICE_HIP_OK;
+  HIP_TRY(ctx, ctx->force.ensure((size_t)ctx->n), "hipMalloc force");
+  HIP_TRY(ctx, hipMemcpy(ctx->force.p, force, (size_t)ctx->n, hipMemcpyHostToDevice), "hipMemcpy force");
+  ctx->has_force = true;
+  drop_graph(ctx);
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_subset_masks(dice_hip_context *ctx, const int32_t *mask_offsets, const int32_t *mask_x,
+                              const int32_t *mask_y) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (ctx->n <= 0) return ctx->fail(DICE_HIP_ESTATE, "subsets not set");
+  (void)hipSetDevice(ctx->device);
+  ctx->has_masks = false;
+  drop_graph(ctx);
+  if (!mask_offsets) return DICE_HIP_OK;
+  if (!mask_x || !mask_y) return ctx->fail(DICE_HIP_EINVAL, "null mask coordinates");
+  const int n = ctx->n;
+  if (mask_offsets[0] != 0) return ctx->fail(DICE_HIP_EINVAL, "mask offsets must start at 0");
+  for (int i = 0; i < n; ++i)
+    if (mask_offsets[i + 1] < mask_offsets[i]) return ctx->fail(DICE_HIP_EINVAL, "mask offsets must not decrease");
+  // per subset: the bbox of its blocked pixels and one bit per pixel of the box, row-major
+  std::vector<int4> box(n);
+  std::vector<int> woff(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const int b = mask_offsets[i], e = mask_offsets[i + 1];
+    int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    for (int k = b; k < e; ++k) {
+      if (k == b) { x0 = x1 = mask_x[k]; y0 = y1 = mask_y[k]; }
+      x0 = std::min(x0, (int)mask_x[k]);
+      x1 = std::max(x1, (int)mask_x[k]);
+      y0 = std::min(y0, (int)mask_y[k]);
+      y1 = std::max(y1, (int)mask_y[k]);
+    }
+    const long long w = e > b ? (long long)x1 - x0 + 1 : 0, h = e > b ? (long long)y1 - y0 + 1 : 0;
+    if (w * h > (1ll << 28)) return ctx->fail(DICE_HIP_EINVAL, "a subset's mask spans too large a box");
+    box[i] = make_int4(x0, y0, (int)w, (int)h);
+    const long long words = (w * h + 31) / 32;
+    if ((long long)woff[i] + words > (1ll << 30)) return ctx->fail(DICE_HIP_EINVAL, "masks too large");
+    woff[i + 1] = woff[i] + (int)words;
+  }
+  std::vector<uint32_t> bits((size_t)std::max(woff[n], 1), 0u);
+  for (int i = 0; i < n; ++i)
+    for (int k = mask_offsets[i]; k < mask_offsets[i + 1]; ++k) {
+      const unsigned bit = (unsigned)(mask_y[k] - box[i].y) * (unsigned)box[i].z + (unsigned)(mask_x[k] - box[i].x);
+      bits[(size_t)woff[i] + (bit >> 5)] |= 1u << (bit & 31u);
+    }
+  HIP_TRY(ctx, ctx->mbox.ensure((size_t)n * sizeof(int4)), "hipMalloc masks");
+  HIP_TRY(ctx, ctx->moff.ensure((size_t)(n + 1) * sizeof(int)), "hipMalloc masks");
+  HIP_TRY(ctx, ctx->mbits.ensure(bits.size() * sizeof(uint32_t)), "hipMalloc masks");
+  HIP_TRY(ctx, hipMemcpy(ctx->mbox.p, box.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy masks");
+  HIP_TRY(ctx, hipMemcpy(ctx->moff.p, woff.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice),
+          "hipMemcpy masks");
+  HIP_TRY(ctx, hipMemcpy(ctx->mbits.p, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+          "hipMemcpy masks");
+  ctx->has_masks = mask_offsets[n] > 0;
+  return DICE_HIP_OK;
+}
+
+int dice_hip_set_image_offsets(dice_hip_context *ctx, int ref_offset_x, int ref_offset_y, int def_offset_x,
+                               int def_offset_y) {
+  if (!ctx) return DICE_HIP_EINVAL;
+  if (ref_offset_x < 0 || ref_offset_y < 0 || def_offset_x < 0 || def_offset_y < 0)
+    return ctx->fail(DICE_HIP_EINVAL, "image offsets must be >= 0");
+  ctx->rox = ref_offset_x;
+  ctx->roy = ref_offset_y;
+  ctx->dox = def_offset_x;
+  ctx->doy = def_offset_y;
+  drop_graph(ctx);
   return DICE_HIP_OK;
 }
 
@@ -1104,43 +1199,49 @@ int dice_hip_gauss_filter(int device, int width, int height, int storage, const 
   return rc;
 }
 
-int dice_hip_correlate_points(float *points, int n_points, int subset_size, const float *ref_img, int ref_w, int ref_h,
-                              const float *def_img, int def_w, int def_h, const dice_params *params,
-                              int update_params) {
-  // DICe_api.cpp:51-153: static state across calls, stride-9 points
+// dice_correlate (DICe_api.cpp:51-153): one static context for the process, created with the
+// subsets and the reference image of the first call; later calls set the deformed image and correlate
+// the same points.  The state is committed only once a call has gone through: a first call that
+// fails leaves nothing behind, so the next call starts again.
+static int correlate_points_impl(float *points, int n_points, int subset_size, const float *ref_img, int ref_w,
+                                 int ref_h, const float *def_img, int def_w, int def_h, const dice_params *params,
+                                 bool update_params) {
+  static std::mutex m;
   static dice_hip_context *ctx = nullptr;
-  static bool initialized = false;
   static int n_init = 0;
+  std::lock_guard<std::mutex> lock(m);
   if (ref_w != def_w || ref_h != def_h) return -1;
   if (subset_size < 1 || n_points < 1 || !points || !ref_img || !def_img) return -1;
   const int stride = 9;
-  if (!initialized) {
-    if (dice_hip_create(&ctx, 0) != DICE_HIP_OK) return -1;
+  const bool first = ctx == nullptr;
+  dice_hip_context *c = ctx;
+  auto give_up = [&]() {
+    if (first && c) dice_hip_destroy(c);
+    return -1;
+  };
+  if (first) {
+    if (dice_hip_create(&c, 0) != DICE_HIP_OK) return -1;
     std::vector<double> cx(n_points), cy(n_points);
     for (int i = 0; i < n_points; ++i) {
       cx[i] = points[i * stride + 0];
       cy[i] = points[i * stride + 1];
     }
-    if (dice_hip_set_subsets(ctx, n_points, cx.data(), cy.data(), subset_size, nullptr, nullptr, nullptr, nullptr))
-      return -1;
-    n_init = n_points;
+    if (dice_hip_set_subsets(c, n_points, cx.data(), cy.data(), subset_size, nullptr, nullptr, nullptr, nullptr))
+      return give_up();
+  } else if (n_points != n_init) {
+    return -1;
   }
-  if (n_points != n_init) return -1;
-  if (!initialized || update_params) {
+  if (first || update_params) {
     dice_params p;
     if (params) p = *params;
     else dice_hip_tracking_default_params(&p);
     // set_def_image(w, h, array) builds the deformed image with compute_ref_gradients, which is on by
     // default (DICe_Schema.cpp:339, :628), so the API path always has deformed-image gradients
     p.compute_def_gradients = 1;
-    if (dice_hip_set_params(ctx, &p) != DICE_HIP_OK) return -1;
+    if (dice_hip_set_params(c, &p) != DICE_HIP_OK) return give_up();
   }
-  if (!initialized) {
-    if (dice_hip_set_ref_image(ctx, ref_w, ref_h, DICE_STORAGE_F32, ref_img) != DICE_HIP_OK) return -1;
-  }
-  const bool first = !initialized;
-  initialized = true;
-  if (dice_hip_set_def_image(ctx, def_w, def_h, DICE_STORAGE_F32, def_img) != DICE_HIP_OK) return -1;
+  if (first && dice_hip_set_ref_image(c, ref_w, ref_h, DICE_STORAGE_F32, ref_img) != DICE_HIP_OK) return give_up();
+  if (dice_hip_set_def_image(c, def_w, def_h, DICE_STORAGE_F32, def_img) != DICE_HIP_OK) return give_up();
   std::vector<double> f((size_t)DICE_F_COUNT * n_points, 0.0);
   auto F = [&](int slot, int i) -> double & { return f[(size_t)slot * n_points + i]; };
   for (int i = 0; i < n_points; ++i) {
@@ -1154,7 +1255,9 @@ int dice_hip_correlate_points(float *points, int n_points, int subset_size, cons
     F(DICE_F_BETA, i) = points[i * stride + 7];
     F(DICE_F_STATUS, i) = points[i * stride + 8];
   }
-  if (dice_hip_correlate(ctx, DICE_MODE_BATCHED, f.data(), first ? 1 : 0) != DICE_HIP_OK) return -1;
+  if (dice_hip_correlate(c, DICE_MODE_BATCHED, f.data(), first ? 1 : 0) != DICE_HIP_OK) return give_up();
+  ctx = c;
+  n_init = n_points;
   for (int i = 0; i < n_points; ++i) {
     points[i * stride + 2] = (float)F(DICE_F_DISP_X, i);
     points[i * stride + 3] = (float)F(DICE_F_DISP_Y, i);
@@ -1165,6 +1268,19 @@ int dice_hip_correlate_points(float *points, int n_points, int subset_size, cons
     points[i * stride + 8] = (float)F(DICE_F_STATUS, i);
   }
   return 0;
+}
+
+int dice_hip_correlate_points(float *points, int n_points, int subset_size, const float *ref_img, int ref_w, int ref_h,
+                              const float *def_img, int def_w, int def_h, const dice_params *params,
+                              int update_params) {
+  return correlate_points_impl(points, n_points, subset_size, ref_img, ref_w, ref_h, def_img, def_w, def_h, params,
+                               update_params != 0);
+}
+
+int dice_correlate(float points[], int n_points, int subset_size, float ref_img[], int ref_w, int ref_h,
+                   float def_img[], int def_w, int def_h, const dice_params *input_params, bool update_params) {
+  return correlate_points_impl(points, n_points, subset_size, ref_img, ref_w, ref_h, def_img, def_w, def_h,
+                               input_params, update_params);
 }
 
 }  // extern "C"

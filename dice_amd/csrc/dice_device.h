@@ -469,6 +469,10 @@ struct Solver {
   const gu8 *pact;
   const gu32 *pk;       // conformal pixels packed (x | y << 15 | inactive << 30), or null
   int abx0, aby0, abx1, aby1;  // conformal: bbox of the active pixels
+  // blocked image pixels of the subset: box (mx0, my0, mw, mh), bits at mb (mw = 0: none)
+  int mx0, my0, mw, mh;
+  const gu32 *mb;
+  bool forced;  // Nelder-Mead only
   float meanF;
   float en[6];  // 1 for enabled parameters
 
@@ -520,6 +524,18 @@ struct Solver {
       n_def_active = ss.n_active_def[i];
     }
     ref_in = bx0 - ref.ox >= 0 && by0 - ref.oy >= 0 && bx1 - ref.ox < ref.w && by1 - ref.oy < ref.h;
+    mw = 0;
+    mx0 = my0 = mh = 0;
+    mb = nullptr;
+    if (ss.mbox) {
+      const int4 b = ss.mbox[i];
+      mx0 = b.x;
+      my0 = b.y;
+      mw = b.z;
+      mh = b.w;
+      mb = (const gu32 *)ss.mbits + ss.moff[i];
+    }
+    forced = ss.force ? ss.force[i] != 0 : false;
   }
 
   // Visit the group's share of the subset pixels: f(x, y, active).  Negative
@@ -595,10 +611,20 @@ struct Solver {
 
   // Is the warped pixel kept?  Rounding + 4-px guard of Subset::initialize
   // (DICe_Subset.cpp:404-411); (ddx, ddy) = x' - cx, y' - cy.
+  // A pixel that rounds onto a blocked image pixel of the subset is dropped as
+  // well (is_obstructed_pixel and the blocked-pixel test, :412-419, 447-460).
   __device__ __forceinline__ bool kept(float ddx, float ddy) const {
     const int rx = cx + (int)floorf(ddx + 0.5f);
     const int ry = cy + (int)floorf(ddy + 0.5f);
-    return !(rx < def.ox + 4 || rx >= def.ox + def.w - 4 || ry < def.oy + 4 || ry >= def.oy + def.h - 4);
+    if (rx < def.ox + 4 || rx >= def.ox + def.w - 4 || ry < def.oy + 4 || ry >= def.oy + def.h - 4) return false;
+    if (mw > 0) {
+      const unsigned ux = (unsigned)(rx - mx0), uy = (unsigned)(ry - my0);
+      if (ux < (unsigned)mw && uy < (unsigned)mh) {
+        const unsigned bit = uy * (unsigned)mw + ux;
+        if ((mb[bit >> 5] >> (bit & 31u)) & 1u) return false;
+      }
+    }
+    return true;
   }
 
   // Deformed intensity at x' = cx + ddx, y' = cy + ddy (interior interpolant of
@@ -738,6 +764,7 @@ struct Solver {
   // rounded positions are monotone in the displacement, so the bbox extremes
   // decide; a 1e-3 px margin covers the rounding of the per-pixel map.
   __device__ __forceinline__ bool all_kept(const Affine &A) const {
+    if (mw > 0) return false;  // blocked pixels: the per-pixel test decides
     float lo_x, hi_x, lo_y, hi_y;
     warp_bounds(A, lo_x, hi_x, lo_y, hi_y);
     const int rx0 = cx + (int)floorf(lo_x + 0.5f - 1e-3f), rx1 = cx + (int)floorf(hi_x + 0.5f + 1e-3f);
@@ -1950,6 +1977,13 @@ struct Solver {
         return;
       }
     }
+    if constexpr (!SIMPLEX_OPT) {
+      // a forced subset goes to the Nelder-Mead pass, which starts it from its initial guess
+      if (pass == 0 && forced) {
+        push(fb_list, fb_count, gid);
+        return;
+      }
+    }
     const float prev_u = (float)fv(F_DISP_X), prev_v = (float)fv(F_DISP_Y), prev_t = (float)fv(F_ROT_Z);
     float p_state[6];
     int corr;
@@ -1973,7 +2007,7 @@ struct Solver {
     const bool jump = fabsf(du) > kp.disp_jump_tol || fabsf(dv) > kp.disp_jump_tol || fabsf(dt) > kp.theta_jump_tol;
     if (pass == 0) {
       if (corr != CORRELATION_SUCCESSFUL || jump) {
-        if (kp.opt == 3 || kp.opt == 4) {
+        if ((kp.opt == 3 || kp.opt == 4) && !forced) {
           // the fallback starts again from the initial guess (:2238, :2266)
           push(fb_list, fb_count, gid);
           return;

@@ -49,6 +49,7 @@ struct KParams {
   int normalize_gamma;
   int uncertainty;
   int first_frame;
+  int has_forced;        // some subset has force[i] set: a GN primary pass hands them to a Nelder-Mead pass
   // USE_NEIGHBOR_VALUES: per-subset initial guesses, slot-major n x kGuessSlots
   // (u, v, theta, ex, ey, gxy, sigma of the source subset, init status), or null
   // to start every subset from its own fields (USE_FIELD_VALUES)
@@ -77,6 +78,14 @@ struct DSubsets {
   const uint32_t *pxy;         // conformal, may be null: pixel i packed as x | y << 15 | inactive << 30
                                // (coordinates clamped at 0, below 2^15)
   const int4 *abox;            // conformal: bbox of the active pixels (of all pixels if none is active)
+  // Per-subset options (either may be null).  force[i] != 0: Nelder-Mead only (Schema::set_force_simplex
+  // and the blocking subset ids, DICe_Schema.cpp:2160-2185).  mbox[i] = (x0, y0, w, h) and the row-major
+  // bits at mbits + moff[i]: image pixels that deactivate a warped pixel of subset i rounding onto them
+  // (Subset::obstructed_coords_, pixels_blocked_by_other_subsets_; w = 0: none).
+  const uint8_t *force;
+  const int4 *mbox;
+  const int *moff;
+  const uint32_t *mbits;
 };
 
 // BATCHED Gauss-Newton reduction: kGnSums per-lane partials (q 6, H upper

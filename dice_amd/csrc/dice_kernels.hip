@@ -708,7 +708,8 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
   PassArgs pa{0, items, items ? item_count : nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work, nullptr};
   e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, nm, stream);
-  if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4)) return e;
+  // the second pass: the fallback optimizer, or Nelder-Mead for the forced subsets of a Gauss-Newton pass
+  if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4 || (kp.has_forced && !simplex_first))) return e;
   PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work, nullptr};
   return launch_pass(!simplex_first, storage, L, kp, ref, def, subs, fields, pb, ov1, work + kWorkOv1Count, nm,
                      stream);

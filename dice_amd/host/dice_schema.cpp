@@ -1,11 +1,14 @@
 // C++ host mirror of the reference's Schema interface (see dice_schema.hpp).
 #include "dice_schema.hpp"
+#include "dice_api.h"
 
 #include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
+#include <mutex>
 #include <set>
 #include <sstream>
 
@@ -55,6 +58,13 @@ const std::map<std::string, Slot> &slots() {
   return m;
 }
 const char *kBackend = "correlation_backend";
+// Schema-level parameters of the obstruction and blocking logic (DICe.h:273, 277), not in dice_params
+const char *kOverrideForceSimplex = "override_force_simplex";
+const char *kSkinFactor = "obstruction_skin_factor";
+const char *kReadFullImages = "read_full_images";
+bool schema_level(const std::string &name) {
+  return name == kBackend || name == kOverrideForceSimplex || name == kSkinFactor || name == kReadFullImages;
+}
 
 void to_list(const dice_params &p, ParameterList *out) {
   for (const auto &kv : slots()) {
@@ -64,21 +74,33 @@ void to_list(const dice_params &p, ParameterList *out) {
     out->set(kv.first, v);
   }
   out->set(kBackend, HIP_BATCHED);
+  // DICe_ParameterUtilities.cpp:323,333 (tracking) and :392,396 (full field)
+  const bool tracking = p.correlation_routine == TRACKING_ROUTINE;
+  out->set(kOverrideForceSimplex, tracking ? 1.0 : 0.0);
+  out->set(kSkinFactor, tracking ? 1.8 : 1.0);
+  out->set(kReadFullImages, 0.0);  // :326, :377
 }
 
-// atan2 difference wrapped to [-pi, pi] (angle_2d, DICe_Shape.cpp:88-103).
-double angle_2d(double x1, double y1, double x2, double y2) {
-  const double pi = 3.14159265358979323846;
-  double d = std::atan2(y2, x2) - std::atan2(y1, x1);
-  while (d > pi) d -= 2.0 * pi;
-  while (d < -pi) d += 2.0 * pi;
+// atan2 difference wrapped to [-pi, pi] (angle_2d, DICe_Shape.cpp:88-103), in
+// scalar_t = float as the reference's default build computes it: the float atan2,
+// the wrap formed against the double constants and rounded back to float.  A
+// pixel on a polygon edge sums to +-pi up to rounding, so the arithmetic decides
+// whether it belongs to the polygon (the float build's conformal subset golden,
+// tests/component/images/conformal.rawi, has one such pixel fewer than the double
+// build's conformal_d.rawi).
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+float angle_2d(float x1, float y1, float x2, float y2) {
+  float d = std::atan2(y2, x2) - std::atan2(y1, x1);
+  while (d > kPi) d -= kTwoPi;
+  while (d < -kPi) d += kTwoPi;
   return d;
 }
 
 }  // namespace
 
 ParameterList &ParameterList::set(const std::string &name, double value) {
-  if (name != kBackend && slots().count(name) == 0)
+  if (!schema_level(name) && slots().count(name) == 0)
     throw std::invalid_argument("Error, unknown correlation parameter: " + name);
   values_[name] = value;
   return *this;
@@ -88,6 +110,79 @@ double ParameterList::get(const std::string &name) const {
   auto it = values_.find(name);
   if (it == values_.end()) throw std::invalid_argument("Error, parameter not set: " + name);
   return it->second;
+}
+
+ParameterList read_correlation_params(const std::string &file_name, bool *use_tracking_defaults) {
+  std::ifstream in(file_name);
+  if (!in) throw std::runtime_error("Error, cannot read the parameter file " + file_name);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string text = ss.str();
+  ParameterList out;
+  if (use_tracking_defaults) *use_tracking_defaults = false;
+  // enum-valued parameters and their strings, in enum order (DICe.h:440-600)
+  static const std::map<std::string, std::vector<std::string>> enums = {
+      {"correlation_routine", {"GENERIC_ROUTINE", "TRACKING_ROUTINE"}},
+      {"interpolation_method", {"BILINEAR", "BICUBIC", "KEYS_FOURTH"}},
+      {"optimization_method",
+       {"SIMPLEX", "GRADIENT_BASED", "GRADIENT_THEN_SEARCH", "SIMPLEX_THEN_GRADIENT_BASED",
+        "GRADIENT_BASED_THEN_SIMPLEX"}},
+      {"initialization_method", {"USE_FIELD_VALUES", "USE_NEIGHBOR_VALUES", "USE_NEIGHBOR_VALUES_FIRST_STEP_ONLY"}},
+      {"gradient_method", {"FINITE_DIFFERENCE", "CONVOLUTION_5_POINT"}}};
+  static const std::set<std::string> output_only = {"output_deformed_subset_images",
+                                                    "output_deformed_subset_intensity_images",
+                                                    "output_evolved_subset_images"};
+  auto attr = [](const std::string &tag, const char *key) -> std::string {
+    const std::string k = std::string(key) + "=\"";
+    const size_t a = tag.find(k);
+    if (a == std::string::npos) return "";
+    const size_t b = tag.find('"', a + k.size());
+    return b == std::string::npos ? "" : tag.substr(a + k.size(), b - a - k.size());
+  };
+  size_t pos = 0;
+  int depth = 0;
+  while ((pos = text.find('<', pos)) != std::string::npos) {
+    const size_t end = text.find('>', pos);
+    if (end == std::string::npos) break;
+    const std::string tag = text.substr(pos, end - pos + 1);
+    pos = end + 1;
+    if (tag.compare(0, 4, "<!--") == 0 || tag.compare(0, 2, "<?") == 0) continue;
+    if (tag.compare(0, 14, "<ParameterList") == 0) {
+      if (++depth > 1) throw std::invalid_argument("Error, nested parameter lists are not supported: " + attr(tag, "name"));
+      continue;
+    }
+    if (tag.compare(0, 15, "</ParameterList") == 0) {
+      --depth;
+      continue;
+    }
+    if (tag.compare(0, 10, "<Parameter") != 0) continue;
+    std::string name = attr(tag, "name"), type = attr(tag, "type"), value = attr(tag, "value");
+    std::transform(name.begin(), name.end(), name.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
+    if (name == "use_tracking_default_params") {
+      if (use_tracking_defaults) *use_tracking_defaults = value == "true" || value == "1";
+      continue;
+    }
+    if (output_only.count(name)) continue;
+    if (name == "shape_function_type") {
+      if (value != "AFFINE") throw std::invalid_argument("Error, only the AFFINE shape function is supported");
+      continue;
+    }
+    double v = 0.0;
+    auto e = enums.find(name);
+    if (e != enums.end()) {
+      auto it = std::find(e->second.begin(), e->second.end(), value);
+      if (it == e->second.end()) throw std::invalid_argument("Error, unknown value " + value + " of " + name);
+      v = (double)(it - e->second.begin());
+    } else if (type == "bool") {
+      v = (value == "true" || value == "1") ? 1.0 : 0.0;
+    } else {
+      char *stop = nullptr;
+      v = std::strtod(value.c_str(), &stop);
+      if (value.empty() || (stop && *stop)) throw std::invalid_argument("Error, bad value of " + name + ": " + value);
+    }
+    out.set(name, v);  // throws for a name this path does not read
+  }
+  return out;
 }
 
 void dice_default_params(ParameterList *params) {
@@ -118,10 +213,10 @@ Polygon::Polygon(const std::vector<int> &coords_x, const std::vector<int> &coord
 }
 
 bool Polygon::inside(int x, int y) const {
-  double angle = 0.0;
+  float angle = 0.0f;
   for (size_t i = 0; i + 1 < vx_.size(); ++i)
-    angle += angle_2d(vx_[i] - x, vy_[i] - y, vx_[i + 1] - x, vy_[i + 1] - y);
-  return std::abs(angle) >= 3.14159265358979323846;
+    angle += angle_2d((float)(vx_[i] - x), (float)(vy_[i] - y), (float)(vx_[i + 1] - x), (float)(vy_[i + 1] - y));
+  return std::abs(angle) >= kPi;
 }
 
 std::vector<std::pair<int, int>> Polygon::get_owned_pixels() const {
@@ -182,6 +277,104 @@ std::vector<std::pair<int, int>> Rectangle::get_owned_pixels() const {
 void Rectangle::deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const {
   for (int j = 0; j < size; ++j)
     if (x[j] >= origin_x_ && x[j] < origin_x_ + width_ && y[j] >= origin_y_ && y[j] < origin_y_ + height_) flags[j] = 0;
+}
+
+namespace {
+
+// Affine_Shape_Function::map (DICe_LocalShapeFunction.cpp:220-248) in scalar_t = float: the
+// disabled parameters read as 0.
+void map_float(const Affine_Shape_Function &sf, float x, float y, float cx, float cy, float &out_x, float &out_y) {
+  const double *q = sf.parameters();
+  const float dispx = (float)q[0], dispy = (float)q[1], theta = (float)q[2], dudx = (float)q[3],
+              dvdy = (float)q[4], gxy = (float)q[5];
+  const float cost = std::cos(theta), sint = std::sin(theta);
+  const float dx = x - cx, dy = y - cy;
+  const float Dx = (float)((1.0 + dudx) * dx + gxy * dy);
+  const float Dy = (float)((1.0 + dvdy) * dy + gxy * dx);
+  out_x = cost * Dx - sint * Dy + dispx + cx;
+  out_y = sint * Dx + cost * Dy + dispy + cy;
+}
+
+// The moved, stretched vertices of a closed vertex list (vertex 0 repeated at the end) and their
+// bbox, as Polygon/Rectangle::get_owned_pixels(shape_function, ...) form them (DICe_Shape.cpp:
+// 147-199, 347-395): each vertex mapped and rounded (c = (int)X, +1 for a fraction >= 0.5), the
+// integer centroid of the open list, then x = (int)(skin (x - c) + c).
+void moved_vertices(const Affine_Shape_Function &sf, int cx, int cy, double skin, const std::vector<int> &vx0,
+                    const std::vector<int> &vy0, std::vector<int> &vx, std::vector<int> &vy, int &min_x, int &max_x,
+                    int &min_y, int &max_y) {
+  const size_t m = vx0.size();
+  vx.assign(m, 0);
+  vy.assign(m, 0);
+  for (size_t i = 0; i < m; ++i) {
+    float X = 0.0f, Y = 0.0f;
+    map_float(sf, (float)vx0[i], (float)vy0[i], (float)cx, (float)cy, X, Y);
+    int nx = (int)X, ny = (int)Y;
+    if (X - (int)X >= 0.5f) nx++;
+    if (Y - (int)Y >= 0.5f) ny++;
+    vx[i] = nx;
+    vy[i] = ny;
+  }
+  int ccx = 0, ccy = 0;
+  for (size_t i = 0; i + 1 < m; ++i) {
+    ccx += vx[i];
+    ccy += vy[i];
+  }
+  const double cnt = m - 1 == 0 ? 1.0 : (double)(m - 1);
+  ccx = (int)(ccx / cnt);
+  ccy = (int)(ccy / cnt);
+  const float sk = (float)skin;
+  for (size_t i = 0; i < m; ++i) {
+    const int nx = (int)(sk * (vx[i] - ccx) + ccx), ny = (int)(sk * (vy[i] - ccy) + ccy);
+    vx[i] = nx;
+    vy[i] = ny;
+    if (i == 0) {
+      min_x = max_x = nx;
+      min_y = max_y = ny;
+    } else {
+      min_x = std::min(min_x, nx);
+      max_x = std::max(max_x, nx);
+      min_y = std::min(min_y, ny);
+      max_y = std::max(max_y, ny);
+    }
+  }
+}
+
+// pixels of the bbox whose winding angle around the closed vertex list is at least pi
+std::vector<std::pair<int, int>> winding_pixels(const std::vector<int> &vx, const std::vector<int> &vy, int min_x,
+                                                int max_x, int min_y, int max_y) {
+  std::vector<std::pair<int, int>> out;
+  for (int y = min_y; y <= max_y; ++y)
+    for (int x = min_x; x <= max_x; ++x) {
+      float angle = 0.0f;
+      for (size_t i = 0; i + 1 < vx.size(); ++i)
+        angle += angle_2d((float)(vx[i] - x), (float)(vy[i] - y), (float)(vx[i + 1] - x), (float)(vy[i + 1] - y));
+      if (std::abs(angle) >= kPi) out.emplace_back(y, x);
+    }
+  return out;
+}
+
+}  // namespace
+
+std::vector<std::pair<int, int>> Polygon::get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                           double skin_factor) const {
+  std::vector<int> vx, vy;
+  int min_x, max_x, min_y, max_y;
+  moved_vertices(sf, cx, cy, skin_factor, vx_, vy_, vx, vy, min_x, max_x, min_y, max_y);
+  return winding_pixels(vx, vy, min_x, max_x, min_y, max_y);
+}
+
+std::vector<std::pair<int, int>> Circle::get_owned_pixels(const Affine_Shape_Function &, int, int, double) const {
+  throw std::runtime_error("Error, circle deformation has not been implemented yet");
+}
+
+std::vector<std::pair<int, int>> Rectangle::get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                             double skin_factor) const {
+  const std::vector<int> vx0 = {origin_x_, origin_x_ + width_, origin_x_ + width_, origin_x_, origin_x_};
+  const std::vector<int> vy0 = {origin_y_, origin_y_, origin_y_ + height_, origin_y_ + height_, origin_y_};
+  std::vector<int> vx, vy;
+  int min_x, max_x, min_y, max_y;
+  moved_vertices(sf, cx, cy, skin_factor, vx0, vy0, vx, vy, min_x, max_x, min_y, max_y);
+  return winding_pixels(vx, vy, min_x, max_x, min_y, max_y);
 }
 
 namespace {
@@ -531,7 +724,7 @@ void Schema::set_params(const ParameterList &params) {
   dice_params p;
   dice_hip_default_params(&p);
   for (const auto &kv : params.values()) {
-    if (kv.first == kBackend) continue;
+    if (schema_level(kv.first)) continue;
     const Slot &s = slots().at(kv.first);
     char *base = reinterpret_cast<char *>(&p) + s.offset;
     if (s.kind == I32) *reinterpret_cast<int32_t *>(base) = (int32_t)kv.second;
@@ -546,6 +739,9 @@ void Schema::set_params(const ParameterList &params) {
   check(dice_hip_set_params(ctx_, &p), "set_params");
   p_ = p;
   backend_ = backend;
+  override_force_simplex_ = params.isParameter(kOverrideForceSimplex) && params.get(kOverrideForceSimplex) != 0.0;
+  skin_factor_ = params.isParameter(kSkinFactor) ? params.get(kSkinFactor) : 1.0;
+  read_full_images_ = params.isParameter(kReadFullImages) && params.get(kReadFullImages) != 0.0;
 }
 
 void create_regular_grid_of_correlation_points(int img_w, int img_h, int subset_size, int step_size,
@@ -652,6 +848,23 @@ void Schema::init_subsets(const std::vector<double> &cx, const std::vector<doubl
   n_ = n;
   subset_dim_ = subset_size;
   fields_.assign((size_t)DICE_F_COUNT * n, 0.0);
+  defs_.clear();
+  obstructed_.assign(n, {});
+  obstructing_ids_.clear();
+  force_simplex_ids_.clear();
+  if (defs) {
+    defs_ = *defs;
+    // Subset::obstructed_coords_ (DICe_Subset.cpp:162-167): the pixels of the obstructed shapes
+    for (const auto &kv : *defs) {
+      if (kv.first < 0 || kv.first >= n) continue;
+      std::set<std::pair<int, int>> yx;
+      for (const auto &shape : kv.second.obstructed_area) {
+        const auto px = shape->get_owned_pixels();
+        yx.insert(px.begin(), px.end());
+      }
+      for (const auto &p : yx) obstructed_[kv.first].emplace_back(p.second, p.first);
+    }
+  }
   if (!defs || defs->empty()) {
     if (subset_size <= 0) throw std::runtime_error("Error, square subsets need a subset size");
     check(dice_hip_set_subsets(ctx_, n, cx.data(), cy.data(), subset_size, nullptr, nullptr, nullptr, nullptr),
@@ -691,6 +904,83 @@ void Schema::init_subsets(const std::vector<double> &cx, const std::vector<doubl
     fields_[(size_t)DICE_F_COORD_X * n + i] = cx[i];
     fields_[(size_t)DICE_F_COORD_Y * n + i] = cy[i];
   }
+  upload_masks({});
+}
+
+void Schema::set_obstructing_subset_ids(const std::map<int, std::vector<int>> &ids) {
+  for (const auto &kv : ids) {
+    if (kv.first < 0 || kv.first >= n_) throw std::runtime_error("Error, blocked subset id out of range");
+    for (int b : kv.second)
+      if (b < 0 || b >= n_) throw std::runtime_error("Error, blocking subset id out of range");
+  }
+  obstructing_ids_ = ids;
+}
+
+void Schema::set_force_simplex(const std::vector<int> &ids) {
+  for (int id : ids)
+    if (id < 0 || id >= n_) throw std::runtime_error("Error, force simplex subset id out of range");
+  force_simplex_ids_ = ids;
+}
+
+// generic_correlation_routine's force_simplex (DICe_Schema.cpp:2160-2185): the FORCE_SIMPLEX subsets,
+// the subsets with blockers and the blockers, unless override_force_simplex
+void Schema::upload_force_flags() {
+  std::vector<uint8_t> force(n_, 0);
+  bool any = false;
+  if (!override_force_simplex_) {
+    for (int id : force_simplex_ids_) force[id] = 1;
+    for (const auto &kv : obstructing_ids_) {
+      if (kv.second.empty()) continue;
+      force[kv.first] = 1;
+      for (int b : kv.second) force[b] = 1;
+    }
+    for (uint8_t f : force) any = any || f;
+  }
+  check(dice_hip_set_force_simplex(ctx_, any ? force.data() : nullptr), "set_force_simplex");
+}
+
+// The obstructed pixels of every subset, with the pixels blocked by other subsets (blocked[i], may be
+// empty) added.
+void Schema::upload_masks(const std::vector<std::vector<std::pair<int, int>>> &blocked) {
+  std::vector<int32_t> off(1, 0), mx, my;
+  for (int i = 0; i < n_; ++i) {
+    for (const auto &p : obstructed_[i]) {
+      mx.push_back(p.first);
+      my.push_back(p.second);
+    }
+    if (i < (int)blocked.size())
+      for (const auto &p : blocked[i]) {
+        mx.push_back(p.first);
+        my.push_back(p.second);
+      }
+    off.push_back((int32_t)mx.size());
+  }
+  check(dice_hip_set_subset_masks(ctx_, mx.empty() ? nullptr : off.data(), mx.data(), my.data()), "set_subset_masks");
+}
+
+// check_for_blocking_subsets (DICe_Schema.cpp:4529-4556): the pixels of the conformal blockers of
+// `subset`, each moved by the blocker's current shape function (its fields) about its centroid and
+// stretched by the skin factor, as (x, y); square blockers block nothing (Subset::deformed_shapes,
+// DICe_Subset.cpp:462-474).
+std::vector<std::pair<int, int>> Schema::blocking_pixels(int subset) const {
+  std::vector<std::pair<int, int>> out;
+  auto it = obstructing_ids_.find(subset);
+  if (it == obstructing_ids_.end()) return out;
+  std::set<std::pair<int, int>> yx;
+  for (int b : it->second) {
+    auto d = defs_.find(b);
+    if (d == defs_.end()) continue;
+    Affine_Shape_Function sf(rotation_enabled(), normal_strain_enabled(), shear_strain_enabled());
+    sf.initialize_parameters_from_fields(this, b);
+    const int cx = (int)local_field_value(b, SUBSET_COORDINATES_X_FS);
+    const int cy = (int)local_field_value(b, SUBSET_COORDINATES_Y_FS);
+    for (const auto &shape : d->second.boundary) {
+      const auto px = shape->get_owned_pixels(sf, cx, cy, skin_factor_);
+      yx.insert(px.begin(), px.end());
+    }
+  }
+  for (const auto &p : yx) out.emplace_back(p.second, p.first);
+  return out;
 }
 
 template <typename T>
@@ -817,17 +1107,70 @@ void Schema::set_seed_values(const Subset_File_Info &info) {
   }
 }
 
+void Schema::update_extents() {
+  if (!defs_.empty() || read_full_images_) {
+    has_extents_ = false;
+    return;
+  }
+  // in scalar_t = float from INT_MAX / 0, as the reference
+  float mnx_r = (float)std::numeric_limits<int>::max(), mxx_r = 0.0f, mny_r = mnx_r, mxy_r = 0.0f;
+  float mnx_d = mnx_r, mxx_d = 0.0f, mny_d = mnx_r, mxy_d = 0.0f;
+  for (int i = 0; i < n_; ++i) {
+    const float x = (float)local_field_value(i, SUBSET_COORDINATES_X_FS);
+    const float y = (float)local_field_value(i, SUBSET_COORDINATES_Y_FS);
+    const float xd = (float)(local_field_value(i, SUBSET_COORDINATES_X_FS) + local_field_value(i, SUBSET_DISPLACEMENT_X_FS));
+    const float yd = (float)(local_field_value(i, SUBSET_COORDINATES_Y_FS) + local_field_value(i, SUBSET_DISPLACEMENT_Y_FS));
+    mnx_r = std::min(mnx_r, x); mxx_r = std::max(mxx_r, x);
+    mny_r = std::min(mny_r, y); mxy_r = std::max(mxy_r, y);
+    mnx_d = std::min(mnx_d, xd); mxx_d = std::max(mxx_d, xd);
+    mny_d = std::min(mny_d, yd); mxy_d = std::max(mxy_d, yd);
+  }
+  const int buffer = 100;
+  auto lower = [&](float v) { v -= buffer; return v < buffer ? 0.0f : std::round(v); };
+  // the scalar_t extents stored in the int_t vectors
+  ref_ext_[0] = (int)lower(mnx_r);
+  ref_ext_[1] = (int)(mxx_r + buffer);
+  ref_ext_[2] = (int)lower(mny_r);
+  ref_ext_[3] = (int)(mxy_r + buffer);
+  def_ext_[0] = (int)lower(mnx_d);
+  def_ext_[1] = (int)(mxx_d + buffer);
+  def_ext_[2] = (int)lower(mny_d);
+  def_ext_[3] = (int)(mxy_d + buffer);
+  has_extents_ = true;
+}
+
+// The sub-image of the frame that the extents cover (DICe_Schema.cpp:200-210, 366-373): a side whose
+// extent lies within 100 px of the image edge takes the whole image on that side.
+void Schema::read_with_extents(const std::string &file_name, const int *ext, std::vector<float> &pix, int &w,
+                               int &h, int &ox, int &oy) const {
+  ox = oy = 0;
+  if (!has_extents_) {
+    read_image(file_name, pix, w, h);
+    return;
+  }
+  int fw = 0, fh = 0;
+  read_image_dimensions(file_name, fw, fh);
+  const int buffer = 100;
+  ox = ext[0] > buffer && ext[0] < fw - buffer ? ext[0] : 0;
+  oy = ext[2] > buffer && ext[2] < fh - buffer ? ext[2] : 0;
+  const int end_x = ext[1] > buffer && ext[1] < fw - buffer ? ext[1] : fw;
+  const int end_y = ext[3] > buffer && ext[3] < fh - buffer ? ext[3] : fh;
+  read_image(file_name, pix, w, h, ox, oy, end_x - ox, end_y - oy);
+}
+
 void Schema::set_ref_image(const std::string &file_name) {
   std::vector<float> pix;
   int w = 0, h = 0;
-  read_image(file_name, pix, w, h);
+  read_with_extents(file_name, ref_ext_, pix, w, h, rox_, roy_);
+  check(dice_hip_set_image_offsets(ctx_, rox_, roy_, dox_, doy_), "set_image_offsets");
   set_image(true, w, h, pix.data(), DICE_STORAGE_F32);
   ref_name_ = file_name;
 }
 void Schema::set_def_image(const std::string &file_name) {
   std::vector<float> pix;
   int w = 0, h = 0;
-  read_image(file_name, pix, w, h);
+  read_with_extents(file_name, def_ext_, pix, w, h, dox_, doy_);
+  check(dice_hip_set_image_offsets(ctx_, rox_, roy_, dox_, doy_), "set_image_offsets");
   set_image(false, w, h, pix.data(), DICE_STORAGE_F32);
   def_name_ = file_name;
 }
@@ -835,8 +1178,41 @@ void Schema::set_def_image(const std::string &file_name) {
 int Schema::execute_correlation() {
   if (n_ <= 0) throw std::runtime_error("Error, the schema has no subsets");
   const int mode = backend_ == HIP_PLAIN ? DICE_MODE_PLAIN : DICE_MODE_BATCHED;
-  check(dice_hip_correlate(ctx_, mode, fields_.data(), frame_id_ == first_frame_id_ ? 1 : 0),
-        "execute_correlation");
+  const int first = frame_id_ == first_frame_id_ ? 1 : 0;
+  upload_force_flags();
+  // TRACKING_ROUTINE with conformal blockers: the subsets in id order, each blocked by its blockers'
+  // current positions (a lower-id blocker's result of this frame).  The subsets go in runs in which no
+  // subset has a conformal blocker earlier in the same run; each run is correlated with the masks its
+  // blockers give at the start of the run, and only the run's results are kept.
+  bool rounds = false;
+  if (correlation_routine() == TRACKING_ROUTINE)
+    for (const auto &kv : obstructing_ids_)
+      for (int b : kv.second) rounds = rounds || defs_.count(b) != 0;
+  if (!rounds) {
+    check(dice_hip_correlate(ctx_, mode, fields_.data(), first), "execute_correlation");
+  } else {
+    std::vector<double> work;
+    int begin = 0;
+    while (begin < n_) {
+      int end = begin + 1;
+      for (; end < n_; ++end) {
+        auto it = obstructing_ids_.find(end);
+        bool dep = false;
+        if (it != obstructing_ids_.end())
+          for (int b : it->second) dep = dep || (b >= begin && b < end && defs_.count(b) != 0);
+        if (dep) break;
+      }
+      std::vector<std::vector<std::pair<int, int>>> blocked(n_);
+      for (int i = begin; i < end; ++i) blocked[i] = blocking_pixels(i);
+      upload_masks(blocked);
+      work = fields_;
+      check(dice_hip_correlate(ctx_, mode, work.data(), first), "execute_correlation");
+      for (int s = 0; s < DICE_F_COUNT; ++s)
+        for (int i = begin; i < end; ++i) fields_[(size_t)s * n_ + i] = work[(size_t)s * n_ + i];
+      begin = end;
+    }
+    upload_masks({});
+  }
   // post_execution_tasks: the post-processors of the frame
   if (vsg_window_ != 0) {
     vsg_.assign((size_t)DICE_VSG_COUNT * n_, 0.0);
@@ -1100,3 +1476,57 @@ int dice_polygon_pixels(int n_polygons, const int *vert_offsets, const int *vx, 
 }
 
 }  // extern "C"
+
+extern "C" int dice_correlate_conformal(float points[], float ref_img[], int ref_w, int ref_h, float def_img[],
+                                        int def_w, int def_h, const char *subset_file, const char *param_file,
+                                        bool write_output) {
+  // DICe_api.cpp:155-282: the schema, its subsets and reference image persist from the first call
+  using namespace dice_hip;
+  static std::mutex m;
+  static std::unique_ptr<Schema> schema;
+  std::lock_guard<std::mutex> lock(m);
+  if (ref_w != def_w || ref_h != def_h) return -1;
+  if (!points || !ref_img || !def_img || !subset_file) return -1;
+  const int stride = 9;
+  try {
+    if (!schema) {
+      ParameterList params;
+      tracking_default_params(&params);
+      if (param_file) {
+        // the user's parameters over the tracking defaults (use_tracking_default_params set, :196-199)
+        const ParameterList user = read_correlation_params(param_file);
+        for (const auto &kv : user.values()) params.set(kv.first, kv.second);
+      }
+      const Subset_File_Info info = read_subset_file(subset_file, ref_w, ref_h);
+      const int n = (int)info.coordinates_x.size();
+      if ((int)info.conformal_area_defs.size() != n)
+        throw std::runtime_error("Error there is a mismatch between the number of conformal subsets defined and the "
+                                 "number of coordinate sets");
+      std::unique_ptr<Schema> s(new Schema(info.coordinates_x, info.coordinates_y, -1, &info.conformal_area_defs,
+                                           &params));
+      s->set_obstructing_subset_ids(info.blocking_subset_ids);
+      s->set_force_simplex(info.force_simplex);
+      s->set_ref_image(ref_w, ref_h, ref_img);
+      schema = std::move(s);
+    }
+    const int n = schema->local_num_subsets();
+    // the centroids from the subset file
+    for (int i = 0; i < n; ++i) {
+      points[i * stride + 0] = (float)schema->local_field_value(i, SUBSET_COORDINATES_X_FS);
+      points[i * stride + 1] = (float)schema->local_field_value(i, SUBSET_COORDINATES_Y_FS);
+    }
+    schema->set_def_image(ref_w, ref_h, def_img);
+    const Field_Spec slots[7] = {SUBSET_DISPLACEMENT_X_FS, SUBSET_DISPLACEMENT_Y_FS, ROTATION_Z_FS, SIGMA_FS,
+                                 GAMMA_FS, BETA_FS, STATUS_FLAG_FS};
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 7; ++k) schema->local_field_value(i, slots[k]) = points[i * stride + 2 + k];
+    schema->execute_correlation();
+    if (write_output) schema->write_output("./");
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 7; ++k) points[i * stride + 2 + k] = (float)schema->local_field_value(i, slots[k]);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "dice_correlate_conformal: %s\n", e.what());
+    return -1;
+  }
+  return 0;
+}

@@ -103,13 +103,28 @@ class ParameterList {
 void dice_default_params(ParameterList *params);
 void tracking_default_params(ParameterList *params);
 
+// read_correlation_params (DICe_Parser.cpp:353-430): the <Parameter name=".." type=".." value=".."/>
+// entries of a Teuchos XML parameter file, names lower-cased, enum-valued strings (correlation_routine,
+// interpolation_method, optimization_method, initialization_method, gradient_method) translated to
+// their values, bools as 0 / 1.  use_tracking_default_params is returned in *use_tracking_defaults
+// (may be null).  Parameters that only switch the reference's image output are accepted and ignored;
+// shape_function_type must be AFFINE; any other parameter this path does not read, a nested
+// ParameterList or an unknown enum string throws std::invalid_argument naming it.
+ParameterList read_correlation_params(const std::string &file_name, bool *use_tracking_defaults = nullptr);
+
 // Shapes of conformal subsets (DICe_Shape.h:50-200): the pixels a shape owns,
 // as (y, x) pairs in (y, x) order (std::set<std::pair<int,int>> in the
 // reference), and clearing flags[j] for the pixels (x[j], y[j]) it covers.
+class Affine_Shape_Function;
 class Shape {
  public:
   virtual ~Shape() = default;
   virtual std::vector<std::pair<int, int>> get_owned_pixels() const = 0;
+  // The pixels of the shape moved by a blocking subset's shape function about its centroid (cx, cy)
+  // and stretched by skin_factor about the moved vertices' centroid (get_owned_pixels(shape_function,
+  // cx, cy, skin_factor), DICe_Shape.cpp:136-219, 322-420).
+  virtual std::vector<std::pair<int, int>> get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                            double skin_factor) const = 0;
   virtual void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const = 0;
 };
 
@@ -119,6 +134,8 @@ class Polygon : public Shape {
  public:
   Polygon(const std::vector<int> &coords_x, const std::vector<int> &coords_y);
   std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  std::vector<std::pair<int, int>> get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                    double skin_factor) const override;
   void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
 
  private:
@@ -133,6 +150,9 @@ class Circle : public Shape {
  public:
   Circle(int centroid_x, int centroid_y, double radius);
   std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  // the reference has no deformed circle: throws std::runtime_error, as it does
+  std::vector<std::pair<int, int>> get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                    double skin_factor) const override;
   void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
 
  private:
@@ -147,6 +167,8 @@ class Rectangle : public Shape {
  public:
   Rectangle(int centroid_x, int centroid_y, int width, int height);
   std::vector<std::pair<int, int>> get_owned_pixels() const override;
+  std::vector<std::pair<int, int>> get_owned_pixels(const Affine_Shape_Function &sf, int cx, int cy,
+                                                    double skin_factor) const override;
   void deactivate_pixels(int size, std::vector<uint8_t> &flags, const int *x, const int *y) const override;
 
  private:
@@ -157,8 +179,8 @@ typedef std::vector<std::shared_ptr<Shape>> multi_shape;
 
 // Conformal_Area_Def (DICe_Shape.h): the union of the boundary shapes, with
 // excluded areas whose pixels stay in the subset but are inactive
-// (DICe_Subset.cpp:115-168).  Obstructed areas are kept for the caller; the
-// obstruction tracking of TRACKING_ROUTINE is not part of this path.
+// (DICe_Subset.cpp:115-168), and obstructed areas: static image pixels that
+// deactivate any warped subset pixel rounding onto them (:162-167, 412-415).
 struct Conformal_Area_Def {
   multi_shape boundary;
   multi_shape excluded_area;
@@ -262,6 +284,16 @@ class Schema {
   // The id of the first frame (the reference's first_frame_id_, from
   // start_image_index): frame ids and output file numbers count from it.
   void set_first_frame_index(int index) { first_frame_id_ = frame_id_ = index; }
+  // Schema::set_obstructing_subset_ids (DICe_Schema.h:1076-1082): subset id -> the ids of the subsets
+  // that can block it (BLOCKING_SUBSETS of a subset file).  Every subset in such a list, and every
+  // subset with a non-empty list, runs Nelder-Mead only unless override_force_simplex is set
+  // (generic_correlation_routine, DICe_Schema.cpp:2160-2185); under TRACKING_ROUTINE the pixels of a
+  // conformal blocker, moved by its current shape function and stretched by obstruction_skin_factor,
+  // are blocked for the subsets it blocks (check_for_blocking_subsets, :4529-4556), and the subsets are
+  // correlated in id order (a blocker with a lower id blocks with this frame's result).
+  void set_obstructing_subset_ids(const std::map<int, std::vector<int>> &ids);
+  // Schema::set_force_simplex (DICe_Schema.h:1083-1086): FORCE_SIMPLEX subsets run Nelder-Mead only.
+  void set_force_simplex(const std::vector<int> &ids);
   // NEIGHBOR_ID_FS of every subset (Schema::initialize, DICe_Schema.cpp:1298-1303), the
   // source of the initial guess under USE_NEIGHBOR_VALUES.
   void set_neighbor_ids(const std::vector<int> &neighbor_ids);
@@ -274,6 +306,14 @@ class Schema {
   // name goes into the output header.
   void set_ref_image(const std::string &file_name);
   void set_def_image(const std::string &file_name);
+  // Schema::update_extents (DICe_Schema.cpp:886-985): unless the subsets are conformal or
+  // read_full_images is set, the next file-based set_ref_image / set_def_image read only the part of
+  // the frame the subsets reach -- the bbox of the centroids (reference) or of the centroids moved by
+  // their displacements (deformed), widened by 100 px, the whole extent on a side within 100 px of the
+  // image edge (:200-210, 366-373) -- and hand the sub-image to the device with its offsets
+  // (dice_hip_set_image_offsets).  The frame loop calls it before each image, as DICe_Main.cpp does.
+  void update_extents();
+  bool has_extents() const { return has_extents_; }
 
   // Schema::execute_correlation (DICe_Schema.cpp:1622-1792): all subsets of the
   // frame, then the frame id advances.  Returns 0.
@@ -352,6 +392,22 @@ class Schema {
   std::vector<double> vsg_;  // DICE_VSG_COUNT x n
   bool uncertainty_pp_ = false;
   std::vector<double> unc_;  // 2 x n: UNCERTAINTY, UNCERTAINTY_ANGLE
+  // obstruction and blocking
+  std::map<int, Conformal_Area_Def> defs_;                  // conformal subset definitions
+  std::vector<std::vector<std::pair<int, int>>> obstructed_;  // per subset: (x, y) of its obstructed pixels
+  std::map<int, std::vector<int>> obstructing_ids_;
+  std::vector<int> force_simplex_ids_;
+  bool override_force_simplex_ = false;
+  double skin_factor_ = 1.0;
+  // sub-image extents (x min, x max, y min, y max) and the images' offsets
+  bool read_full_images_ = false, has_extents_ = false;
+  int ref_ext_[4] = {0, 0, 0, 0}, def_ext_[4] = {0, 0, 0, 0};  // std::vector<int_t> in the reference
+  int rox_ = 0, roy_ = 0, dox_ = 0, doy_ = 0;
+  void read_with_extents(const std::string &file_name, const int *ext, std::vector<float> &pix, int &w, int &h,
+                         int &ox, int &oy) const;
+  void upload_force_flags();
+  void upload_masks(const std::vector<std::vector<std::pair<int, int>>> &blocked);
+  std::vector<std::pair<int, int>> blocking_pixels(int subset) const;
 };
 
 // Status_Flag values of DICe.h:609-677 that the correlation records.

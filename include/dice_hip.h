@@ -143,6 +143,32 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
                          int subset_size, const int32_t *conf_offsets, const int32_t *conf_x,
                          const int32_t *conf_y, const uint8_t *conf_active);
 
+/* Per-subset Nelder-Mead override of the current subset set: generic_correlation_routine forces
+ * computeUpdateRobust, with no fallback optimizer, on the subsets of Schema::set_force_simplex
+ * (FORCE_SIMPLEX in a subset file) and on every subset that blocks or is blocked by another
+ * (BLOCKING_SUBSETS; DICe_Schema.cpp:2160-2185, 2197-2204, 2226-2229).  force: n flags, 1 = forced;
+ * NULL clears.  Cleared by dice_hip_set_subsets. */
+int dice_hip_set_force_simplex(dice_hip_context *ctx, const uint8_t *force);
+
+/* Image pixels that block the current subsets: the union of Subset::obstructed_coords_ (the
+ * pixels of a conformal subset's OBSTRUCTED shapes, DICe_Subset.cpp:162-167) and
+ * pixels_blocked_by_other_subsets_ (Schema::check_for_blocking_subsets, DICe_Schema.cpp:4529-4556).
+ * A warped subset pixel whose rounded position is one of subset i's blocked pixels is deactivated
+ * for that evaluation (Subset::initialize, DICe_Subset.cpp:412-419; is_obstructed_pixel :447-460).
+ * Pixels of subset i: (mask_x, mask_y)[mask_offsets[i] .. mask_offsets[i+1]) in global image
+ * coordinates; mask_offsets NULL clears.  Cleared by dice_hip_set_subsets. */
+int dice_hip_set_subset_masks(dice_hip_context *ctx, const int32_t *mask_offsets, const int32_t *mask_x,
+                              const int32_t *mask_y);
+
+/* Image_::offset_x / offset_y (DICe_Image.h:168-176): the global coordinates of the (0, 0) pixel
+ * of the reference and deformed images when they hold a sub-region of the frame (the sub-image
+ * extents of Schema::update_extents, DICe_Schema.cpp:886-985).  Subset coordinates and fields
+ * stay global; every gather subtracts the offset, and the 4-px guard, the field-of-view test and
+ * the noise window use the sub-image bounds, as in Subset::initialize (DICe_Subset.cpp:380-411).
+ * Default 0, 0, 0, 0. */
+int dice_hip_set_image_offsets(dice_hip_context *ctx, int ref_offset_x, int ref_offset_y, int def_offset_x,
+                               int def_offset_y);
+
 /* Schema::execute_correlation (DICe_Schema.cpp:1622-1792, GENERIC_ROUTINE with
  * generic_correlation_routine :2025-2368 per subset).  fields: host SoA,
  * DICE_F_COUNT * n doubles; read (displacements, rotation, strains, SIGMA as
@@ -287,7 +313,7 @@ int dice_hip_get_launch_shape(dice_hip_context *ctx, int *waves_per_group, int *
 int dice_hip_set_launch_shape(dice_hip_context *ctx, int waves_per_group, int groups_per_cu,
                               int lds_floats_per_wave);
 
-/* dice_correlate drop-in (DICe_api.h:129-133, DICe_api.cpp:51-153): stride-9
+/* dice_correlate (include/dice_api.h) with an int flag: (DICe_api.h:129-133, DICe_api.cpp:51-153): stride-9
  * points [x, y, u, v, theta, sigma, gamma, beta, status], float images of equal
  * size, a static context created on the first call with the given parameters
  * (NULL = tracking defaults) and kept between calls; the parameters are

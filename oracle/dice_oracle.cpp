@@ -285,6 +285,9 @@ struct Subset {
   std::vector<S> ref, def, gx, gy;
   std::vector<char> active, deact;
   bool has_gradients = false;
+  // obstructed_coords_ and pixels_blocked_by_other_subsets_ (DICe_Subset.h), merged: image pixels
+  // (y, x) that deactivate a warped pixel rounding onto them
+  std::set<std::pair<int, int>> blocked;
 
   void alloc() {
     ref.assign(n, S(0)); def.assign(n, S(0)); gx.assign(n, S(0)); gy.assign(n, S(0));
@@ -346,6 +349,17 @@ struct Subset {
       if (px < ox + 4 || px >= ox + w - 4 || py < oy + 4 || py >= oy + h - 4) {
         deact[i] = 1;
         continue;
+      }
+      if (!blocked.empty()) {
+        // is_obstructed_pixel (:447-460): c = (int)x, +1 when the fraction is >= 0.5; the blocked-pixel
+        // test (:412-419) uses (px, py), the same pixel for the positions the guard lets through
+        int c_x = (int)mx, c_y = (int)my;
+        if (mx - (int)mx >= 0.5) c_x++;
+        if (my - (int)my >= 0.5) c_y++;
+        if (blocked.count(std::make_pair(c_y, c_x)) || blocked.count(std::make_pair(py, px))) {
+          deact[i] = 1;
+          continue;
+        }
       }
       deact[i] = 0;
       im.interp_all(interp, def[i], gx[i], gy[i], cg, mx - sox, my - soy);
@@ -776,9 +790,19 @@ void record_failed(double *F, int n, int gid, int status, int iters) {
 // Schema::generic_correlation_routine (GENERIC_ROUTINE), DICe_Schema.cpp:2025-2368.  sid: the subset
 // whose fields give the initial guess (Field_Value_Initializer::initial_guess,
 // DICe_Initializer.cpp:398-434): gid itself, or its neighbour with USE_NEIGHBOR_VALUES.
+// Per-subset options beyond the subset set: Schema::set_force_simplex and the blocking ids
+// (generic_correlation_routine, DICe_Schema.cpp:2160-2185) as force[i] != 0, and the blocked image
+// pixels of subset i, mask_x/mask_y[mask_off[i] .. mask_off[i + 1]) (Subset::obstructed_coords_ and
+// pixels_blocked_by_other_subsets_).  Either may be null.
+struct Extras {
+  const unsigned char *force = nullptr;
+  const int *mask_off = nullptr, *mask_x = nullptr, *mask_y = nullptr;
+};
+
 template <typename S, typename T>
 void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<S, T> &defi,
-                   Subset<S> &sub, double *F, int n, int gid, int subset_size, bool first_frame, int sid) {
+                   Subset<S> &sub, double *F, int n, int gid, int subset_size, bool first_frame, int sid,
+                   bool force_simplex = false) {
   auto f = [&](int slot) -> double & { return F[(size_t)slot * n + gid]; };
   // Field-of-view check on the first frame for square subsets (:2033-2049).
   if (subset_size > 0 && first_frame) {
@@ -808,7 +832,7 @@ void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<
   const S prev_u = f(OF_DISP_X), prev_v = f(OF_DISP_Y), prev_t = f(OF_ROT_Z);
   const int om = P->optimization_method;
   int corr = CORRELATION_FAILED;
-  if (om == SIMPLEX || om == SIMPLEX_THEN_GRADIENT_BASED) corr = obj.update_robust(sf, num_iterations);
+  if (om == SIMPLEX || om == SIMPLEX_THEN_GRADIENT_BASED || force_simplex) corr = obj.update_robust(sf, num_iterations);
   else corr = obj.update_fast(sf, num_iterations);
   auto jump_fail = [&]() {
     const S du = sf.p[sf.iu] - prev_u, dv = sf.p[sf.iv] - prev_v, dt = sf.theta() - prev_t;
@@ -816,7 +840,7 @@ void correlate_one(const oracle_params *P, const Image<S, T> &refi, const Image<
   };
   if (corr != CORRELATION_SUCCESSFUL || jump_fail()) {
     bool second_failed = false;
-    if (om == SIMPLEX || om == GRADIENT_BASED) {
+    if (om == SIMPLEX || om == GRADIENT_BASED || force_simplex) {
       second_failed = true;
     } else if (om == GRADIENT_BASED_THEN_SIMPLEX || om == GRADIENT_THEN_SEARCH) {
       init_from_fields(sf, F, n, sid);
@@ -895,7 +919,7 @@ int gauss_filter_t(int M, int w, int h, T *img) {
 template <typename S, typename T>
 int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def, int n, int subset_size,
                 const int *co, const int *cxs, const int *cys, const unsigned char *cact, double *F,
-                int first_frame, int threads) {
+                int first_frame, int threads, const Extras &ex) {
   if (!P || !ref || !def || !F || n < 1 || w < 1 || h < 1) return -1;
   if (!co && subset_size < 1) return -1;
   Image<S, T> refi, defi;
@@ -934,7 +958,10 @@ int correlate_t(const oracle_params *P, int w, int h, const T *ref, const T *def
       sub.make_square(cx, cy, subset_size);
     }
     sub.init_ref(refi);
-    correlate_one<S, T>(P, refi, defi, sub, F, n, gid, co ? 0 : subset_size, first_frame != 0, sid);
+    if (ex.mask_off)
+      for (int k = ex.mask_off[gid]; k < ex.mask_off[gid + 1]; ++k) sub.blocked.insert({ex.mask_y[k], ex.mask_x[k]});
+    correlate_one<S, T>(P, refi, defi, sub, F, n, gid, co ? 0 : subset_size, first_frame != 0, sid,
+                        ex.force && ex.force[gid]);
   };
   const bool neighbors = P->initialization_method == 1 || (P->initialization_method == 2 && first_frame);
   if (neighbors) {
@@ -1025,31 +1052,61 @@ int objective_t(const oracle_params *P, int w, int h, const T *ref, const T *def
 template <typename S>
 int correlate_any(const oracle_params *P, int w, int h, int storage, const void *ref, const void *def, int n,
                   int subset_size, const int *co, const int *cx, const int *cy, const unsigned char *ca,
-                  double *F, int first_frame, int threads) {
+                  double *F, int first_frame, int threads, const Extras &ex = Extras{}) {
   if (!ref || !def || w < 1 || h < 1) return -1;
   if (storage == 1)
     return correlate_t<S, uint16_t>(P, w, h, (const uint16_t *)ref, (const uint16_t *)def, n, subset_size, co,
-                                    cx, cy, ca, F, first_frame, threads);
+                                    cx, cy, ca, F, first_frame, threads, ex);
   if (storage == 0 && sizeof(S) == sizeof(float))
     return correlate_t<S, S>(P, w, h, (const S *)ref, (const S *)def, n, subset_size, co, cx, cy, ca, F,
-                             first_frame, threads);
+                             first_frame, threads, ex);
   if (storage == 2 && sizeof(S) == sizeof(double))
     return correlate_t<S, S>(P, w, h, (const S *)ref, (const S *)def, n, subset_size, co, cx, cy, ca, F,
-                             first_frame, threads);
+                             first_frame, threads, ex);
   if (storage == 0) {
     const size_t np = (size_t)w * h;
     std::vector<S> r(np), d(np);
     for (size_t i = 0; i < np; ++i) { r[i] = ((const float *)ref)[i]; d[i] = ((const float *)def)[i]; }
-    return correlate_t<S, S>(P, w, h, r.data(), d.data(), n, subset_size, co, cx, cy, ca, F, first_frame, threads);
+    return correlate_t<S, S>(P, w, h, r.data(), d.data(), n, subset_size, co, cx, cy, ca, F, first_frame, threads,
+                             ex);
   }
   return -1;
 }
 
-double angle_2d(double x1, double y1, double x2, double y2) {  // DICe_Shape.cpp:88-103
-  double d = std::atan2(y2, x2) - std::atan2(y1, x1);
+// angle_2d, DICe_Shape.cpp:88-103, in scalar_t S: std::atan2 of S, the wrap against
+// the double constants rounded back to S.
+template <typename S>
+S angle_2d(S x1, S y1, S x2, S y2) {
+  S d = std::atan2(y2, x2) - std::atan2(y1, x1);
   while (d > kPi) d -= kTwoPi;
   while (d < -kPi) d += kTwoPi;
   return d;
+}
+
+// Polygon ctor + get_owned_pixels (no map), DICe_Shape.cpp:50-80,136-219, in S: the
+// winding angle of each pixel of the vertex bbox, |sum| >= pi inside.
+template <typename S>
+int polygon_pixels_t(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y) {
+  if (nv < 3) return -1;
+  std::vector<int> X(vx, vx + nv), Y(vy, vy + nv);
+  X.push_back(vx[0]); Y.push_back(vy[0]);
+  int mnx = X[0], mny = Y[0], mxx = X[0], mxy = Y[0];
+  for (int i = 0; i < nv; ++i) {
+    mnx = std::min(mnx, X[i]); mny = std::min(mny, Y[i]);
+    mxx = std::max(mxx, X[i]); mxy = std::max(mxy, Y[i]);
+  }
+  int count = 0;
+  for (int y = mny; y <= mxy; ++y)
+    for (int x = mnx; x <= mxx; ++x) {
+      S angle = 0.0;
+      for (int i = 0; i < nv; ++i)
+        angle += angle_2d<S>((S)(X[i] - x), (S)(Y[i] - y), (S)(X[i + 1] - x), (S)(Y[i + 1] - y));
+      if (std::abs(angle) >= kPi) {
+        if (count < cap) { out_x[count] = x; out_y[count] = y; }
+        ++count;
+      }
+    }
+  return count;
 }
 
 }  // namespace
@@ -1164,26 +1221,10 @@ double oracle_subset_gamma_f(int w, int h, const float *ref, const float *def, i
 }
 
 int oracle_polygon_pixels(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y) {
-  // Polygon ctor + get_owned_pixels (no map), DICe_Shape.cpp:50-80,136-219.
-  if (nv < 3) return -1;
-  std::vector<int> X(vx, vx + nv), Y(vy, vy + nv);
-  X.push_back(vx[0]); Y.push_back(vy[0]);
-  int mnx = X[0], mny = Y[0], mxx = X[0], mxy = Y[0];
-  for (int i = 0; i < nv; ++i) {
-    mnx = std::min(mnx, X[i]); mny = std::min(mny, Y[i]);
-    mxx = std::max(mxx, X[i]); mxy = std::max(mxy, Y[i]);
-  }
-  int count = 0;
-  for (int y = mny; y <= mxy; ++y)
-    for (int x = mnx; x <= mxx; ++x) {
-      double angle = 0.0;
-      for (int i = 0; i < nv; ++i) angle += angle_2d(X[i] - x, Y[i] - y, X[i + 1] - x, Y[i + 1] - y);
-      if (std::abs(angle) >= kPi) {
-        if (count < cap) { out_x[count] = x; out_y[count] = y; }
-        ++count;
-      }
-    }
-  return count;
+  return polygon_pixels_t<double>(nv, vx, vy, cap, out_x, out_y);
+}
+int oracle_polygon_pixels_f(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y) {
+  return polygon_pixels_t<float>(nv, vx, vy, cap, out_x, out_y);
 }
 
 int oracle_correlate_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
@@ -1195,6 +1236,30 @@ int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const 
                        int n, int subset_size, const int *co, const int *cx, const int *cy,
                        const unsigned char *ca, double *fields, int first_frame, int threads) {
   return correlate_any<double>(p, w, h, storage, ref, def, n, subset_size, co, cx, cy, ca, fields, first_frame, threads);
+}
+int oracle_correlate_ex_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                          int n, int subset_size, const int *co, const int *cx, const int *cy,
+                          const unsigned char *ca, double *fields, int first_frame, int threads,
+                          const unsigned char *force, const int *mask_off, const int *mask_x, const int *mask_y) {
+  Extras ex;
+  ex.force = force;
+  ex.mask_off = mask_off;
+  ex.mask_x = mask_x;
+  ex.mask_y = mask_y;
+  return correlate_any<float>(p, w, h, storage, ref, def, n, subset_size, co, cx, cy, ca, fields, first_frame, threads,
+                              ex);
+}
+int oracle_correlate_ex_d(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                          int n, int subset_size, const int *co, const int *cx, const int *cy,
+                          const unsigned char *ca, double *fields, int first_frame, int threads,
+                          const unsigned char *force, const int *mask_off, const int *mask_x, const int *mask_y) {
+  Extras ex;
+  ex.force = force;
+  ex.mask_off = mask_off;
+  ex.mask_x = mask_x;
+  ex.mask_y = mask_y;
+  return correlate_any<double>(p, w, h, storage, ref, def, n, subset_size, co, cx, cy, ca, fields, first_frame,
+                               threads, ex);
 }
 
 int oracle_gauss_filter_f(int w, int h, int storage, void *img, int mask_size) {

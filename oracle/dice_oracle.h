@@ -108,8 +108,11 @@ void oracle_apply_transform_f(int w, int h, const float *in, float *out, int cx,
                               float u, float v, float theta);
 /* Un-warped subset gamma (Subset::initialize without a map, then Subset::gamma). */
 double oracle_subset_gamma_f(int w, int h, const float *ref, const float *def, int cx, int cy, int size);
-/* Polygon::get_owned_pixels: writes up to cap (x,y) pairs in (y,x) order; returns the count. */
+/* Polygon::get_owned_pixels: writes up to cap (x,y) pairs in (y,x) order; returns the count.
+ * The winding angles in double (the reference's DICE_USE_DOUBLE build) or, _f, in float (its
+ * default build); they differ on pixels that lie on an edge. */
 int oracle_polygon_pixels(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y);
+int oracle_polygon_pixels_f(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y);
 
 /*
  * One Schema::execute_correlation (GENERIC_ROUTINE) over n subsets.
@@ -129,6 +132,20 @@ int oracle_correlate_d(const oracle_params *p, int w, int h, int storage, const 
                        int n, int subset_size, const int *conf_offsets, const int *conf_x,
                        const int *conf_y, const unsigned char *conf_active, double *fields,
                        int first_frame, int threads);
+
+/* The same with per-subset options: force[i] != 0 runs Nelder-Mead only on subset i (Schema::set_force_simplex,
+ * and the subsets in BLOCKING_SUBSETS lists, DICe_Schema.cpp:2160-2185); mask_x/mask_y[mask_off[i] ..
+ * mask_off[i+1]) are image pixels that deactivate a warped pixel of subset i rounding onto them
+ * (Subset::obstructed_coords_ and pixels_blocked_by_other_subsets_, DICe_Subset.cpp:162-167,412-419,447-460).
+ * Either may be null. */
+int oracle_correlate_ex_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                          int n, int subset_size, const int *conf_offsets, const int *conf_x, const int *conf_y,
+                          const unsigned char *conf_active, double *fields, int first_frame, int threads,
+                          const unsigned char *force, const int *mask_off, const int *mask_x, const int *mask_y);
+int oracle_correlate_ex_d(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
+                          int n, int subset_size, const int *conf_offsets, const int *conf_x, const int *conf_y,
+                          const unsigned char *conf_active, double *fields, int first_frame, int threads,
+                          const unsigned char *force, const int *mask_off, const int *mask_x, const int *mask_y);
 
 /* Image_::gauss_filter in place (mask 5, 7, 9, 11 or 13): float arithmetic over float32
  * (storage 0) or uint16 (storage 1) pixels, or double arithmetic over double pixels.

@@ -72,6 +72,8 @@ def lib():
         i = ctypes.c_int
         _lib.oracle_correlate_f.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i]
         _lib.oracle_correlate_d.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i]
+        _lib.oracle_correlate_ex_f.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i, P, P, P, P]
+        _lib.oracle_correlate_ex_d.argtypes = [P, i, i, i, P, P, i, i, P, P, P, P, P, i, i, P, P, P, P]
         _lib.oracle_subset_gamma_f.restype = ctypes.c_double
         _lib.oracle_subset_gamma_f.argtypes = [i, i, P, P, i, i, i]
         _lib.oracle_apply_transform_d.argtypes = [i, i, P, P, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -81,6 +83,7 @@ def lib():
         _lib.oracle_interp_f.argtypes = [i, i, i, P, i, P, P, P]
         _lib.oracle_interp_d.argtypes = [i, i, i, P, i, P, P, P]
         _lib.oracle_polygon_pixels.argtypes = [i, P, P, i, P, P]
+        _lib.oracle_polygon_pixels_f.argtypes = [i, P, P, i, P, P]
         _lib.oracle_lu_inverse.argtypes = [i, P]
         _lib.oracle_smooth_gradients_f.argtypes = [i, i, P, P]
         _lib.oracle_nelder_mead.argtypes = [i, i, P, P, P, i, ctypes.c_double, ctypes.c_double, i, P, P, P, P]
@@ -119,11 +122,13 @@ def new_fields(cx, cy):
 
 
 def correlate(params, ref, defm, fields, subset_size=0, conformal=None, first_frame=True,
-              precision="double", threads=1):
+              precision="double", threads=1, force_simplex=None, masks=None):
     """One execute_correlation over the subsets in `fields` (modified in place).
 
     ref/defm: 2-D float32, float64 or uint16 arrays.  conformal: None or a tuple
-    (offsets int32[n+1], x int32[], y int32[], active uint8[]).
+    (offsets int32[n+1], x int32[], y int32[], active uint8[]).  force_simplex: None or
+    uint8[n] (Nelder-Mead only where set).  masks: None or (offsets int32[n+1], x int32[],
+    y int32[]) of the image pixels that block each subset's warped pixels.
     """
     h, w = ref.shape
     if ref.dtype == np.uint16:
@@ -142,9 +147,13 @@ def correlate(params, ref, defm, fields, subset_size=0, conformal=None, first_fr
     if conformal is not None:
         co, cx, cy, ca = [np.ascontiguousarray(a) for a in conformal]
         co = co.astype(np.int32); cx = cx.astype(np.int32); cy = cy.astype(np.int32); ca = ca.astype(np.uint8)
-    fn = lib().oracle_correlate_d if precision == "double" else lib().oracle_correlate_f
+    fo = None if force_simplex is None else np.ascontiguousarray(force_simplex, dtype=np.uint8)
+    mo = mx = my = None
+    if masks is not None:
+        mo, mx, my = [np.ascontiguousarray(a, dtype=np.int32) for a in masks]
+    fn = lib().oracle_correlate_ex_d if precision == "double" else lib().oracle_correlate_ex_f
     rc = fn(ctypes.byref(params), w, h, storage, _p(ref), _p(defm), n, subset_size, _p(co), _p(cx), _p(cy),
-            _p(ca), _p(fields), 1 if first_frame else 0, threads)
+            _p(ca), _p(fields), 1 if first_frame else 0, threads, _p(fo), _p(mo), _p(mx), _p(my))
     if rc != 0:
         raise ValueError("oracle_correlate rejected its arguments")
     return fields
@@ -190,13 +199,16 @@ def subset_gamma(ref, defm, cx, cy, size):
     return lib().oracle_subset_gamma_f(w, h, _p(ref), _p(defm), cx, cy, size)
 
 
-def polygon_pixels(vx, vy):
+def polygon_pixels(vx, vy, precision="float"):
+    """Polygon::get_owned_pixels: the winding angles in float (the reference's default build) or
+    double (its DICE_USE_DOUBLE build)."""
     vx = np.ascontiguousarray(vx, dtype=np.int32)
     vy = np.ascontiguousarray(vy, dtype=np.int32)
     cap = (int(vx.max() - vx.min()) + 1) * (int(vy.max() - vy.min()) + 1)
     ox = np.zeros(cap, np.int32)
     oy = np.zeros(cap, np.int32)
-    cnt = lib().oracle_polygon_pixels(len(vx), _p(vx), _p(vy), cap, _p(ox), _p(oy))
+    fn = lib().oracle_polygon_pixels_f if precision == "float" else lib().oracle_polygon_pixels
+    cnt = fn(len(vx), _p(vx), _p(vy), cap, _p(ox), _p(oy))
     return ox[:cnt], oy[:cnt]
 
 

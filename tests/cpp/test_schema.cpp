@@ -8,6 +8,7 @@
 //   test_schema polygon            (prints the owned pixels "y x" of the golden's polygons)
 //   test_schema subset_file_info <subset file>   (what read_subset_file makes of it)
 //   test_schema deck <config>      (a regression deck end to end, see deck())
+//   test_schema read_params <xml>  (read_correlation_params: "name value" lines)
 //   test_schema grid <w> <h> <subset size> <step> [subset file]   (prints "x y neighbor" of the
 //                                  regular grid, then "seed id u v" for each seed)
 //   test_schema errors             (error behaviour that needs no GPU)
@@ -250,7 +251,14 @@ int deck(const char *config_path) {
     cx = info.coordinates_x;
     cy = info.coordinates_y;
   }
+  if (c.count("override_force_simplex")) params.set("override_force_simplex", num("override_force_simplex"));
+  if (c.count("read_full_images")) params.set("read_full_images", num("read_full_images"));
   Schema schema(cx, cy, num("subset_size"), grid ? nullptr : &info.conformal_area_defs, &params);
+  if (!grid) {
+    // Schema(input_params, correlation_params) with a subset file (DICe_Schema.cpp:1052-1068)
+    schema.set_obstructing_subset_ids(info.blocking_subset_ids);
+    schema.set_force_simplex(info.force_simplex);
+  }
   if (grid) {
     schema.set_step_sizes(num("step_size"), num("step_size"));
     schema.set_neighbor_ids(neighbors);
@@ -267,8 +275,13 @@ int deck(const char *config_path) {
     std::string f;
     while (std::getline(ss, f, ',')) def_files.push_back(f);
   }
-  if (files) schema.set_ref_image(c.at("ref_file"));
-  else schema.set_ref_image(w, h, ref.data());
+  // DICe_Main.cpp:314-330: the extents are updated before the reference and each deformed image is read
+  if (files) {
+    schema.update_extents();
+    schema.set_ref_image(c.at("ref_file"));
+  } else {
+    schema.set_ref_image(w, h, ref.data());
+  }
   schema.set_image_file_names(c.at("ref_name"), c.at("def_name"));
   std::vector<std::string> fields;
   if (c.count("fields")) {
@@ -284,6 +297,7 @@ int deck(const char *config_path) {
     while (std::getline(ss, f, ',')) def_names.push_back(f);
   }
   for (int frame = 0; frame < frames; ++frame) {
+    if (files) schema.update_extents();
     if (!def_files.empty()) schema.set_def_image(def_files.at(frame));
     else if (files) schema.set_def_image(c.at("def_file"));
     else schema.set_def_image(w, h, def.data());
@@ -292,6 +306,7 @@ int deck(const char *config_path) {
     schema.write_output(c.at("out_dir") + "/", c.at("prefix"), num("separate") != 0, fields, delimiter, omit_row_id,
                         c.count("num_frames") ? num("num_frames") : frames);
   }
+  std::cout << "extents " << (schema.has_extents() ? 1 : 0) << "\n";
   for (int i = 0; i < schema.local_num_subsets(); ++i)
     std::cout << "subset " << i << " x " << cx[i] << " y " << cy[i] << " u "
               << schema.local_field_value(i, SUBSET_DISPLACEMENT_X_FS) << " v "
@@ -337,6 +352,13 @@ int main(int argc, char **argv) {
     if (cmd == "errors") return errors_without_gpu();
     if (cmd == "subset_file_info" && argc >= 3) return subset_file_info(argv[2]);
     if (cmd == "deck" && argc >= 3) return deck(argv[2]);
+    if (cmd == "read_params" && argc >= 3) {
+      bool tracking = false;
+      const ParameterList pl = read_correlation_params(argv[2], &tracking);
+      for (const auto &kv : pl.values()) std::cout << kv.first << " " << kv.second << "\n";
+      std::cout << "use_tracking_default_params " << (tracking ? 1 : 0) << "\n";
+      return 0;
+    }
     if (cmd == "objective" && argc >= 11) {
       const int w = std::atoi(argv[4]), h = std::atoi(argv[5]), S = std::atoi(argv[6]);
       const std::vector<float> ref = read_raw(argv[2], w, h), def = read_raw(argv[3], w, h);

@@ -29,10 +29,10 @@ def to_oracle(p):
 
 
 def run_oracle(p, ref, de, cx, cy, subset_size, fields=None, conformal=None, first_frame=True,
-               precision="double"):
+               precision="double", force_simplex=None, masks=None):
     f = o.new_fields(cx, cy) if fields is None else np.ascontiguousarray(fields).copy()
     o.correlate(to_oracle(p), ref, de, f, subset_size=subset_size, conformal=conformal, first_frame=first_frame,
-                precision=precision, threads=oracle_threads())
+                precision=precision, threads=oracle_threads(), force_simplex=force_simplex, masks=masks)
     return f
 
 
@@ -62,3 +62,50 @@ def csr_take(conf, idx):
     np.cumsum(cnt, out=new_off[1:])
     sel = np.concatenate([np.arange(off[i], off[i + 1]) for i in idx])
     return new_off, px[sel], py[sel], act[sel]
+
+
+def compare_spread(fg, fd, ff, label="", nelder_mead=False):
+    """The GPU (float data, FP64 sums) against the double oracle, with the reference algorithm's own
+    float-vs-double spread (the oracle's float build: fd vs ff) as the allowance where that spread
+    exceeds the bar.  Options that slow the Gauss-Newton convergence (momentum, smoothed gradients)
+    move the iteration at which |du| first drops below the tolerance, and the two reference builds
+    already stop one iteration apart on some subsets; a Nelder-Mead solve forks on rounding at any
+    comparison of nearly equal gamma values, even at equal iteration counts.
+    * statuses: agreement with the double build >= min(0.999, float-vs-double agreement - 0.005);
+    * share of subsets converged on all three with the double build's iteration count >= the float
+      build's share - 0.01;
+    * Gauss-Newton (nelder_mead False), on those same-iteration subsets: shape parameters within 1e-4
+      and gamma within max(1e-5, 2x the float build's own deviation there); on all converged subsets
+      the GPU's deviation at most twice the float build's, plus 1e-4 px;
+    * Nelder-Mead: the median and 95th percentile of the GPU's deviation at most twice the float
+      build's, plus 1e-5 / 1e-4 px.
+    Returns the mask of subsets converged on all three."""
+    sg, sd, sf = fg[dh.F["STATUS"]], fd[o.F["STATUS"]], ff[o.F["STATUS"]]
+    inherent = float(np.mean(sf == sd))
+    agree = float(np.mean(sg == sd))
+    assert agree >= min(STATUS_AGREE, inherent - 0.005), "%s status agreement %.4f (float vs double %.4f)" % (
+        label, agree, inherent)
+    conv = (np.isin(sg, dh.CONVERGED_STATUSES) & np.isin(sd, dh.CONVERGED_STATUSES) &
+            np.isin(sf, dh.CONVERGED_STATUSES))
+    it_g, it_d, it_f = fg[dh.F["ITERATIONS"]], fd[o.F["ITERATIONS"]], ff[o.F["ITERATIONS"]]
+    same_g, same_f = conv & (it_g == it_d), conv & (it_f == it_d)
+    if not nelder_mead:
+        assert same_g.sum() >= (same_f.sum() / max(conv.sum(), 1) - 0.01) * conv.sum(), label
+    for name in SHAPE_FIELDS:
+        dg = np.abs(fg[dh.F[name]] - fd[o.F[name]])
+        df = np.abs(ff[o.F[name]] - fd[o.F[name]])
+        if not conv.any():
+            continue
+        if nelder_mead:
+            assert np.median(dg[conv]) <= 2.0 * np.median(df[conv]) + 1e-5, (label, name)
+            assert np.percentile(dg[conv], 95) <= 2.0 * np.percentile(df[conv], 95) + 1e-4, (label, name)
+        else:
+            assert dg[same_g].size == 0 or dg[same_g].max() <= U_TOL, "%s %s %.3g" % (label, name, dg[same_g].max())
+            assert dg[conv].max() <= 2.0 * df[conv].max() + U_TOL, (label, name)
+    if not nelder_mead:
+        rg = np.abs(fg[dh.F["GAMMA"]] - fd[o.F["GAMMA"]]) / np.abs(fd[o.F["GAMMA"]])
+        rf = np.abs(ff[o.F["GAMMA"]] - fd[o.F["GAMMA"]]) / np.abs(fd[o.F["GAMMA"]])
+        bar = max(GAMMA_RTOL, 2.0 * (rf[same_f].max() if same_f.any() else 0.0))
+        assert rg[same_g].size == 0 or rg[same_g].max() <= bar, "%s gamma %.3g > %.3g" % (label, rg[same_g].max(),
+                                                                                        bar)
+    return conv

@@ -34,6 +34,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_reference_entry_points_are_exported():
+    """include/dice_api.h: dice_correlate from libdice_hip.so, dice_correlate_conformal from
+    libdice_schema.so, under the reference's names (DICe_api.h:129-133, 176-180)."""
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "dice_api.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(dice_correlate\w*)\s*\(", text)))
+    assert names == ["dice_correlate", "dice_correlate_conformal"]
+    lib_dir = os.path.join(ROOT, "dice_amd", "lib")
+    if not os.path.exists(os.path.join(lib_dir, "libdice_schema.so")):
+        pytest.skip("libraries not built")
+    assert hasattr(ctypes.CDLL(os.path.join(lib_dir, "libdice_hip.so")), "dice_correlate")
+    assert hasattr(ctypes.CDLL(os.path.join(lib_dir, "libdice_schema.so")), "dice_correlate_conformal")
+
+
 def test_create_without_gpu_fails_cleanly():
     """With no visible GPU, context creation reports an error code instead of crashing."""
     if not os.path.exists(dh.LIB_PATH):

@@ -44,14 +44,15 @@ def test_cpp_polygon_raster_matches_oracle_and_golden():
         ys.append(py)
     oracle = np.stack([np.concatenate(ys), np.concatenate(xs)], axis=1)
     assert np.array_equal(pts, oracle)
-    g = np.load(os.path.join(GOLD, "conformal_raster.npz"))["conformal_d"]
+    # the float build's golden (the library rasterises in float, as the reference's default build)
+    g = np.load(os.path.join(GOLD, "conformal_raster.npz"))["conformal_f"]
     union = sorted(set(map(tuple, pts.tolist())))
     min_y = min(p[0] for p in union)
     min_x = min(p[1] for p in union)
     mask = np.zeros(g.shape, bool)
     for yy, xx in union:
         mask[yy - min_y, xx - min_x] = True
-    assert np.all(mask[g != 0])
+    assert np.array_equal(mask, g != 0)
 
 
 def test_batch_polygon_raster_matches_oracle():
@@ -443,7 +444,7 @@ def test_oracle_on_regression_deck():
         assert f[o.F["STATUS"], i] == g[10] == 1
         for val, gv, tol in ((u, g[3], 5e-5), (v, g[4], 5e-5), (sig, g[9], 2e-6)):
             assert abs(val - gv) <= 0.05 * abs(gv)
-            if i in (0, 1, 4):
+            if i in (0, 1, 2, 4):
                 assert abs(val - gv) <= tol
 
 
@@ -515,8 +516,8 @@ def test_vsg_regression_deck(tmp_path, backend):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
-@pytest.mark.parametrize("name", ["conformal_subset_def", "single_speckled_shift", "parallel_subsets",
-                                  "parallel_subsets_seed", "parallel_seed_first_step_only"])
+@pytest.mark.parametrize("name", ["conformal_subset_def", "conformal_subset_def_override", "single_speckled_shift",
+                                  "parallel_subsets", "parallel_subsets_seed", "parallel_seed_first_step_only"])
 def test_regression_deck(tmp_path, name, backend):
     """A deck of tests/regression end to end through the C++ Schema mirror -- subsets from its subset file
     (read_subset_file) or its regular grid (create_regular_grid_of_correlation_points), the correlation,
@@ -524,6 +525,15 @@ def test_regression_deck(tmp_path, name, backend):
     the regression suite's options (-t 0.05 -f 5.0E-5, header strings included).  Square subsets also
     within the tolerances of the gold tests of tests/test_gpu_parity.py: u, v 5e-5, sigma and gamma 2e-6."""
     import json
+    # conformal_subset_def: subset 3 is blocked by the square subsets 0 and 4, and has an OBSTRUCTED polygon.
+    # With the reference's present logic (dice_default_params: override_force_simplex false) subsets 0, 3
+    # and 4 run Nelder-Mead only, and the deck passes at the suite's tolerance; its gold holds the
+    # Gauss-Newton results for 0 and 4 (within 1e-5 px), so it was written before blocking subsets forced
+    # the simplex: with override_force_simplex the square subsets and the polygon subset 2 match it at
+    # 5e-5, and the obstructed subset 3 (whose gold sigma the obstruction reproduces to 5e-6) at the
+    # suite's tolerance.
+    override = name.endswith("_override")
+    name = name.replace("_override", "")
     ddir = os.path.join(DECKS, name)
     deck = json.load(open(os.path.join(ddir, "deck.json")))
     d = np.load(os.path.join(GOLD, "speckle_512.npz"))
@@ -532,6 +542,8 @@ def test_regression_deck(tmp_path, name, backend):
            "height": h, "subset_size": deck["subset_size"], "out_dir": tmp_path, "prefix": deck["prefix"],
            "separate": int(deck["separate"]), "backend": backend, "ref_name": deck["reference_image"],
            "def_name": deck["deformed_image"], "routine": deck.get("correlation_routine", 0)}
+    if override:
+        cfg["override_force_simplex"] = 1
     if "subset_file" in deck:
         cfg["subset_file"] = os.path.join(ddir, deck["subset_file"])
     if "step_size" in deck:
@@ -556,8 +568,10 @@ def test_regression_deck(tmp_path, name, backend):
         ours = [ln.split() for ln in open(tmp_path / g) if ln[:1].isdigit()]
         header = [ln.split() for ln in open(os.path.join(ddir, "gold", g)) if ln.startswith(("FRAME", "SUBSET_ID"))][0]
         for gr, orow in zip(gold_rows, ours):
-            if name == "conformal_subset_def" and g in ("DICe_solution_2.txt", "DICe_solution_3.txt"):
-                continue  # polygon subsets: within the suite's tolerance (above)
+            if name == "conformal_subset_def" and g in (("DICe_solution_3.txt",) if override else
+                                                          ("DICe_solution_0.txt", "DICe_solution_3.txt",
+                                                           "DICe_solution_4.txt")):
+                continue  # within the suite's tolerance (above)
             for col, tol in (("DISPLACEMENT_X", 5e-5), ("DISPLACEMENT_Y", 5e-5), ("SIGMA", 2e-6), ("GAMMA", 2e-6)):
                 if col in header:
                     k = header.index(col)
@@ -604,3 +618,79 @@ def test_unicode_filename_deck(tmp_path, backend):
         assert g.shape == ours.shape == (100, 4)
         assert np.array_equal(ours[:, :2], g[:, :2])
         assert np.abs(ours[:, 2:4]).max() <= 1e-6
+
+
+def test_read_correlation_params(tmp_path):
+    """read_correlation_params (DICe_Parser.cpp:353-430): names lower-cased, enum strings to their values,
+    bools to 0 / 1, output-only switches ignored; parameters this path does not read, nested lists and
+    unknown enum strings are errors."""
+    r = run("read_params", os.path.join(GOLD, "api", "params.xml"))
+    assert r.returncode == 0, r.stderr
+    got = dict(ln.split() for ln in r.stdout.splitlines())
+    assert got == {"interpolation_method": "2", "optimization_method": "1", "enable_rotation": "1",
+                   "max_solver_iterations_fast": "30", "use_tracking_default_params": "0"}
+    f = tmp_path / "p.xml"
+    f.write_text('<ParameterList>\n<Parameter name="correlation_routine" type="string" value="TRACKING_ROUTINE"/>\n'
+                 '<Parameter name="use_tracking_default_params" type="bool" value="true"/>\n</ParameterList>\n')
+    got = dict(ln.split() for ln in run("read_params", f).stdout.splitlines())
+    assert got == {"correlation_routine": "1", "use_tracking_default_params": "1"}
+    for bad in ('<Parameter name="use_incremental_formulation" type="bool" value="true"/>',
+                '<Parameter name="optimization_method" type="string" value="NEWTON"/>',
+                '<ParameterList name="post_process_vsg_strain"></ParameterList>',
+                '<Parameter name="shape_function_type" type="string" value="QUADRATIC"/>'):
+        f.write_text("<ParameterList>\n%s\n</ParameterList>\n" % bad)
+        assert run("read_params", f).returncode != 0, bad
+
+
+@pytest.mark.gpu
+def test_c_caller_resolves_reference_entry_points(tmp_path):
+    """tests/cpp/test_api.c, a plain C program, dlopens the libraries and resolves dice_correlate and
+    dice_correlate_conformal by name (DICe_api.h:129-133, 176-180), then correlates the integer-shift
+    sequence of DICe_TestAPI.cpp:70-150: square subsets with the tracking defaults, and two conformal
+    subsets (a polygon and a rectangle) from a subset file with an XML parameter file; u, v within 1e-4
+    of k, theta and gamma within 1e-4 of 0, status 1."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/test_api is not built (run __graft_entry__.build())")
+    d = np.load(os.path.join(GOLD, "synthetic_speckled.npz"))
+    imgs = [raw(tmp_path, "def%d.f32" % k, d["def%d" % k].astype(np.float32)) for k in range(6)]
+    lib = os.path.join(ROOT, "dice_amd", "lib")
+    r = subprocess.run([exe, os.path.join(lib, "libdice_hip.so"), os.path.join(lib, "libdice_schema.so"), imgs[0]] +
+                       imgs + [os.path.join(GOLD, "api", "subsets.txt"), os.path.join(GOLD, "api", "params.xml")],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0 and "API PASSED" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", [0, 1], ids=["batched", "plain"])
+def test_sub_image_extents_deck(tmp_path, backend):
+    """Schema::update_extents (DICe_Schema.cpp:886-985) through the Schema mirror: square subsets in the
+    middle of the 512^2 speckle pair read from TIFF files, so that only a sub-image of each frame is read
+    and handed to the device with its offsets, against the same deck with read_full_images: identical
+    output, and the subsets' u, v within 5e-5 of the whole-image oracle."""
+    d = np.load(os.path.join(GOLD, "speckle_512.npz"))
+    write_tiff_u8(tmp_path / "ref.tif", d["ref"])
+    write_tiff_u8(tmp_path / "def.tif", d["defm"])
+    pts = [(230, 240), (260, 250), (250, 290), (300, 270), (280, 310)]
+    (tmp_path / "subsets.txt").write_text("BEGIN SUBSET_COORDINATES\n" + "".join("%d %d\n" % p for p in pts) +
+                                          "END SUBSET_COORDINATES\n")
+    outs = []
+    for full in (0, 1):
+        od = tmp_path / ("out%d" % full)
+        od.mkdir()
+        cfg = {"ref_file": tmp_path / "ref.tif", "def_file": tmp_path / "def.tif", "width": 512, "height": 512,
+               "subset_size": 27, "subset_file": tmp_path / "subsets.txt", "out_dir": od, "prefix": "DICe_solution",
+               "separate": 0, "backend": backend, "ref_name": "ref.tif", "def_name": "def.tif",
+               "def_names": "def.tif", "routine": 0, "read_full_images": full}
+        cfg_path = tmp_path / ("deck%d.cfg" % full)
+        cfg_path.write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+        r = run("deck", cfg_path)
+        assert r.returncode == 0, r.stderr
+        assert ("extents %d" % (1 - full)) in r.stdout
+        outs.append(open(od / "DICe_solution_0.txt").read())
+    assert outs[0] == outs[1]
+    f = o.new_fields(np.array([p[0] for p in pts], float), np.array([p[1] for p in pts], float))
+    o.correlate(o.default_params(), d["ref"], d["defm"], f, subset_size=27, precision="double", threads=4)
+    rows = [ln.split() for ln in outs[0].splitlines() if ln[:1].isdigit()]
+    for i, row in enumerate(rows):
+        assert abs(float(row[3]) - f[o.F["DISP_X"], i]) < 5e-5 and abs(float(row[4]) - f[o.F["DISP_Y"], i]) < 5e-5

@@ -301,14 +301,17 @@ def test_max_iterations_then_simplex_fallback():
     assert abs(f[o.F["DISP_X"], 0] - 0.4) < 0.1
 
 
-def test_polygon_raster_against_conformal_golden():
-    """DICe_TestSubset.cpp:248-309: the conformal subset (two polygons, one excluded
-    rectangle) written over its bounding box; non-subset pixels are 0 in the golden."""
-    g = load("conformal_raster.npz")["conformal_d"]
+@pytest.mark.parametrize("precision,key", [("float", "conformal_f"), ("double", "conformal_d")])
+def test_polygon_raster_against_conformal_golden(precision, key):
+    """DICe_TestSubset.cpp:248-309: the conformal subset (two polygons) written over its bounding box;
+    non-subset pixels are 0 in the golden.  The reference's float build (conformal.rawi) and double build
+    (conformal_d.rawi) differ by one pixel on a polygon edge, where the winding angle sums to pi up to
+    rounding: the float and double rasters each reproduce their own golden exactly."""
+    g = load("conformal_raster.npz")[key]
     xs, ys = [], []
     for vx, vy in [([940, 951, 964, 980, 959], [422, 399, 410, 413, 432]),
                    ([982, 1000, 991, 978], [426, 428, 456, 445])]:
-        px, py = o.polygon_pixels(vx, vy)
+        px, py = o.polygon_pixels(vx, vy, precision=precision)
         xs.append(px)
         ys.append(py)
     px = np.concatenate(xs)
@@ -322,7 +325,5 @@ def test_polygon_raster_against_conformal_golden():
     mask = np.zeros((h, w), bool)
     for yy, xx in pts:
         mask[yy - min_y, xx - min_x] = True
-    # every golden non-zero pixel is in the raster, and the raster adds no pixel outside it
-    # other than ones whose (histogram-spread) intensity happens to be 0
-    assert np.all(mask[g != 0])
-    assert (mask & (g == 0)).sum() <= 0.01 * mask.sum()
+    # the speckle image has no zero intensity inside the subset: the non-zero pixels are the subset
+    assert np.array_equal(mask, g != 0)
