@@ -1227,6 +1227,50 @@ int oracle_polygon_pixels_f(int nv, const int *vx, const int *vy, int cap, int *
   return polygon_pixels_t<float>(nv, vx, vy, cap, out_x, out_y);
 }
 
+int oracle_deformed_polygon_pixels(int nv, const int *vx, const int *vy, const double *params, int cx, int cy,
+                                   double skin_factor, int cap, int *out_x, int *out_y) {
+  // Polygon::get_owned_pixels(shape_function, cx, cy, skin_factor), DICe_Shape.cpp:136-219, in float
+  if (nv < 3 || !params) return -1;
+  Affine<float> sf;
+  sf.init(true, true, true, 1.0f, 0.1f);
+  for (int k = 0; k < 6; ++k) sf.p[k] = (float)params[k];
+  std::vector<int> X(nv + 1), Y(nv + 1);
+  for (int i = 0; i <= nv; ++i) {
+    const int x = vx[i % nv], y = vy[i % nv];
+    float mx = 0.0f, my = 0.0f;
+    sf.map((float)x, (float)y, (float)cx, (float)cy, mx, my);
+    int nx = (int)mx, ny = (int)my;
+    if (mx - (int)mx >= 0.5) nx++;
+    if (my - (int)my >= 0.5) ny++;
+    X[i] = nx;
+    Y[i] = ny;
+  }
+  int ccx = 0, ccy = 0;
+  for (int i = 0; i < nv; ++i) { ccx += X[i]; ccy += Y[i]; }
+  ccx = (int)(ccx / (double)nv);
+  ccy = (int)(ccy / (double)nv);
+  int mnx = 0, mxx = 0, mny = 0, mxy = 0;
+  for (int i = 0; i <= nv; ++i) {
+    X[i] = (int)((float)skin_factor * (X[i] - ccx) + ccx);
+    Y[i] = (int)((float)skin_factor * (Y[i] - ccy) + ccy);
+    if (i == 0) { mnx = mxx = X[i]; mny = mxy = Y[i]; }
+    mnx = std::min(mnx, X[i]); mxx = std::max(mxx, X[i]);
+    mny = std::min(mny, Y[i]); mxy = std::max(mxy, Y[i]);
+  }
+  int count = 0;
+  for (int y = mny; y <= mxy; ++y)
+    for (int x = mnx; x <= mxx; ++x) {
+      float angle = 0.0f;
+      for (int i = 0; i < nv; ++i)
+        angle += angle_2d<float>((float)(X[i] - x), (float)(Y[i] - y), (float)(X[i + 1] - x), (float)(Y[i + 1] - y));
+      if (std::abs(angle) >= kPi) {
+        if (count < cap) { out_x[count] = x; out_y[count] = y; }
+        ++count;
+      }
+    }
+  return count;
+}
+
 int oracle_correlate_f(const oracle_params *p, int w, int h, int storage, const void *ref, const void *def,
                        int n, int subset_size, const int *co, const int *cx, const int *cy,
                        const unsigned char *ca, double *fields, int first_frame, int threads) {

@@ -113,6 +113,12 @@ double oracle_subset_gamma_f(int w, int h, const float *ref, const float *def, i
  * default build); they differ on pixels that lie on an edge. */
 int oracle_polygon_pixels(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y);
 int oracle_polygon_pixels_f(int nv, const int *vx, const int *vy, int cap, int *out_x, int *out_y);
+/* Polygon::get_owned_pixels(shape_function, cx, cy, skin_factor) (DICe_Shape.cpp:136-219), the pixels
+ * a blocking subset's polygon covers at its current shape function: the vertices mapped by the affine
+ * parameters params[6] (u, v, theta, ex, ey, gxy) about (cx, cy) and rounded, stretched by
+ * skin_factor about their integer centroid, then the float winding rule. */
+int oracle_deformed_polygon_pixels(int nv, const int *vx, const int *vy, const double *params, int cx, int cy,
+                                   double skin_factor, int cap, int *out_x, int *out_y);
 
 /*
  * One Schema::execute_correlation (GENERIC_ROUTINE) over n subsets.

@@ -84,6 +84,7 @@ def lib():
         _lib.oracle_interp_d.argtypes = [i, i, i, P, i, P, P, P]
         _lib.oracle_polygon_pixels.argtypes = [i, P, P, i, P, P]
         _lib.oracle_polygon_pixels_f.argtypes = [i, P, P, i, P, P]
+        _lib.oracle_deformed_polygon_pixels.argtypes = [i, P, P, P, i, i, ctypes.c_double, i, P, P]
         _lib.oracle_lu_inverse.argtypes = [i, P]
         _lib.oracle_smooth_gradients_f.argtypes = [i, i, P, P]
         _lib.oracle_nelder_mead.argtypes = [i, i, P, P, P, i, ctypes.c_double, ctypes.c_double, i, P, P, P, P]
@@ -209,6 +210,20 @@ def polygon_pixels(vx, vy, precision="float"):
     oy = np.zeros(cap, np.int32)
     fn = lib().oracle_polygon_pixels_f if precision == "float" else lib().oracle_polygon_pixels
     cnt = fn(len(vx), _p(vx), _p(vy), cap, _p(ox), _p(oy))
+    return ox[:cnt], oy[:cnt]
+
+
+def deformed_polygon_pixels(vx, vy, params, cx, cy, skin_factor=1.0, cap=1 << 20):
+    """Polygon::get_owned_pixels(shape_function, cx, cy, skin_factor) of a blocking subset: returns (x, y)."""
+    vx = np.ascontiguousarray(vx, dtype=np.int32)
+    vy = np.ascontiguousarray(vy, dtype=np.int32)
+    q = np.ascontiguousarray(params, dtype=np.float64)
+    ox = np.zeros(cap, np.int32)
+    oy = np.zeros(cap, np.int32)
+    cnt = lib().oracle_deformed_polygon_pixels(len(vx), _p(vx), _p(vy), _p(q), int(cx), int(cy), skin_factor, cap,
+                                               _p(ox), _p(oy))
+    if cnt < 0 or cnt > cap:
+        raise ValueError("deformed_polygon_pixels: bad arguments")
     return ox[:cnt], oy[:cnt]
 
 

@@ -694,3 +694,77 @@ def test_sub_image_extents_deck(tmp_path, backend):
     rows = [ln.split() for ln in outs[0].splitlines() if ln[:1].isdigit()]
     for i, row in enumerate(rows):
         assert abs(float(row[3]) - f[o.F["DISP_X"], i]) < 5e-5 and abs(float(row[4]) - f[o.F["DISP_Y"], i]) < 5e-5
+
+
+BLOCK_A = ([28, 50, 51, 29], [30, 31, 55, 54])   # blocker polygon
+BLOCK_B = ([42, 64, 65, 43], [31, 30, 54, 55])   # overlaps A over x = 42..50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("blocker_first", [True, False], ids=["blocker_first", "blocked_first"])
+def test_tracking_blocking_subsets(tmp_path, blocker_first):
+    """TRACKING_ROUTINE with a conformal blocker (check_for_blocking_subsets, DICe_Schema.cpp:4529-4556)
+    over the integer-shift frames defSyntheticSpeckled1..3 (1 px a frame): the blocked subset loses the
+    warped pixels that round onto the blocker's polygon moved by the blocker's current shape function --
+    this frame's result when the blocker has the lower id, its previous frame's otherwise (one pixel
+    apart here).  Gauss-Newton for both (override_force_simplex), against the oracle run frame by frame
+    with the blocked pixels formed by its own deformed-polygon rule."""
+    d = np.load(os.path.join(GOLD, "synthetic_speckled.npz"))
+    ida, idb = (0, 1) if blocker_first else (1, 0)
+    polys = {ida: BLOCK_A, idb: BLOCK_B}
+    cen = {k: (int(np.mean(v[0])), int(np.mean(v[1]))) for k, v in polys.items()}
+    lines = ["BEGIN SUBSET_COORDINATES"] + ["%d %d" % cen[k] for k in (0, 1)] + ["END SUBSET_COORDINATES"]
+    for k in (0, 1):
+        vx, vy = polys[k]
+        lines += ["BEGIN CONFORMAL_SUBSET", "SUBSET_ID %d" % k, "BEGIN BOUNDARY", "BEGIN POLYGON", "BEGIN VERTICES"]
+        lines += ["%d %d" % xy for xy in zip(vx, vy)]
+        lines += ["END VERTICES", "END POLYGON", "END BOUNDARY"]
+        if k == idb:
+            lines += ["BEGIN BLOCKING_SUBSETS", "%d" % ida, "END BLOCKING_SUBSETS"]
+        lines += ["END CONFORMAL_SUBSET"]
+    (tmp_path / "subsets.txt").write_text("\n".join(lines) + "\n")
+    for k in range(4):
+        write_tiff_u8(tmp_path / ("def%d.tif" % k), d["def%d" % k])
+    frames = [str(tmp_path / ("def%d.tif" % k)) for k in (1, 2, 3)]
+    cfg = {"ref_file": tmp_path / "def0.tif", "def_file": frames[0], "def_files": ",".join(frames), "width": 100,
+           "height": 100, "subset_size": 27, "subset_file": tmp_path / "subsets.txt", "out_dir": tmp_path,
+           "prefix": "DICe_solution", "separate": 0, "backend": 0, "ref_name": "r", "def_name": "d",
+           "def_names": "d,d,d", "routine": 1, "override_force_simplex": 1, "optimization": 1, "frames": 3}
+    cfg_path = tmp_path / "deck.cfg"
+    cfg_path.write_text("".join("%s %s\n" % kv for kv in cfg.items()))
+    r = run("deck", cfg_path)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for ln in r.stdout.splitlines():
+        t = ln.split()
+        if t and t[0] == "subset":
+            got[int(t[1])] = (float(t[7]), float(t[9]), float(t[11]))
+    # the oracle, frame by frame: the blocker, then the blocked subset with the blocker's moved polygon
+    p = o.default_params()
+    p.optimization_method = 1
+    ref = d["def0"].astype(np.float32)
+    names = ("DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "SS_XY")
+    state = {k: o.new_fields(np.array([cen[k][0]], float), np.array([cen[k][1]], float)) for k in (0, 1)}
+
+    def solve(k, de, masks=None):
+        xs, ys = o.polygon_pixels(*polys[k])
+        conf = (np.array([0, len(xs)], np.int32), xs, ys, np.ones(len(xs), np.uint8))
+        f = state[k].copy()
+        o.correlate(p, ref, de, f, conformal=conf, precision="double", masks=masks, first_frame=False)
+        return f
+
+    moved = []
+    for k in (1, 2, 3):
+        de = d["def%d" % k].astype(np.float32)
+        before = [state[ida][o.F[n], 0] for n in names]
+        state[ida] = solve(ida, de)
+        after = [state[ida][o.F[n], 0] for n in names]
+        mx, my = o.deformed_polygon_pixels(*polys[ida], after if blocker_first else before, cen[ida][0], cen[ida][1],
+                                           1.0)
+        moved.append(int(np.min(mx)))
+        state[idb] = solve(idb, de, masks=(np.array([0, len(mx)], np.int32), mx, my))
+    assert moved[-1] == int(np.min(o.polygon_pixels(*BLOCK_A)[0])) + (3 if blocker_first else 2)
+    for k in (0, 1):
+        f = state[k]
+        assert got[k][2] == f[o.F["STATUS"], 0] == 1
+        assert abs(got[k][0] - f[o.F["DISP_X"], 0]) < 1e-4 and abs(got[k][1] - f[o.F["DISP_Y"], 0]) < 1e-4, (k, got[k])

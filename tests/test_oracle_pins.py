@@ -181,3 +181,32 @@ def test_uncertainty_fields_zero_shift():
     assert f2 == 1.0
     assert abs(f3) < 1e-6 and abs(f5) < 1e-6 and abs(f6) < 1e-6 and abs(f4) < 1e-6
     assert 0.0 <= f1 <= 1.0 + 1e-12
+
+
+# ---------------------------------------------------------------- blocking subsets
+
+POLY = ([30, 52, 61, 40, 27], [20, 18, 39, 47, 35])
+
+
+def test_deformed_polygon_identity_and_integer_translation():
+    """Polygon::get_owned_pixels(shape_function, ...) (DICe_Shape.cpp:136-219): zero parameters and skin 1
+    give the polygon's own pixels; an integer translation moves every vertex, and so every pixel, by it."""
+    px, py = o.polygon_pixels(*POLY)
+    dx, dy = o.deformed_polygon_pixels(*POLY, [0, 0, 0, 0, 0, 0], 44, 33, 1.0)
+    assert sorted(zip(dy, dx)) == sorted(zip(py, px))
+    tx, ty = o.deformed_polygon_pixels(*POLY, [3, -2, 0, 0, 0, 0], 44, 33, 1.0)
+    assert sorted(zip(ty, tx)) == sorted(zip(py - 2, px + 3))
+
+
+def test_deformed_polygon_skin_and_rotation():
+    """The skin factor stretches the vertices about their integer centroid: skin 2 is the polygon whose
+    vertices are 2 v - c (c = (int)(mean)); a quarter turn about a vertex maps (x, y) to
+    (cx - (y - cy), cy + (x - cx)), the vertices land on integers."""
+    vx, vy = np.array(POLY[0]), np.array(POLY[1])
+    cx, cy = int(vx.mean()), int(vy.mean())
+    sx, sy = o.deformed_polygon_pixels(vx, vy, [0, 0, 0, 0, 0, 0], 44, 33, 2.0)
+    ex, ey = o.polygon_pixels(2 * vx - cx, 2 * vy - cy)
+    assert sorted(zip(sy, sx)) == sorted(zip(ey, ex))
+    rx, ry = o.deformed_polygon_pixels(vx, vy, [0, 0, np.pi / 2, 0, 0, 0], 30, 20, 1.0)
+    qx, qy = o.polygon_pixels(30 - (vy - 20), 20 + (vx - 30))
+    assert sorted(zip(ry, rx)) == sorted(zip(qy, qx))
