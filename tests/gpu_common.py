@@ -36,8 +36,13 @@ def run_oracle(p, ref, de, cx, cy, subset_size, fields=None, conformal=None, fir
     return f
 
 
-def compare(fg, fo, u_tol=U_TOL, gamma_rtol=GAMMA_RTOL, status_agree=STATUS_AGREE, label=""):
-    """Status agreement, then shape parameters and gamma on the subsets both runs converged."""
+def compare(fg, fo, u_tol=U_TOL, gamma_rtol=GAMMA_RTOL, status_agree=STATUS_AGREE, label="", ff=None):
+    """Status agreement, then shape parameters and gamma on the subsets both runs converged.
+    ff (optional): the oracle's float build on the same inputs.  The reference's own float build
+    (scalar_t = float, its default) computes gamma from float intensities and interpolants and differs
+    from its double build by up to several 1e-5 relative on some subsets (6.4e-5 on the 5,000 polygon
+    subsets of tests/test_gpu_configs.py); with ff, gamma is within gamma_rtol on >= 99.9 % of the
+    converged subsets and on all of them within the float build's own largest deviation."""
     sg, so = fg[dh.F["STATUS"]], fo[o.F["STATUS"]]
     agree = float(np.mean(sg == so))
     if agree < status_agree:
@@ -50,7 +55,15 @@ def compare(fg, fo, u_tol=U_TOL, gamma_rtol=GAMMA_RTOL, status_agree=STATUS_AGRE
         assert d.size == 0 or d.max() <= u_tol, "%s %s max diff %.3g" % (label, name, d.max())
     g, go = fg[dh.F["GAMMA"]][both], fo[o.F["GAMMA"]][both]
     rel = np.abs(g - go) / np.maximum(np.abs(go), 1e-30)
-    assert rel.size == 0 or rel.max() <= gamma_rtol, "%s gamma max rel diff %.3g" % (label, rel.max())
+    if ff is None or rel.size == 0:
+        assert rel.size == 0 or rel.max() <= gamma_rtol, "%s gamma max rel diff %.3g" % (label, rel.max())
+    else:
+        bf = both & np.isin(ff[o.F["STATUS"]], dh.CONVERGED_STATUSES)
+        rf = np.abs(ff[o.F["GAMMA"]] - fo[o.F["GAMMA"]])[bf] / np.abs(fo[o.F["GAMMA"]][bf])
+        assert np.mean(rel <= gamma_rtol) >= STATUS_AGREE, "%s gamma within %.0e on %.4f" % (
+            label, gamma_rtol, np.mean(rel <= gamma_rtol))
+        assert rel.max() <= max(gamma_rtol, rf.max()), "%s gamma max rel diff %.3g, float build %.3g" % (
+            label, rel.max(), rf.max())
     return agree, both
 
 

@@ -13,7 +13,9 @@
   differences of the forward over all subsets (dL/dp) and at sampled pixels (dL/dI_def).
 
 Bar: status identical on >= 99.9 % of subsets, u / v / theta / strains within 1e-4, gamma within 1e-5
-relative (tests/gpu_common.py); adjoint within 1e-3 relative of the central differences."""
+relative (on the polygon subsets: on 99.9 % of them, and on all within the reference's own float-build
+deviation from its double build, tests/gpu_common.py); adjoint within 1e-3 relative of the central
+differences."""
 import numpy as np
 import pytest
 
@@ -128,7 +130,9 @@ def test_c3_uneven_conformal_against_oracle(c3_scene, mode):
                 # the largest polygons do not fit the default 2,560-float slot: the PLAIN kernel solves them
                 assert ov0 > 0, "no subset overflowed the LDS slot"
             fo = run_oracle(p, ref, de, cx, cy, 0, fields=inits[k], conformal=conf, first_frame=False)
-            agree, both = compare(fg, fo, label="conformal frame %d" % (k + 1))
+            ff = run_oracle(p, ref, de, cx, cy, 0, fields=inits[k], conformal=conf, first_frame=False,
+                            precision="float")
+            agree, both = compare(fg, fo, label="conformal frame %d" % (k + 1), ff=ff)
             assert both.mean() > 0.99
             it_g, it_o = fg[dh.F["ITERATIONS"]][both], fo[o.F["ITERATIONS"]][both]
             assert np.mean(it_g == it_o) >= 0.99
