@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -132,6 +133,11 @@ struct dice_hip_context {
   int rox = 0, roy = 0, dox = 0, doy = 0;
   // fields + work
   DevBuf fields, work;
+  // lean kernel: per-subset constants of the reference (kLeanCacheDoubles each),
+  // current while their tag equals lean_gen; every change of parameters,
+  // images, subsets or offsets starts a new generation (drop_graph)
+  DevBuf lean_cache;
+  int lean_gen = 1;
   int *scratch_work = nullptr;  // this device's copy of the static work area (no-allocation entry point)
   int *last_work = nullptr;     // work area of the last correlation launch
   // launch shape of the batched mode
@@ -302,6 +308,9 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
   KParams kp = make_kparams(ctx->params, first_frame);
   kp.guess = guess;
   kp.has_forced = ctx->has_force ? 1 : 0;
+  // the cache holds one entry per subset of the current set (square subsets only)
+  kp.lean_cache = ctx->S > 0 && ctx->lean_cache.p ? ctx->lean_cache.as<double>() : nullptr;
+  kp.lean_gen = ctx->lean_gen;
   const Launch L = current_launch(ctx, mode);
   // per-launch timing events on the context stream (eager launches only; a captured
   // graph is timed as a whole by the caller)
@@ -325,6 +334,8 @@ void drop_graph(dice_hip_context *ctx) {
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   ctx->graph_exec = nullptr;
   ctx->graph_key = GraphKey{};
+  // the configuration changes: the cached per-subset constants are stale
+  ctx->lean_gen = ctx->lean_gen >= (1 << 30) ? 1 : ctx->lean_gen + 1;
 }
 
 int upload_image(dice_hip_context *ctx, DevBuf &buf, int w, int h, int storage, const void *host) {
@@ -395,7 +406,7 @@ int dice_hip_destroy(dice_hip_context *ctx) {
                     &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2,
                     &ctx->vsg_rec, &ctx->vsg_temp, &ctx->vsg_bounds,
                     &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work, &ctx->force, &ctx->mbox, &ctx->moff,
-                    &ctx->mbits})
+                    &ctx->mbits, &ctx->lean_cache})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -626,6 +637,14 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
   ctx->last_work = nullptr;  // the work area may move
   HIP_TRY(ctx, ctx->work.ensure(dice_hip::work_ints(n) * sizeof(int)), "hipMalloc work");
   HIP_TRY(ctx, ctx->fields.ensure((size_t)DICE_F_COUNT * n * sizeof(double)), "hipMalloc fields");
+  // square subsets: the lean kernel's cache, zeroed (no entry carries a generation >= 1)
+  const size_t cache_ints = (size_t)n * dice_hip::kLeanCacheDoubles * 2;
+  if (ctx->S > 0 && cache_ints <= (size_t)INT_MAX) {
+    HIP_TRY(ctx, ctx->lean_cache.ensure(cache_ints * sizeof(int)), "hipMalloc lean cache");
+    HIP_TRY(ctx, dice_hip::launch_zero_ints(ctx->lean_cache.as<int>(), (int)cache_ints, ctx->stream), "zero lean cache");
+  } else {
+    ctx->lean_cache.release();
+  }
   return DICE_HIP_OK;
 }
 
@@ -823,9 +842,12 @@ int dice_hip_pass_counts(dice_hip_context *ctx, int *counts) {
   return DICE_HIP_OK;
 }
 
-// VSG strains: the subsets binned on the host into a uniform grid of cell side
-// max(r, 1) px (sorted by cell, then id), uploaded with the offsets, then one
-// thread per subset on the context stream.
+// VSG strains: the subsets binned on the device into a uniform grid (cells of
+// side max(r, 1) px for radius windows, about k subsets each for k-nearest):
+// the bounds of the centres (read back, 40 B, to size the grid), a cell key per
+// subset, a stable radix sort of (cell, id), each cell's range and one packed
+// record per subset, then one thread per subset on the context stream.  The
+// bounds read-back and the final wait make this call synchronous.
 static int vsg_enqueue(dice_hip_context *ctx, int window_size, double *d_fields, double *d_vsg) {
   if (window_size == 0) return ctx->fail(DICE_HIP_EINVAL, "strain_window_size_in_pixels must not be 0");
   if (window_size < -dice_hip::kVsgMaxK)

@@ -160,16 +160,28 @@ struct BlockGroup {
   __device__ __forceinline__ int imax(int v) const { return ired(v, false); }
 };
 
+// float element i (i < 2^30) of a global array through a 32-bit byte offset
+typedef __attribute__((address_space(1))) const char gchar;
+__device__ __forceinline__ float ldf_u(const void *p, unsigned i) {
+  return *(const gfloat *)((const gchar *)p + (i << 2));
+}
+
 // Pixel storage of the images (storage_t, DICe.h:105-112): float or uint16.
 template <typename T> struct Pix;
+// at_u: element i (i < 2^30) through a 32-bit byte offset, which the load
+// takes as an unsigned VGPR offset from the scalar base (no 64-bit address math).
 template <> struct Pix<float> {
   typedef gfloat G;
   __device__ __forceinline__ static float at(const void *p, int i) { return ((const gfloat *)p)[i]; }
+  __device__ __forceinline__ static float at_u(const void *p, unsigned i) { return ldf_u(p, i); }
   __device__ __forceinline__ static const gfloat *ptr(const void *p) { return (const gfloat *)p; }
 };
 template <> struct Pix<uint16_t> {
   typedef gu16 G;
   __device__ __forceinline__ static float at(const void *p, int i) { return (float)((const gu16 *)p)[i]; }
+  __device__ __forceinline__ static float at_u(const void *p, unsigned i) {
+    return (float)*(const gu16 *)((const gchar *)p + (i << 1));
+  }
   __device__ __forceinline__ static const gu16 *ptr(const void *p) { return (const gu16 *)p; }
 };
 // Kept for the image-level kernels.
@@ -180,13 +192,14 @@ template <typename T> __device__ __forceinline__ float ldpix(const void *p, int 
 // in the fractional offset t in [0, 1).  Same function as the reference's
 // f2(t+2), f1(t+1), f0(t), f0(1-t), f1(2-t), f2(3-t).
 __device__ __forceinline__ void keys_weights(float t, float w[6]) {
-  const float t2 = t * t, t3 = t2 * t;
-  w[0] = (t3 - 2.0f * t2 + t) * (1.0f / 12.0f);
-  w[1] = (-7.0f / 12.0f) * t3 + 1.25f * t2 - (2.0f / 3.0f) * t;
-  w[2] = (4.0f / 3.0f) * t3 - (7.0f / 3.0f) * t2 + 1.0f;
-  w[3] = (-4.0f / 3.0f) * t3 + (5.0f / 3.0f) * t2 + (2.0f / 3.0f) * t;
-  w[4] = (7.0f / 12.0f) * t3 - 0.5f * t2 - (1.0f / 12.0f) * t;
-  w[5] = (t2 - t3) * (1.0f / 12.0f);
+  // Horner forms: 17 operations for the six weights
+  const float t2 = t * t;
+  w[0] = fmaf(fmaf(t, 1.0f / 12.0f, -1.0f / 6.0f), t, 1.0f / 12.0f) * t;
+  w[1] = fmaf(fmaf(t, -7.0f / 12.0f, 1.25f), t, -2.0f / 3.0f) * t;
+  w[2] = fmaf(fmaf(t, 4.0f / 3.0f, -7.0f / 3.0f), t2, 1.0f);
+  w[3] = fmaf(fmaf(t, -4.0f / 3.0f, 5.0f / 3.0f), t, 2.0f / 3.0f) * t;
+  w[4] = fmaf(fmaf(t, 7.0f / 12.0f, -0.5f), t, -1.0f / 12.0f) * t;
+  w[5] = fmaf(t, -1.0f / 12.0f, 1.0f / 12.0f) * t2;
 }
 
 // d/dt of keys_weights (the slopes of the six cubics).
@@ -422,6 +435,67 @@ __device__ __forceinline__ void wave_lu_update6(double (&hrow)[6], double qv, do
   }
 }
 
+// The same update du = -H^-1 q by H = L D L^T without pivoting, for the
+// symmetric positive definite Gauss-Newton matrix: 6 divisions and about a
+// hundred FP64 multiply-adds, every lane computing the same values (no
+// cross-lane traffic).  It agrees with the pivoted LU above to rounding in FP64
+// (the update is then rounded to FP32).  Returns false, leaving du unset, when
+// a pivot is not safely positive (a singular or indefinite system, or NaN):
+// the caller then takes the pivoted LU, which reproduces the reference there.
+// H = L D L^T of a symmetric 6 x 6 (lower triangle read): L strictly lower,
+// inv = 1 / diag(D).  False when a pivot is not safely positive (or NaN).
+__device__ __forceinline__ bool ldlt_factor6(const double (&H)[6][6], double (&L)[6][6], double (&inv)[6]) {
+  double d[6];
+  double hmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) hmax = fmax(hmax, H[i][i]);
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double v[6];
+    double dj = H[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) {
+      v[k] = L[j][k] * d[k];
+      dj = fma(-L[j][k], v[k], dj);
+    }
+    d[j] = dj;
+    ok = ok && dj > 1.0e-13 * hmax;  // false for NaN too
+    inv[j] = 1.0 / dj;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double s = H[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s = fma(-L[i][k], v[k], s);
+      L[i][j] = s * inv[j];
+    }
+  }
+  return ok;
+}
+__device__ __forceinline__ bool ldlt_update6(const double (&H)[6][6], const double (&q)[6], double (&du)[6]) {
+  double L[6][6], inv[6];
+  if (!ldlt_factor6(H, L, inv)) return false;
+  // L y = -q, then z = D^-1 y, then L^T du = z
+  double y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = -q[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s = fma(-L[i][k], y[k], s);
+    y[i] = s;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) y[i] *= inv[i];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) s = fma(-L[k][i], du[k], s);
+    du[i] = s;
+  }
+  return true;
+}
+
 // 2x2 condition estimate of the (u, v) block in scalar_t, as the reference
 // stores it (DICe_Objective.cpp:558-564); -1 when the determinant is 0.
 __device__ __forceinline__ float cond2x2(double h00, double h01, double h10, double h11) {
@@ -458,6 +532,7 @@ struct Solver {
   lds_double *pers;
   bool last_fast;
   int wx0, wy0, wpitch, wrows;
+  bool win_ok;  // the LDS slot holds the deformed image over the window above
   bool overflow;
   // subset
   int cx, cy, n, S, hw;
@@ -478,7 +553,7 @@ struct Solver {
 
   __device__ __forceinline__ Solver(const KParams &k, const DImage &r, const DImage &d, G g, lds_float *w, int cap)
       : kp(k), ref(r), def(d), grp(g), win(w), win_cap(kFast ? cap - kPersistFloats : cap), wx0(0), wy0(0),
-        wpitch(0), wrows(0), overflow(false) {
+        wpitch(0), wrows(0), win_ok(false), overflow(false) {
     pers = kFast ? (lds_double *)(w + win_cap) : nullptr;
     last_fast = false;
     en[0] = en[1] = 1.0f;
@@ -492,6 +567,7 @@ struct Solver {
     cy = ss.cy[i];
     S = ss.S;
     overflow = false;
+    win_ok = false;
     if (S > 0) {
       hw = S / 2;
       S = 2 * hw + 1;  // square subsets are odd (DICe_Subset.cpp:90-92)
@@ -542,19 +618,28 @@ struct Solver {
   // coordinates clamp to 0 (DICe_Subset.cpp:65-68, 100-101).  Square subsets
   // walk (row, col) incrementally in a loop of their own.
   template <class F>
+  __device__ __forceinline__ void for_square(F &&f) const {
+    const int r0 = grp.rank();
+    // (k + 0.5) / S in float has the integer part of k / S for k < 2^8 and S < 2^12
+    const float inv = 1.0f / (float)S;
+    const int drow = S < 4096 ? (int)(((float)G::kSize + 0.5f) * inv) : G::kSize / S;
+    const int dcol = G::kSize - drow * S;
+    int row = S < 4096 ? (int)(((float)r0 + 0.5f) * inv) : r0 / S;
+    int col = r0 - row * S;
+    const int x0 = cx - hw, y0 = cy - hw;
+#pragma unroll 1
+    for (int i = r0; i < n; i += G::kSize) {
+      f(max(x0 + col, 0), max(y0 + row, 0), true);
+      col += dcol;
+      row += drow;
+      if (col >= S) { col -= S; ++row; }
+    }
+  }
+  template <class F>
   __device__ __forceinline__ void for_pixels(F &&f) const {
     const int r0 = grp.rank();
     if (S > 0) {
-      const int drow = G::kSize / S, dcol = G::kSize - drow * S;
-      int row = r0 / S, col = r0 - row * S;
-      const int x0 = cx - hw, y0 = cy - hw;
-#pragma unroll 1
-      for (int i = r0; i < n; i += G::kSize) {
-        f(max(x0 + col, 0), max(y0 + row, 0), true);
-        col += dcol;
-        row += drow;
-        if (col >= S) { col -= S; ++row; }
-      }
+      for_square(f);
     } else if (pk) {
       // one packed word per pixel; the next one is loaded before this pixel's work
       uint32_t next = r0 < n ? pk[r0] : 0u;
@@ -580,6 +665,15 @@ struct Solver {
     else
       for_pixels([&](int x, int y, bool a) { f(std::false_type{}, x, y, a); });
   }
+  // LEAN: a square subset inside the reference image (the lean kernel's case)
+  // walked by one loop; otherwise for_pixels_ref
+  template <bool LEAN, class F>
+  __device__ __forceinline__ void walk(F &&f) const {
+    if constexpr (LEAN)
+      for_square([&](int x, int y, bool a) { f(std::true_type{}, x, y, a); });
+    else
+      for_pixels_ref(f);
+  }
 
   // Reference intensity and gradients at local (rx, ry); outside the image the
   // intensity is pixel 0 (Image_::operator(), DICe_Image.h:186-192) and the
@@ -587,10 +681,10 @@ struct Solver {
   template <bool INSIDE = false>
   __device__ __forceinline__ void ref_pixel(int rx, int ry, float &F, float &gx, float &gy) const {
     if (INSIDE || ref_in || (rx >= 0 && ry >= 0 && rx < ref.w && ry < ref.h)) {
-      const int o = ry * ref.w + rx;
-      F = Pix<T>::at(ref.pix, o);
-      gx = ((const gfloat *)ref.gx)[o];
-      gy = ((const gfloat *)ref.gy)[o];
+      const unsigned o = (unsigned)(ry * ref.w + rx);
+      F = Pix<T>::at_u(ref.pix, o);
+      gx = ldf_u(ref.gx, o);
+      gy = ldf_u(ref.gy, o);
     } else {
       F = Pix<T>::at(ref.pix, 0);
       gx = gy = 0.0f;
@@ -599,7 +693,7 @@ struct Solver {
   template <bool INSIDE = false>
   __device__ __forceinline__ float ref_value(int rx, int ry) const {
     if (INSIDE || ref_in || (rx >= 0 && ry >= 0 && rx < ref.w && ry < ref.h))
-      return Pix<T>::at(ref.pix, ry * ref.w + rx);
+      return Pix<T>::at_u(ref.pix, (unsigned)(ry * ref.w + rx));
     return Pix<T>::at(ref.pix, 0);
   }
 
@@ -684,8 +778,13 @@ struct Solver {
     const int ww = x1 - x0 + 1, hh = y1 - y0 + 1;
     if (ww <= 0 || hh <= 0) {
       wx0 = x0; wy0 = y0; wpitch = 1; wrows = 0;
+      win_ok = false;
       return true;  // nothing is kept
     }
+    // the deformed image does not change within a subset: a window already
+    // staged that contains this one serves it (the taps are read at their
+    // image coordinates through wx0, wy0, wpitch)
+    if (win_ok && x0 >= wx0 && x1 < wx0 + wpitch && y0 >= wy0 && y1 < wy0 + wrows) return true;
     if (ww * hh > win_cap) {
       if (hand_over) overflow = true;
       return false;
@@ -724,6 +823,7 @@ struct Solver {
       }
     }
     grp.sync();
+    win_ok = true;
     return true;
   }
 
@@ -782,6 +882,7 @@ struct Solver {
   __device__ __forceinline__ const lds_double *lds_transpose_sum(const float (&v)[kGnSums]) {
     const int lane = grp.rank();
     lds_float *red = win;
+    win_ok = false;
     grp.sync();
 #pragma unroll
     for (int k = 0; k < kGnSums; ++k) red[k * kGnRedStride + lane] = v[k];
@@ -921,6 +1022,7 @@ struct Solver {
   __device__ __forceinline__ void lds_chunk_sum(const float (&v)[K], D &&dest) {
     const int lane = grp.rank();
     lds_float *red = win;
+    win_ok = false;
     grp.sync();
 #pragma unroll
     for (int k = 0; k < K; ++k) red[k * kGnRedStride + lane] = v[k];
@@ -945,14 +1047,15 @@ struct Solver {
   // phi^T) (a full symmetric 6 x 6) and sum(phi) into pers[], and the mean of
   // the reference intensities (computeUpdateFast's F-bar, DICe_Objective.cpp:464)
   // into meanF, from the same pass.
+  template <bool LEAN = false>
   __device__ __forceinline__ void precompute_phi() {
     float pa[29];
 #pragma unroll
     for (int k = 0; k < 29; ++k) pa[k] = 0.0f;
-    for_pixels([&](int x, int y, bool a) {
+    walk<LEAN>([&](auto inside, int x, int y, bool a) {
       if (!a) return;
       float F, gx, gy;
-      ref_pixel(x - ref.ox, y - ref.oy, F, gx, gy);
+      ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, F, gx, gy);
       pa[27] += F;
       pa[28] += 1.0f;
       const float dx = (float)(x - cx), dy = (float)(y - cy);
@@ -983,19 +1086,23 @@ struct Solver {
     meanF = cnt > 0.0 ? (float)(uniform(P[42]) / cnt) : 0.0f;
   }
 
-  // Fast pass: sum(gmf phi) (6) and sum(G) over the active pixels, all kept.
-  __device__ __forceinline__ const lds_double *accumulate_gn_fast(const Affine &A, float shift) {
+  // Fast pass: sum(gmf phi) (6) and sum(G) over the active pixels, all kept,
+  // into pers[kPersistPass, +7).  The lane partials are summed in FP64 across
+  // the wave without LDS, so the staged window stays valid for the next pass.
+  template <bool LEAN = false>
+  __device__ __forceinline__ void accumulate_gn_fast(const Affine &A, float shift, int out = kPersistPass) {
     float pa[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) pa[k] = 0.0f;
-    for_pixels_ref([&](auto inside, int x, int y, bool a) {
+    walk<LEAN>([&](auto inside, int x, int y, bool a) {
       if (!a) return;
+      // the reference loads first: they are in flight during the interpolation
+      float Fv, gx, gy;
+      ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
       const float Gv = sample(ox, oy);
-      float Fv, gx, gy;
-      ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
       const float gmf = (Gv - shift) - (Fv - meanF);
       const float g1 = gmf * gx, g2 = gmf * gy;
       pa[0] += g1;
@@ -1006,9 +1113,15 @@ struct Solver {
       pa[5] = fmaf(g2, dy, pa[5]);
       pa[6] += Gv;
     });
-    lds_double *tot = (lds_double *)(win + kGnTotOffset);
-    lds_chunk_sum<7>(pa, [&](int r, double t) { tot[r] = t; });
-    return tot;
+    double fs[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) fs[k] = (double)pa[k];
+    grp.sum(fs);
+    if (grp.rank() == 0) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) pers[out + k] = fs[k];
+    }
+    grp.sync();
   }
 
   // Objective::gamma (DICe_Objective.cpp:58-76) at parameters A: warp, then
@@ -1018,23 +1131,49 @@ struct Solver {
     contrast = 0.0f;
     if (!stage_affine(A)) return -1.0f;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
-    for_pixels([&](int x, int y, bool a) {
+    bool every = false;  // BATCHED: every active pixel is kept (no per-pixel guard test)
+    if constexpr (kIsWave) every = all_kept(A);
+    if (every) {
+      gamma_sums_every<false>(A, acc);
+    } else {
+      for_pixels([&](int x, int y, bool a) {
+        if (!a) return;
+        const double r = ref_value(x - ref.ox, y - ref.oy);
+        const float dx = (float)(x - cx), dy = (float)(y - cy);
+        float ox, oy;
+        A.disp(dx, dy, ox, oy);
+        if (!kept(ox, oy)) return;
+        const double d = sample(ox, oy);
+        acc[0] += 1.0;
+        acc[1] += d;
+        acc[2] += d * d;
+        acc[3] += r;
+        acc[4] += r * r;
+        acc[5] += d * r;
+      });
+    }
+    grp.sum(acc);
+    if (every) acc[0] = (double)n_def_active;
+    return gamma_of(acc, active, contrast);
+  }
+  // The gamma sums (sd, sdd, sr, srr, sdr) when every active pixel is kept
+  // (acc[0], the count, is left to the caller: the defined-active count).
+  template <bool LEAN>
+  __device__ __forceinline__ void gamma_sums_every(const Affine &A, double (&acc)[6]) {
+    walk<LEAN>([&](auto inside, int x, int y, bool a) {
       if (!a) return;
+      // the reference load first: it is in flight during the interpolation
+      const double r = ref_value<decltype(inside)::value>(x - ref.ox, y - ref.oy);
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
-      if (!kept(ox, oy)) return;
       const double d = sample(ox, oy);
-      const double r = ref_value(x - ref.ox, y - ref.oy);
-      acc[0] += 1.0;
       acc[1] += d;
       acc[2] += d * d;
       acc[3] += r;
       acc[4] += r * r;
       acc[5] += d * r;
     });
-    grp.sum(acc);
-    return gamma_of(acc, active, contrast);
   }
   // ZNSSD gamma from the FP64 sums (n, sd, sdd, sr, srr, sdr) of the kept pixels.
   __device__ __forceinline__ float gamma_of(const double (&acc)[6], int &active, float &contrast) const {
@@ -1303,6 +1442,234 @@ struct Solver {
 
   // Objective_ZNSSD::computeUpdateFast.  Returns the solver status (or
   // kOverflow); p_state receives the parameters of the last warp (for sigma).
+  // Fast-pass system (every lane the same values): with reference gradients the
+  // residual row is r = M phi, so H = M S M^T with the per-subset
+  // S = sum(phi phi^T) (pers[0, 36)), and q = M (sum(gmf phi) - corr sum(phi))
+  // from the pass sums at pers[kPersistPass, +6) and sum(phi) at pers[36, 42).
+  // Rows of M over phi = (gx, gy, dx gx, dy gx, dx gy, dy gy): u (c, -s, 0, 0, 0, 0),
+  // v (s, c, 0, 0, 0, 0), theta (0, 0, -g, -ey1, ex1, g), ex (0, 0, 1, 0, 0, 0),
+  // ey (0, 0, 0, 0, 0, 1), gxy (0, 0, 0, 1, 1, 0).
+  __device__ __forceinline__ void fast_system(const Affine &A, double corr, double (&H)[6][6], double (&q)[6]) const {
+    const double c = A.c, sn = A.s, ex1 = 1.0 + (double)A.p[3], ey1 = 1.0 + (double)A.p[4], g = A.p[5];
+    // (M t)_r for a vector t over phi
+    auto apply = [&](int r, const double (&t)[6]) -> double {
+      switch (r) {
+        case 0: return c * t[0] - sn * t[1];
+        case 1: return sn * t[0] + c * t[1];
+        case 2: return ex1 * t[4] - ey1 * t[3] + g * (t[5] - t[2]);
+        case 3: return t[2];
+        case 4: return t[5];
+        default: return t[3] + t[4];
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double t[6];  // S m_j (S is symmetric: row k of S against row j of M)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        double srow[6];
+#pragma unroll
+        for (int l = 0; l < 6; ++l) srow[l] = pers[6 * k + l];
+        t[k] = apply(j, srow);
+      }
+#pragma unroll
+      for (int i = 0; i <= j; ++i) {
+        H[i][j] = apply(i, t);
+        H[j][i] = H[i][j];
+      }
+    }
+    double v[6];
+#pragma unroll
+    for (int l = 0; l < 6; ++l) v[l] = pers[kPersistPass + l] - corr * pers[36 + l];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) q[i] = apply(i, v);
+  }
+  // Identity rows and columns and a zero gradient for disabled parameters (the
+  // same solution as the reference's reduced N x N system), and the LM term.
+  __device__ __forceinline__ void finish_system(double (&H)[6][6], double (&q)[6]) const {
+#pragma unroll
+    for (int k = 2; k < 6; ++k)
+      if (en[k] == 0.0f) {
+#pragma unroll
+        for (int l = 0; l < 6; ++l) { H[k][l] = 0.0; H[l][k] = 0.0; }
+        H[k][k] = 1.0;
+        q[k] = 0.0;
+      }
+    const double lm = kp.lm_alpha > 0.0f ? (double)kp.lm_alpha : 0.0;
+    H[0][0] += lm;
+    H[1][1] += lm;
+  }
+
+  // Per-subset constants of the lean kernel in pers (kLean* layout): the
+  // L D L^T factors of S = sum(phi phi^T), sum(phi), S00, S01, S11 and the mean
+  // of the reference intensities.  They depend on the reference image and the
+  // subset only, so they come from the per-subset cache entry when its tag is
+  // the current generation; otherwise they are computed and the entry written.
+  // Returns false when S is not safely positive definite.
+  __device__ __forceinline__ bool lean_setup(int gid) {
+    const int lane = grp.rank();
+    double *entry = kp.lean_cache ? kp.lean_cache + (size_t)gid * kLeanCacheDoubles : nullptr;
+    const double gen = (double)kp.lean_gen;
+    if (entry) {
+      const double e = lane < kLeanCacheDoubles ? entry[lane] : 0.0;
+      const double tag = lane_value(e, kLeanTag);
+      if (tag == gen || tag == -gen) {
+        if (lane < kLeanCacheDoubles) pers[lane] = e;
+        grp.sync();
+        meanF = (float)uniform(pers[kLeanMeanF]);
+        return tag > 0.0;
+      }
+    }
+    precompute_phi<true>();  // S, sum(phi) and meanF
+    double Sm[6][6], sphi[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+      for (int j = i; j < 6; ++j) {
+        Sm[i][j] = pers[6 * i + j];
+        Sm[j][i] = Sm[i][j];
+      }
+      sphi[i] = pers[36 + i];
+    }
+    grp.sync();
+    double L[6][6], dinv[6];
+    const bool ok = __builtin_amdgcn_readfirstlane((int)ldlt_factor6(Sm, L, dinv)) != 0;
+    if (lane == 0) {
+      int h = kLeanL;
+#pragma unroll
+      for (int i = 1; i < 6; ++i)
+#pragma unroll
+        for (int k = 0; k < i; ++k) pers[h++] = L[i][k];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        pers[kLeanDinv + i] = dinv[i];
+        pers[kLeanSphi + i] = sphi[i];
+      }
+      pers[kLeanS00] = Sm[0][0];
+      pers[kLeanS01] = Sm[0][1];
+      pers[kLeanS11] = Sm[1][1];
+      pers[kLeanMeanF] = (double)meanF;
+      pers[kLeanTag] = ok ? gen : -gen;
+    }
+    grp.sync();
+    if (entry && lane < kLeanCacheDoubles) entry[lane] = pers[lane];
+    return ok;
+  }
+
+  // computeUncertaintyFields in the lean kernel: the square subset inside both
+  // images with every sample x + u, y + v away from the border, so the Keys
+  // weights are those of the fractional parts of (u, v) for every pixel and the
+  // unshifted deformed pixel is in the window (otherwise false: hand over).
+  __device__ __forceinline__ bool lean_uncertainty(const Affine &A, double *fields, int nsub, int gid) {
+    const float u = A.p[0], v = A.p[1];
+    const int fu = (int)floorf(u), fv = (int)floorf(v);
+    const int lx0 = bx0 - ref.ox, lx1 = bx1 - ref.ox, ly0 = by0 - ref.oy, ly1 = by1 - ref.oy;
+    const float m = 1e-3f;
+    const bool uniform_w = (float)lx0 + u > 2.5f + m && (float)lx1 + u < (float)def.w - 3.5f - m &&
+                           (float)ly0 + v > 2.5f + m && (float)ly1 + v < (float)def.h - 3.5f - m;
+    if (!uniform_w || lx1 > def.w - 2 || ly1 > def.h - 2) return false;
+    if (!stage(lx0 + fu - 3, lx1 + fu + 4, ly0 + fv - 3, ly1 + fv + 4)) return false;
+    if (!(wx0 <= lx0 && wx0 + wpitch > lx1 && wy0 <= ly0 && wy0 + wrows > ly1)) return false;
+    float wxu[6], wyu[6];
+    keys_weights(u - (float)fu, wxu);
+    keys_weights(v - (float)fv, wyu);
+    float pt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for_square([&](int x, int y, bool) {
+      const int rx = x - ref.ox, ry = y - ref.oy;
+      float refv, gx, gy;
+      ref_pixel<true>(rx, ry, refv, gx, gy);
+      const float sub_r =
+          keys_sum(win + (__umul24(ry + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0)), wpitch, wxu, wyu) - refv;
+      const float e = win[__umul24(ry - wy0, wpitch) + (rx - wx0)];
+      unc_terms(u, v, gx, gy, sub_r, e - refv, pt);
+    });
+    unc_record(pt, u, v, fields, nsub, gid);
+    return true;
+  }
+
+  // Gauss-Newton of the lean kernel (all six parameters, no LM term or
+  // momentum: the host sends other runs to the general kernel).
+  // Every step takes the fast pass, whose system is H = M S M^T, q = M v with
+  // v = sum(gmf phi) - (meanG - shift) sum(phi) (see fast_system).  M is
+  // invertible, so H du = -q is M^T du = -S^-1 v: w = S^-1 v from the cached
+  // factors of S, then du from the closed form of M^T (a rotation block for
+  // (u, v); for theta, ex, ey, gxy the rows phi3 - phi4 give theta, the others
+  // one parameter each).  kOverflow hands the subset to the general kernel: a
+  // window that does not fit, or a warped pixel that is not kept.
+  __device__ __forceinline__ int gauss_newton_lean(Affine &A, int &iters, float p_state[6], double *fields,
+                                                   int nsub, int gid) {
+    const int max_its = kp.max_fast;
+    float shift = meanF;  // running estimate of meanG (one-pass ZSSD gradient)
+    int it = 0;
+#pragma unroll 1
+    for (; it <= max_its; ++it) {
+      iters = it;
+      A.prepare();
+#pragma unroll
+      for (int k = 0; k < 6; ++k) p_state[k] = A.p[k];
+      if (!stage_affine(A) || !all_kept(A)) return kOverflow;
+      accumulate_gn_fast<true>(A, shift, kLeanPass);
+      const double cnt = (double)n_def_active;
+      const float meanG = cnt > 0.0 ? (float)(uniform(pers[kLeanPass + 6]) / cnt) : 0.0f;
+      const double corr = (double)meanG - (double)shift;
+      const double c = A.c, sn = A.s, ex1 = 1.0 + (double)A.p[3], ey1 = 1.0 + (double)A.p[4], g = A.p[5];
+      // the (u, v) block of H for the condition estimate, as fast_system forms it
+      {
+        const double s00 = pers[kLeanS00], s01 = pers[kLeanS01], s11 = pers[kLeanS11];
+        const double t0a = c * s00 - sn * s01, t0b = c * s01 - sn * s11;
+        const double t1a = sn * s00 + c * s01, t1b = sn * s01 + c * s11;
+        const double h00 = c * t0a - sn * t0b, h01 = c * t1a - sn * t1b, h11 = sn * t1a + c * t1b;
+        const float cond = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cond2x2(h00, h01, h01, h11))));
+        if (grp.rank() == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
+        if (cond > 1.0e12f) return HESSIAN_SINGULAR;
+      }
+      // w = S^-1 v: L y = v, z = D^-1 y, L^T w = z
+      double w[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double t = pers[kLeanPass + i] - corr * pers[kLeanSphi + i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) t = fma(-pers[kLeanL + i * (i - 1) / 2 + k], w[k], t);
+        w[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) w[i] *= pers[kLeanDinv + i];
+#pragma unroll
+      for (int i = 4; i >= 0; --i) {
+        double t = w[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) t = fma(-pers[kLeanL + k * (k - 1) / 2 + i], w[k], t);
+        w[i] = t;
+      }
+      // M^T du = -w
+      double dd[6];
+      const double det = c * c + sn * sn;
+      dd[0] = -(c * w[0] - sn * w[1]) / det;
+      dd[1] = -(sn * w[0] + c * w[1]) / det;
+      dd[2] = (w[3] - w[4]) / (ex1 + ey1);
+      dd[3] = g * dd[2] - w[2];
+      dd[4] = -w[5] - g * dd[2];
+      dd[5] = ey1 * dd[2] - w[3];
+      float old[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const float du = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)dd[k])));
+        old[k] = A.p[k];
+        A.p[k] += du;
+      }
+      // test_for_convergence (DICe_LocalShapeFunction.cpp:433-459): u, v, theta
+      bool conv = fabsf(A.p[0] - old[0]) < kp.fast_tol && fabsf(A.p[1] - old[1]) < kp.fast_tol;
+      if (kp.en_rot) conv = conv && fabsf(A.p[2] - old[2]) < kp.fast_tol;
+      if (conv) {
+        if (kp.uncertainty && !lean_uncertainty(A, fields, nsub, gid)) return kOverflow;
+        break;
+      }
+      shift = meanG;
+    }
+    if (it > max_its) return MAX_ITERATIONS_REACHED;
+    return CORRELATION_SUCCESSFUL;
+  }
+
   __device__ __forceinline__ int gauss_newton(Affine &A, int &iters, float p_state[6], double *fields, int nsub,
                                               int gid) {
     const int max_its = kp.max_fast;
@@ -1349,91 +1716,81 @@ struct Solver {
         if (grp.rank() == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
         if (cond > 1.0e12f) return HESSIAN_SINGULAR;
         double dd[6];
-        lu_update6(H, q, dd);
+        if (!ldlt_update6(H, q, dd)) lu_update6(H, q, dd);
 #pragma unroll
         for (int k = 0; k < 6; ++k) du[k] = (float)dd[k];
       } else {
-        // lane i < 6 owns row i of the system.  The enable flags are indexed
-        // through selects (a per-lane index would move the array to scratch).
-        const int lane = grp.rank();
-        const int i = lane < 6 ? lane : 0;
-        float en_i = en[0];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) en_i = (i == k) ? en[k] : en_i;
-        double hrow[6], qv, h00, h01, h11;
+        // every lane builds the whole system (the same values in all lanes) and
+        // solves it by L D L^T; the pivoted LU is the fallback
         bool fast = false;
         if constexpr (kFast) fast = all_kept(A);
         last_fast = fast;
+        const lds_double *tot = nullptr;
+        double corr;
         if (fast) {
           // every active pixel is kept: with reference gradients the residual
-          // row is r = M phi, M(theta, ex, ey, gxy) below, so H = M S M^T with
-          // the per-subset S = sum(phi phi^T), and
-          // q = M (sum(gmf phi) - (meanG - shift) sum(phi))
-          const lds_double *tot = accumulate_gn_fast(A, shift);
+          // row is r = M phi, so H = M S M^T with the per-subset
+          // S = sum(phi phi^T), and q = M (sum(gmf phi) - (meanG - shift) sum(phi))
+          accumulate_gn_fast(A, shift);
           const double cnt = (double)n_def_active;
-          meanG = cnt > 0.0 ? (float)(uniform(tot[6]) / cnt) : 0.0f;
-          const double corr = (double)meanG - (double)shift;
-          const double c = A.c, sn = A.s, ex1 = 1.0 + (double)A.p[3], ey1 = 1.0 + (double)A.p[4], g = A.p[5];
-          // row i of M (phi = gx, gy, dx gx, dy gx, dx gy, dy gy)
-          double mi[6];
-          mi[0] = i == 0 ? c : (i == 1 ? sn : 0.0);
-          mi[1] = i == 0 ? -sn : (i == 1 ? c : 0.0);
-          mi[2] = i == 2 ? -g : (i == 3 ? 1.0 : 0.0);
-          mi[3] = i == 2 ? -ey1 : (i == 5 ? 1.0 : 0.0);
-          mi[4] = i == 2 ? ex1 : (i == 5 ? 1.0 : 0.0);
-          mi[5] = i == 2 ? g : (i == 4 ? 1.0 : 0.0);
-#pragma unroll
-          for (int k = 0; k < 6; ++k) mi[k] *= (double)en_i;
-          double vv[6];
-          qv = 0.0;
-#pragma unroll
-          for (int l = 0; l < 6; ++l) {
-            qv = fma(mi[l], tot[l] - corr * pers[36 + l], qv);
-            double t = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) t = fma(pers[6 * l + k], mi[k], t);
-            vv[l] = t;
-          }
-          hrow[0] = c * vv[0] - sn * vv[1];
-          hrow[1] = sn * vv[0] + c * vv[1];
-          hrow[2] = (double)en[2] * (ex1 * vv[4] - ey1 * vv[3] + g * (vv[5] - vv[2]));
-          hrow[3] = (double)en[3] * vv[2];
-          hrow[4] = (double)en[4] * vv[5];
-          hrow[5] = (double)en[5] * (vv[3] + vv[4]);
-          h00 = lane_value(hrow[0], 0);
-          h01 = lane_value(hrow[1], 0);
-          h11 = lane_value(hrow[1], 1);
+          meanG = cnt > 0.0 ? (float)(uniform(pers[kPersistPass + 6]) / cnt) : 0.0f;
         } else {
           // totals land in LDS
-          const lds_double *tot = accumulate_gn_wave(A, shift);
+          tot = accumulate_gn_wave(A, shift);
           const double cnt = uniform(tot[kGnTotR + 7]);
           meanG = cnt > 0.0 ? (float)(uniform(tot[kGnTotR + 6]) / cnt) : 0.0f;
-          const double corr = (double)meanG - (double)shift;
-          const lds_double *Hi = tot + kGnTotH + 6 * i;
-#pragma unroll
-          for (int k = 0; k < 6; ++k) hrow[k] = Hi[k];
-          qv = tot[kGnTotQ + i] - corr * tot[kGnTotR + i];
-          h00 = uniform(tot[kGnTotH]);
-          h01 = uniform(tot[kGnTotH + 1]);
-          h11 = uniform(tot[kGnTotH + 7]);
         }
-        // disabled parameters: identity rows and columns, zero gradient
-        const bool row_on = en_i != 0.0f;
+        corr = (double)meanG - (double)shift;
+        // the system from the LDS totals, with identity rows and columns and a
+        // zero gradient for disabled parameters and the LM term on (u, v)
+        auto system = [&](double (&H)[6][6], double (&q)[6]) {
+          if (fast) {
+            fast_system(A, corr, H, q);
+          } else {
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-          if (!row_on || en[k] == 0.0f) hrow[k] = (k == i) ? 1.0 : 0.0;
-        if (!row_on) qv = 0.0;
-        const double lm = kp.lm_alpha > 0.0f ? (double)kp.lm_alpha : 0.0;
+            for (int i = 0; i < 6; ++i) {
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-          if (k == i) hrow[k] += lm;
-        const float cond = cond2x2(h00 + lm, h01, h01, h11 + lm);
+              for (int j = i; j < 6; ++j) {
+                H[i][j] = tot[kGnTotH + 6 * i + j];
+                H[j][i] = H[i][j];
+              }
+              q[i] = tot[kGnTotQ + i] - corr * tot[kGnTotR + i];
+            }
+          }
+          finish_system(H, q);
+        };
+        double H[6][6], q[6];
+        system(H, q);
+        const float cond =
+            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cond2x2(H[0][0], H[0][1], H[0][1], H[1][1]))));
+        const int lane = grp.rank();
         if (lane == 0) fields[(size_t)F_CONDITION * nsub + gid] = cond;
         if (cond > 1.0e12f) return HESSIAN_SINGULAR;
         double dd[6];
-        wave_lu_update6(hrow, qv, dd);
+        const bool spd = ldlt_update6(H, q, dd);
+        if (!__builtin_amdgcn_readfirstlane((int)spd)) {
+          // the system again from LDS (after the fence: H is not kept live
+          // across the solve); lane i < 6 owns row i (selects, not a per-lane
+          // index, which would move the array to scratch)
+          grp.sync();
+          double H2[6][6], q2[6];
+          system(H2, q2);
+          const int i = lane < 6 ? lane : 0;
+          double hrow[6], qv = q2[0];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) du[k] = (float)dd[k];
+          for (int k = 0; k < 6; ++k) hrow[k] = H2[0][k];
+#pragma unroll
+          for (int r = 1; r < 6; ++r)
+            if (i == r) {
+#pragma unroll
+              for (int k = 0; k < 6; ++k) hrow[k] = H2[r][k];
+              qv = q2[r];
+            }
+          wave_lu_update6(hrow, qv, dd);
+        }
+        // the update is wave-uniform: keep it (and the parameters) scalar
+#pragma unroll
+        for (int k = 0; k < 6; ++k) du[k] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)dd[k])));
       }
       float old[6];
 #pragma unroll
@@ -1490,7 +1847,7 @@ struct Solver {
       if (!stage(bx0 - ref.ox + fu - 3, bx1 - ref.ox + fu + 4, by0 - ref.oy + fv - 3, by1 - ref.oy + fv + 4))
         return false;
     }
-    double acc[6] = {0, 0, 0, 0, 0, 0};  // sssig, n_uhat_g2, n_err_g2, n_err_jg2, r_tot2, n_uhat2
+    float pt[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // lane partials of unc_terms
     // BATCHED, subset away from the image border: every sample x + u, y + v
     // takes the Keys branch of interpolate_keys_fourth and the fractional
     // offsets are the fractional parts of (u, v), so the weights are computed
@@ -1511,18 +1868,12 @@ struct Solver {
     if constexpr (kIsWave)
       e_in_win = wx0 <= bx0 - ref.ox && wx0 + wpitch > bx1 - ref.ox && wy0 <= by0 - ref.oy &&
                  wy0 + wrows > by1 - ref.oy;
-    float p1 = 0.0f, p3 = 0.0f, p4 = 0.0f;  // FP32 lane partials of sum ug^2/g2, jg^2/g2, (sub_r - sub_e)^2/g2
     for_pixels([&](int x, int y, bool a) {
       (void)a;
       const int rx = x - ref.ox, ry = y - ref.oy;
       // the subset's gradient arrays: the reference gradients
       float refv, gx, gy;
       ref_pixel(rx, ry, refv, gx, gy);
-      const float g2 = gx * gx + gy * gy;
-      acc[0] += (double)g2;
-      const float inv = g2 < 1.0e-4f ? 0.0f : 1.0f / g2;
-      const float ug = u * gx + v * gy;
-      const float jg = v * gx - u * gy;
       float sub_r;
       if constexpr (kIsWave) {
         if (uniform_w)
@@ -1542,29 +1893,47 @@ struct Solver {
           e = Pix<T>::at(def.pix, ry * def.w + rx);
         }
       }
-      const float sub_e = e - refv;
-      p1 = fmaf(ug * ug, inv, p1);
-      p3 = fmaf(jg * jg, inv, p3);
-      p4 = fmaf((sub_r - sub_e) * (sub_r - sub_e), inv, p4);
+      unc_terms(u, v, gx, gy, sub_r, e - refv, pt);
     });
-    acc[1] = p1;
-    acc[3] = p3;
-    acc[4] = p4;
+    unc_record(pt, u, v, fields, nsub, gid);
+    return true;
+  }
+
+  // One pixel of computeUncertaintyFields (DICe_Objective.cpp:235-276; no image
+  // deformer, so the exact motion is 0) into FP32 lane partials: pt[0] sum |g|^2,
+  // pt[1] sum (u gx + v gy)^2 / |g|^2, pt[2] sum (v gx - u gy)^2 / |g|^2,
+  // pt[3] sum (sub_r - sub_e)^2 / |g|^2 with 1 / |g|^2 = 0 below 1e-4.
+  __device__ __forceinline__ static void unc_terms(float u, float v, float gx, float gy, float sub_r, float sub_e,
+                                                   float (&pt)[4]) {
+    const float g2 = gx * gx + gy * gy;
+    const float inv = g2 < 1.0e-4f ? 0.0f : __builtin_amdgcn_rcpf(g2);
+    const float ug = u * gx + v * gy;
+    const float jg = v * gx - u * gy;
+    const float d = sub_r - sub_e;
+    pt[0] += g2;
+    pt[1] = fmaf(ug * ug, inv, pt[1]);
+    pt[2] = fmaf(jg * jg, inv, pt[2]);
+    pt[3] = fmaf(d * d, inv, pt[3]);
+  }
+  // FIELD_1..7 from the lane partials of unc_terms (FP64 totals).
+  __device__ __forceinline__ void unc_record(const float (&pt)[4], float u, float v, double *fields, int nsub,
+                                             int gid) {
+    double acc[4] = {pt[0], pt[1], pt[2], pt[3]};
     grp.sum(acc);
-    // n_uhat_g2 and n_err_g2 add the same terms; n_uhat2 adds u^2 + v^2 once per pixel
-    acc[2] = acc[1];
-    acc[5] = (double)(u * u + v * v) * (double)n;
-    if (grp.rank() != 0) return true;
+    if (grp.rank() != 0) return;
+    // the error and estimate terms coincide (exact motion 0): n_err_g2 = n_uhat_g2;
+    // n_uhat2 adds u^2 + v^2 once per pixel
+    const double n_uhat_g2 = acc[1], n_err_jg2 = acc[2], r_tot2 = acc[3];
+    const double n_uhat2 = (double)(u * u + v * v) * (double)n;
     const double sssig = (1.0 + acc[0]) / (double)(n == 0 ? 1 : n);
-    const double cos_hat = acc[5] == 0.0 ? 1.0 : sqrt(acc[1] / acc[5]);
+    const double cos_hat = n_uhat2 == 0.0 ? 1.0 : sqrt(n_uhat_g2 / n_uhat2);
     fields[(size_t)F_FIELD_1 * nsub + gid] = (float)cos_hat;
     fields[(size_t)F_FIELD_2 * nsub + gid] = 1.0;
-    fields[(size_t)F_FIELD_3 * nsub + gid] = (float)sqrt(acc[2] + acc[3]);
-    fields[(size_t)F_FIELD_4 * nsub + gid] = cos_hat == 0.0 ? 0.0 : (float)(1.0 / cos_hat * sqrt(acc[4]));
-    fields[(size_t)F_FIELD_5 * nsub + gid] = (float)sqrt(acc[2]);
-    fields[(size_t)F_FIELD_6 * nsub + gid] = (float)sqrt(acc[4]);
+    fields[(size_t)F_FIELD_3 * nsub + gid] = (float)sqrt(n_uhat_g2 + n_err_jg2);
+    fields[(size_t)F_FIELD_4 * nsub + gid] = cos_hat == 0.0 ? 0.0 : (float)(1.0 / cos_hat * sqrt(r_tot2));
+    fields[(size_t)F_FIELD_5 * nsub + gid] = (float)sqrt(n_uhat_g2);
+    fields[(size_t)F_FIELD_6 * nsub + gid] = (float)sqrt(r_tot2);
     fields[(size_t)F_FIELD_7 * nsub + gid] = (float)sssig;
-    return true;
   }
 
   // Simplex::minimize via Objective::gamma (DICe_Simplex.cpp:60-277).  The
@@ -2059,6 +2428,120 @@ struct Solver {
       f(F_ITERATIONS) = iters;
     }
   }
+
+  // Pass 0 of a Gauss-Newton correlation in the lean kernel: run() for the
+  // common case only -- a square subset inside the reference image, no blocked
+  // pixels, not forced to Nelder-Mead, every warped pixel kept at every step
+  // and at the final parameters, a safely positive definite S (the host
+  // launches it only for six parameters without an initial-gamma threshold,
+  // LM term, momentum or deformed-image gradients).  Any other subset goes to
+  // ov_list for the general kernel, which starts it again from its initial
+  // guess: the lean pass has then written nothing but CONDITION and FIELD_1..7,
+  // which that run writes again.
+  __device__ __forceinline__ void run_lean(const DSubsets &ss, int gid, double *fields, int *fb_list,
+                                           int *fb_count, int *ov_list, int *ov_count) {
+    const int nsub = ss.n;
+    set_subset(ss, gid);
+    if (S <= 0 || !ref_in || mw > 0 || forced) {
+      push(ov_list, ov_count, gid);
+      return;
+    }
+    auto fv = [&](int s) { return fields[(size_t)s * nsub + gid]; };
+    // field-of-view test on the first frame when a subset size is set (:2033-2049)
+    if (ss.fov > 0 && kp.first_frame) {
+      const float cpx = (float)(fv(F_COORD_X) + fv(F_DISP_X));
+      const float cpy = (float)(fv(F_COORD_Y) + fv(F_DISP_Y));
+      const int h2 = ss.fov / 2;
+      if (cpx < def.ox + h2 || cpx > def.w + def.ox - h2 || cpy < def.oy + h2 || cpy > def.h + def.oy - h2) {
+        record_failed(fields, nsub, gid, INITIALIZE_FAILED_BY_EXCEPTION, -1);
+        return;
+      }
+    }
+    // Field_Value_Initializer (DICe_Initializer.cpp:398-434): the sigma of the source subset
+    if ((kp.guess ? kp.guess[(size_t)6 * nsub + gid] : fv(F_SIGMA)) == -1.0) {
+      record_failed(fields, nsub, gid, INITIALIZE_FAILED, -1);
+      return;
+    }
+    Affine A;
+    init_params(A, fields, nsub, gid);
+    const int init_status = kp.guess ? (int)kp.guess[(size_t)7 * nsub + gid] : INITIALIZE_USING_PREVIOUS_FRAME_SUCCESSFUL;
+    if (!lean_setup(gid)) {
+      push(ov_list, ov_count, gid);
+      return;
+    }
+    const float prev_u = (float)fv(F_DISP_X), prev_v = (float)fv(F_DISP_Y), prev_t = (float)fv(F_ROT_Z);
+    float p_state[6];
+    int iters = -1;
+    const int corr = gauss_newton_lean(A, iters, p_state, fields, nsub, gid);
+    if (corr == kOverflow) {
+      push(ov_list, ov_count, gid);
+      return;
+    }
+    const float du = A.p[0] - prev_u, dv = A.p[1] - prev_v, dt = (kp.en_rot ? A.p[2] : 0.0f) - prev_t;
+    const bool jump = fabsf(du) > kp.disp_jump_tol || fabsf(dv) > kp.disp_jump_tol || fabsf(dt) > kp.theta_jump_tol;
+    if (corr != CORRELATION_SUCCESSFUL || jump) {
+      if (kp.opt == 3 || kp.opt == 4) {
+        // the fallback starts again from the initial guess (:2238, :2266)
+        push(fb_list, fb_count, gid);
+        return;
+      }
+      if (corr != CORRELATION_SUCCESSFUL) {
+        record_failed(fields, nsub, gid, corr, iters);
+        return;
+      }
+    }
+    // sigma with the last warp keeping every pixel: noise over the subset bbox
+    // shifted by (u, v), gradient sums sum(gx^2), sum(gy^2) from S
+    // (Objective::sigma, DICe_Objective.cpp:150-184)
+    const float noise = noise_std_dev(bx0, bx1, by0, by1, A.p[0], A.p[1]);
+    float sig = -1.0f;
+    if (noise >= 0.0f) {
+      const float sgx = (float)uniform(pers[kLeanS00]), sgy = (float)uniform(pers[kLeanS11]);
+      const float sg = sgx > sgy ? sgy : sgx;
+      sig = sg > 0.0f ? (float)sqrt(2.0 * (double)noise * (double)noise / (double)sg) : -1.0f;
+    }
+    if (sig < 0.0f) {
+      record_failed(fields, nsub, gid, FRAME_FAILED_DUE_TO_NEGATIVE_SIGMA, iters);
+      return;
+    }
+    A.prepare();
+    if (!stage_affine(A) || !all_kept(A)) {
+      push(ov_list, ov_count, gid);
+      return;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    gamma_sums_every<true>(A, acc);
+    grp.sum(acc);
+    acc[0] = (double)n_def_active;
+    int active;
+    float contrast;
+    const float gam = gamma_of(acc, active, contrast);
+    if ((kp.final_gamma_thr != -1.0 && gam > kp.final_gamma_thr) || gam < 0.0f) {
+      record_failed(fields, nsub, gid, FRAME_FAILED_DUE_TO_HIGH_GAMMA, iters);
+      return;
+    }
+    if (jump) {
+      record_failed(fields, nsub, gid, JUMP_TOLERANCE_EXCEEDED, iters);
+      return;
+    }
+    if (grp.rank() == 0) {
+      auto f = [&](int s) -> double & { return fields[(size_t)s * nsub + gid]; };
+      f(F_DISP_X) = A.p[0];
+      f(F_DISP_Y) = A.p[1];
+      if (kp.en_rot) f(F_ROT_Z) = A.p[2];
+      if (kp.en_normal) { f(F_NS_XX) = A.p[3]; f(F_NS_YY) = A.p[4]; }
+      if (kp.en_shear) f(F_SS_XY) = A.p[5];
+      f(F_SIGMA) = sig;
+      f(F_MATCH) = 0.0;
+      f(F_GAMMA) = gam;
+      f(F_BETA) = 0.0;
+      f(F_NOISE) = noise;
+      f(F_CONTRAST) = contrast;
+      f(F_ACTIVE_PIXELS) = active;
+      f(F_STATUS) = init_status;
+      f(F_ITERATIONS) = iters;
+    }
+  }
 };
 
 // A PLAIN pass over a device list launches min(n, kListGroups) workgroups.
@@ -2496,6 +2979,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
     s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
                                 pa.ov_count, pa.nm_result);
   }
+}
+
+// BATCHED lean Gauss-Newton pass: the work loop of correlate_batched_kernel
+// over Solver::run_lean; the subsets it hands over go to pa.ov_list.
+template <typename T, int INTERP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHED_WAVES_PER_EU))) void correlate_lean_kernel(
+    KParams kp, DImage ref, DImage def, DSubsets ss, double *fields, PassArgs pa, int lds_per_wave) {
+  extern __shared__ float lds[];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const Items items{pa.items, pa.item_count};
+  WaveGroup g{lane};
+  Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
+                                        lds_per_wave);
+  const int total = items.total(ss.n);
+  if ((int)blockIdx.x * (int)(blockDim.x / kWave) + wave >= total) return;  // as in correlate_batched_kernel
+  int range = (int)(blockIdx.x % kWorkRanges);
+#pragma unroll 1
+  for (int visited = 0; visited < kWorkRanges;) {
+    const int lo = (int)((long long)total * range / kWorkRanges);
+    const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
+    int k = 0;
+    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    k = __builtin_amdgcn_readfirstlane(k) + lo;
+    if (k >= hi) {
+      range = range + 1 == kWorkRanges ? 0 : range + 1;
+      ++visited;
+      continue;
+    }
+    s.run_lean(ss, items.at(k), fields, pa.fb_list, pa.fb_count, pa.ov_list, pa.ov_count);
+  }
+}
+
+template <typename T, int INTERP>
+hipError_t launch_lean_variant(const Launch &L, const KParams &kp, const DImage &ref, const DImage &def,
+                               const DSubsets &subs, double *fields, const PassArgs &pa, hipStream_t stream) {
+  const size_t shm = (size_t)L.waves_per_group * L.lds_floats_per_wave * sizeof(float);
+  hipLaunchKernelGGL((correlate_lean_kernel<T, INTERP>), dim3(L.groups), dim3(L.waves_per_group * kWave), shm,
+                     stream, kp, ref, def, subs, fields, pa, L.lds_floats_per_wave);
+  return hipGetLastError();
 }
 
 // Differentiation path: forward gamma or adjoint for every item.

@@ -50,6 +50,11 @@ struct KParams {
   int uncertainty;
   int first_frame;
   int has_forced;        // some subset has force[i] set: a GN primary pass hands them to a Nelder-Mead pass
+  // lean kernel: per-subset cache of kLeanCacheDoubles doubles (or null), and the
+  // generation its entries must carry to be current (entries of S that is not
+  // positive definite carry -generation)
+  double *lean_cache;
+  int lean_gen;
   // USE_NEIGHBOR_VALUES: per-subset initial guesses, slot-major n x kGuessSlots
   // (u, v, theta, ex, ey, gxy, sigma of the source subset, init status), or null
   // to start every subset from its own fields (USE_FIELD_VALUES)
@@ -97,8 +102,18 @@ constexpr int kGnTotOffset = kGnSums * kGnRedStride + 1;
 constexpr int kGnTotQ = 0, kGnTotH = 8, kGnTotR = 44;
 // Per-subset constants of the BATCHED fast Gauss-Newton pass at the end of a
 // wave's slot: the 6 x 6 matrix sum(phi phi^T), sum(phi), sum(F) and the pixel
-// count, 44 doubles.
-constexpr int kPersistFloats = 88;
+// count (44 doubles), then the sums of the current pass, sum(gmf phi) and
+// sum(G) (7 doubles), and one double of padding.
+constexpr int kPersistFloats = 104;
+constexpr int kPersistPass = 44;  // double offset of the current pass's sums
+// The lean kernel's layout of the same area (double offsets): the L D L^T
+// factors of S (L strictly lower, packed by rows, 15; then 1/d, 6), sum(phi)
+// (6), S00, S01, S11, the mean reference intensity, the cache tag, then the
+// current pass's sums (7).  The first kLeanCacheDoubles are also the per-subset
+// cache entry in global memory.
+constexpr int kLeanL = 0, kLeanDinv = 15, kLeanSphi = 21, kLeanS00 = 27, kLeanS01 = 28, kLeanS11 = 29;
+constexpr int kLeanMeanF = 30, kLeanTag = 31, kLeanPass = 32;
+constexpr int kLeanCacheDoubles = 32;
 constexpr int kMinSlotFloats = kGnTotOffset + 2 * 52 + kPersistFloats;  // smallest LDS slot per wave
 
 struct Launch {
@@ -148,6 +163,9 @@ constexpr int kWorkRanges = 8;
 constexpr int kWorkHeader = 16;
 constexpr int kWorkFbCount = kWorkRanges, kWorkOv0Count = kWorkRanges + 1, kWorkOv1Count = kWorkRanges + 2;
 constexpr int kWorkLaneCount = kWorkRanges + 3;  // work counter of simplex_lanes_kernel
+// count of the lean pass's hand-over list, which lives in the storage of the pass-1
+// overflow list (pass 1 starts after pass 0 has finished with it)
+constexpr int kWorkLeanCount = kWorkRanges + 4;
 
 // Device ints of the work area of launch_correlate: the header, the fallback
 // and overflow lists (n each), the Nelder-Mead results (kNmStride per subset).

@@ -16,6 +16,8 @@ hipError_t launch_gn_batched(int, int, int, const Launch &, const KParams &, con
                              const DSubsets &, double *, const PassArgs &, hipStream_t);
 hipError_t launch_simplex_plain(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
                                 const DSubsets &, double *, const PassArgs &, hipStream_t);
+hipError_t launch_gn_lean(int, int, const Launch &, const KParams &, const DImage &, const DImage &, const DSubsets &,
+                          double *, const PassArgs &, hipStream_t);
 hipError_t launch_simplex_batched(int, int, int, const Launch &, const KParams &, const DImage &, const DImage &,
                                   const DSubsets &, double *, const PassArgs &, hipStream_t);
 
@@ -663,7 +665,7 @@ hipError_t launch_reset_listed(const int *list, const int *count, const double *
 // epilogue of each subset around them.
 static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const KParams &kp, const DImage &ref,
                               const DImage &def, const DSubsets &subs, double *fields, PassArgs pa, int *ov_list,
-                              int *ov_count, float *nm, hipStream_t stream) {
+                              int *ov_count, float *nm, hipStream_t stream, int *lean_list = nullptr) {
   const int defgrad = simplex ? 0 : kp.def_grads;
   PassLauncher plain = simplex ? launch_simplex_plain : launch_gn_plain;
   pa.nm_result = nullptr;
@@ -673,6 +675,22 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
   }
   PassLauncher batched = simplex ? launch_simplex_batched : launch_gn_batched;
   hipError_t e;
+  // Gauss-Newton pass 0 of square subsets with none of the options the lean
+  // kernel leaves out: the lean kernel takes every subset first and hands the
+  // others (and those that leave its common case) to the general kernel
+  if (!simplex && pa.pass == 0 && subs.S > 0 && !subs.mbox && !kp.def_grads &&
+      kp.momentum == 0.0f && kp.init_gamma_thr == -1.0 && !kp.has_forced && kp.en_rot && kp.en_normal &&
+      kp.en_shear && kp.lm_alpha <= 0.0f && lean_list) {
+    e = launch_zero_ints(pa.counter, kWorkRanges, stream);
+    if (e != hipSuccess) return e;
+    PassArgs pl = pa;
+    pl.ov_list = lean_list;
+    pl.ov_count = pa.counter + kWorkLeanCount;
+    e = launch_gn_lean(storage, kp.interp, L, kp, ref, def, subs, fields, pl, stream);
+    if (e != hipSuccess) return e;
+    pa.items = lean_list;
+    pa.item_count = pa.counter + kWorkLeanCount;
+  }
   if (simplex && L.lane_simplex) {
     int *lane_counter = pa.counter + kWorkLaneCount;
     e = launch_zero_ints(lane_counter, 1, stream);
@@ -707,7 +725,8 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
   if (e != hipSuccess) return e;
   const bool simplex_first = kp.opt == 0 || kp.opt == 3;
   PassArgs pa{0, items, items ? item_count : nullptr, fb_list, work + kWorkFbCount, nullptr, nullptr, work, nullptr};
-  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, nm, stream);
+  e = launch_pass(simplex_first, storage, L, kp, ref, def, subs, fields, pa, ov0, work + kWorkOv0Count, nm, stream,
+                  ov1);
   // the second pass: the fallback optimizer, or Nelder-Mead for the forced subsets of a Gauss-Newton pass
   if (e != hipSuccess || !(kp.opt == 3 || kp.opt == 4 || (kp.has_forced && !simplex_first))) return e;
   PassArgs pb{1, fb_list, work + kWorkFbCount, nullptr, nullptr, nullptr, nullptr, work, nullptr};
