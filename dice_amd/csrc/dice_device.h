@@ -635,6 +635,34 @@ struct Solver {
       if (col >= S) { col -= S; ++row; }
     }
   }
+  // The lean kernel's walk of a square subset (wave groups): strips of 32
+  // columns, each half-wave taking 32 consecutive pixels of one row, so that
+  // the taps it reads from the LDS window are consecutive words (no bank
+  // conflicts while the warp is close to a translation); the last S mod 32
+  // columns R at a time, 64 / R rows per step.  f(x, y).
+  template <class F>
+  __device__ __forceinline__ void for_strips(F &&f) const {
+    const int lane = grp.rank();
+    const int x0 = cx - hw, y0 = cy - hw;
+    const int nfull = S >> 5;
+#pragma unroll 1
+    for (int st = 0; st < nfull; ++st) {
+      const int x = max(x0 + (st << 5) + (lane & 31), 0);
+#pragma unroll 1
+      for (int r = lane >> 5; r < S; r += 2) f(x, max(y0 + r, 0));
+    }
+    const int R = S & 31;
+    if (R) {
+      const int rows = kWave / R;
+      // lane / R in float: exact for lane < 64 and R < 32
+      const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
+      if (lr < rows) {
+        const int x = max(x0 + (nfull << 5) + lc, 0);
+#pragma unroll 1
+        for (int r = lr; r < S; r += rows) f(x, max(y0 + r, 0));
+      }
+    }
+  }
   template <class F>
   __device__ __forceinline__ void for_pixels(F &&f) const {
     const int r0 = grp.rank();
@@ -789,14 +817,26 @@ struct Solver {
       if (hand_over) overflow = true;
       return false;
     }
+    // stage a ring of up to 2 px more where the slot has room (within the
+    // image), so that the windows of the next steps of this subset, which move
+    // by a fraction of a pixel, are usually inside this one
+    {
+      int e = 2;
+      while (e > 0 && (ww + 2 * e) * (hh + 2 * e) > win_cap) --e;
+      x0 = max(x0 - e, 0);
+      y0 = max(y0 - e, 0);
+      x1 = min(x1 + e, def.w - 1);
+      y1 = min(y1 + e, def.h - 1);
+    }
+    const int ww_s = x1 - x0 + 1, hh_s = y1 - y0 + 1;
     wx0 = __builtin_amdgcn_readfirstlane(x0);  // uniform already: keep it scalar
     wy0 = __builtin_amdgcn_readfirstlane(y0);
-    wpitch = __builtin_amdgcn_readfirstlane(ww);
-    wrows = __builtin_amdgcn_readfirstlane(hh);
+    wpitch = __builtin_amdgcn_readfirstlane(ww_s);
+    wrows = __builtin_amdgcn_readfirstlane(hh_s);
     grp.sync();
     const auto *P = Pix<T>::ptr(def.pix) + y0 * def.w + x0;
-    const float inv = 1.0f / (float)ww;
-    const int tot = ww * hh;
+    const float inv = 1.0f / (float)ww_s;
+    const int tot = ww_s * hh_s;
     if constexpr (kIsWave && std::is_same<T, float>::value) {
       // direct global -> LDS loads (global_load_lds_dword): the 64 lanes of one
       // load fill 64 consecutive floats of the slot, so window element k =
@@ -808,7 +848,7 @@ struct Solver {
         const int k = k0 + lane;
         if (k < tot) {
           const int r = (int)(((float)k + 0.5f) * inv);
-          const int c = k - r * ww;
+          const int c = k - r * ww_s;
           __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(P + (r * def.w + c)),
                                            (__attribute__((address_space(3))) void *)(win + k0), 4, 0, 0);
         }
@@ -818,7 +858,7 @@ struct Solver {
 #pragma unroll 4
       for (int k = grp.rank(); k < tot; k += G::kSize) {
         const int r = (int)(((float)k + 0.5f) * inv);
-        const int c = k - r * ww;
+        const int c = k - r * ww_s;
         win[k] = (float)P[r * def.w + c];
       }
     }
@@ -852,11 +892,14 @@ struct Solver {
       (void)A;
       return true;
     } else {
+      // the map is affine, so every pixel's position lies between the corner
+      // extremes; a 1e-3 px margin covers the rounding of the per-pixel map
       float lo_x, hi_x, lo_y, hi_y;
       warp_bounds(A, lo_x, hi_x, lo_y, hi_y);
       const int lead = INTERP == 0 ? 0 : 2, trail = INTERP == 0 ? 1 : 3;
-      return stage(cx - def.ox + (int)floorf(lo_x) - lead - 1, cx - def.ox + (int)floorf(hi_x) + trail + 1,
-                   cy - def.oy + (int)floorf(lo_y) - lead - 1, cy - def.oy + (int)floorf(hi_y) + trail + 1);
+      const float m = 1e-3f;
+      return stage(cx - def.ox + (int)floorf(lo_x - m) - lead, cx - def.ox + (int)floorf(hi_x + m) + trail,
+                   cy - def.oy + (int)floorf(lo_y - m) - lead, cy - def.oy + (int)floorf(hi_y + m) + trail);
     }
   }
 
@@ -1094,11 +1137,7 @@ struct Solver {
     float pa[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) pa[k] = 0.0f;
-    walk<LEAN>([&](auto inside, int x, int y, bool a) {
-      if (!a) return;
-      // the reference loads first: they are in flight during the interpolation
-      float Fv, gx, gy;
-      ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
+    auto body = [&](int x, int y, float Fv, float gx, float gy) {
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
@@ -1112,7 +1151,22 @@ struct Solver {
       pa[4] = fmaf(g2, dx, pa[4]);
       pa[5] = fmaf(g2, dy, pa[5]);
       pa[6] += Gv;
-    });
+    };
+    if constexpr (LEAN) {
+      for_strips([&](int x, int y) {
+        float Fv, gx, gy;
+        ref_pixel<true>(x - ref.ox, y - ref.oy, Fv, gx, gy);
+        body(x, y, Fv, gx, gy);
+      });
+    } else {
+      for_pixels_ref([&](auto inside, int x, int y, bool a) {
+        if (!a) return;
+        // the reference loads first: they are in flight during the interpolation
+        float Fv, gx, gy;
+        ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
+        body(x, y, Fv, gx, gy);
+      });
+    }
     double fs[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) fs[k] = (double)pa[k];
@@ -1160,10 +1214,8 @@ struct Solver {
   // (acc[0], the count, is left to the caller: the defined-active count).
   template <bool LEAN>
   __device__ __forceinline__ void gamma_sums_every(const Affine &A, double (&acc)[6]) {
-    walk<LEAN>([&](auto inside, int x, int y, bool a) {
-      if (!a) return;
-      // the reference load first: it is in flight during the interpolation
-      const double r = ref_value<decltype(inside)::value>(x - ref.ox, y - ref.oy);
+    auto body = [&](int x, int y, float rv) {
+      const double r = rv;
       const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
@@ -1173,7 +1225,16 @@ struct Solver {
       acc[3] += r;
       acc[4] += r * r;
       acc[5] += d * r;
-    });
+    };
+    if constexpr (LEAN) {
+      for_strips([&](int x, int y) { body(x, y, ref_value<true>(x - ref.ox, y - ref.oy)); });
+    } else {
+      for_pixels_ref([&](auto inside, int x, int y, bool a) {
+        if (!a) return;
+        // the reference load first: it is in flight during the interpolation
+        body(x, y, ref_value<decltype(inside)::value>(x - ref.ox, y - ref.oy));
+      });
+    }
   }
   // ZNSSD gamma from the FP64 sums (n, sd, sdd, sr, srr, sdr) of the kept pixels.
   __device__ __forceinline__ float gamma_of(const double (&acc)[6], int &active, float &contrast) const {
@@ -1568,13 +1629,14 @@ struct Solver {
     const bool uniform_w = (float)lx0 + u > 2.5f + m && (float)lx1 + u < (float)def.w - 3.5f - m &&
                            (float)ly0 + v > 2.5f + m && (float)ly1 + v < (float)def.h - 3.5f - m;
     if (!uniform_w || lx1 > def.w - 2 || ly1 > def.h - 2) return false;
-    if (!stage(lx0 + fu - 3, lx1 + fu + 4, ly0 + fv - 3, ly1 + fv + 4)) return false;
+    // the taps of x + u are x + fu - 2 .. x + fu + 3 (integer offsets: no rounding margin)
+    if (!stage(lx0 + fu - 2, lx1 + fu + 3, ly0 + fv - 2, ly1 + fv + 3)) return false;
     if (!(wx0 <= lx0 && wx0 + wpitch > lx1 && wy0 <= ly0 && wy0 + wrows > ly1)) return false;
     float wxu[6], wyu[6];
     keys_weights(u - (float)fu, wxu);
     keys_weights(v - (float)fv, wyu);
     float pt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for_square([&](int x, int y, bool) {
+    for_strips([&](int x, int y) {
       const int rx = x - ref.ox, ry = y - ref.oy;
       float refv, gx, gy;
       ref_pixel<true>(rx, ry, refv, gx, gy);
