@@ -635,33 +635,46 @@ struct Solver {
       if (col >= S) { col -= S; ++row; }
     }
   }
-  // The lean kernel's walk of a square subset (wave groups): strips of 32
-  // columns, each half-wave taking 32 consecutive pixels of one row, so that
-  // the taps it reads from the LDS window are consecutive words (no bank
-  // conflicts while the warp is close to a translation); the last S mod 32
-  // columns R at a time, 64 / R rows per step.  f(x, y).
+  // The lean kernel's walk of a square subset (wave groups) whose first row and
+  // column lie in the image: strips of 32 columns, each half-wave taking 32
+  // consecutive pixels of one row, so that the taps it reads from the LDS
+  // window are consecutive words (no bank conflicts while the warp is close to
+  // a translation); the last S mod 32 columns R at a time, 64 / R rows per
+  // step.  Each lane keeps one column per strip and steps down its rows:
+  // f(x, y, dx, dy, o) with (dx, dy) = (x - cx, y - cy) and o the reference
+  // pixel offset (y - ref.oy) * ref.w + x - ref.ox.
   template <class F>
   __device__ __forceinline__ void for_strips(F &&f) const {
     const int lane = grp.rank();
     const int x0 = cx - hw, y0 = cy - hw;
     const int nfull = S >> 5;
+    auto column = [&](int x, int r0, int step) {
+      int o = (y0 + r0 - ref.oy) * ref.w + (x - ref.ox);
+      const int ostep = step * ref.w;
+      const float dx = (float)(x - cx), fstep = (float)step;
+      float dy = (float)(y0 + r0 - cy);
 #pragma unroll 1
-    for (int st = 0; st < nfull; ++st) {
-      const int x = max(x0 + (st << 5) + (lane & 31), 0);
+      for (int r = r0; r < S; r += step) {
+        f(x, y0 + r, dx, dy, o);
+        o += ostep;
+        dy += fstep;
+      }
+    };
 #pragma unroll 1
-      for (int r = lane >> 5; r < S; r += 2) f(x, max(y0 + r, 0));
-    }
+    for (int st = 0; st < nfull; ++st) column(x0 + (st << 5) + (lane & 31), lane >> 5, 2);
     const int R = S & 31;
     if (R) {
       const int rows = kWave / R;
       // lane / R in float: exact for lane < 64 and R < 32
       const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
-      if (lr < rows) {
-        const int x = max(x0 + (nfull << 5) + lc, 0);
-#pragma unroll 1
-        for (int r = lr; r < S; r += rows) f(x, max(y0 + r, 0));
-      }
+      if (lr < rows) column(x0 + (nfull << 5) + lc, lr, rows);
     }
+  }
+  // reference intensity and gradients at pixel offset o of the reference image
+  __device__ __forceinline__ void ref_at(int o, float &F, float &gx, float &gy) const {
+    F = Pix<T>::at_u(ref.pix, (unsigned)o);
+    gx = ldf_u(ref.gx, (unsigned)o);
+    gy = ldf_u(ref.gy, (unsigned)o);
   }
   template <class F>
   __device__ __forceinline__ void for_pixels(F &&f) const {
@@ -1137,8 +1150,7 @@ struct Solver {
     float pa[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) pa[k] = 0.0f;
-    auto body = [&](int x, int y, float Fv, float gx, float gy) {
-      const float dx = (float)(x - cx), dy = (float)(y - cy);
+    auto body = [&](float dx, float dy, float Fv, float gx, float gy) {
       float ox, oy;
       A.disp(dx, dy, ox, oy);
       const float Gv = sample(ox, oy);
@@ -1153,10 +1165,10 @@ struct Solver {
       pa[6] += Gv;
     };
     if constexpr (LEAN) {
-      for_strips([&](int x, int y) {
+      for_strips([&](int, int, float dx, float dy, int o) {
         float Fv, gx, gy;
-        ref_pixel<true>(x - ref.ox, y - ref.oy, Fv, gx, gy);
-        body(x, y, Fv, gx, gy);
+        ref_at(o, Fv, gx, gy);
+        body(dx, dy, Fv, gx, gy);
       });
     } else {
       for_pixels_ref([&](auto inside, int x, int y, bool a) {
@@ -1164,7 +1176,7 @@ struct Solver {
         // the reference loads first: they are in flight during the interpolation
         float Fv, gx, gy;
         ref_pixel<decltype(inside)::value>(x - ref.ox, y - ref.oy, Fv, gx, gy);
-        body(x, y, Fv, gx, gy);
+        body((float)(x - cx), (float)(y - cy), Fv, gx, gy);
       });
     }
     double fs[7];
@@ -1214,9 +1226,8 @@ struct Solver {
   // (acc[0], the count, is left to the caller: the defined-active count).
   template <bool LEAN>
   __device__ __forceinline__ void gamma_sums_every(const Affine &A, double (&acc)[6]) {
-    auto body = [&](int x, int y, float rv) {
+    auto body = [&](float dx, float dy, float rv) {
       const double r = rv;
-      const float dx = (float)(x - cx), dy = (float)(y - cy);
       float ox, oy;
       A.disp(dx, dy, ox, oy);
       const double d = sample(ox, oy);
@@ -1227,12 +1238,12 @@ struct Solver {
       acc[5] += d * r;
     };
     if constexpr (LEAN) {
-      for_strips([&](int x, int y) { body(x, y, ref_value<true>(x - ref.ox, y - ref.oy)); });
+      for_strips([&](int, int, float dx, float dy, int o) { body(dx, dy, Pix<T>::at_u(ref.pix, (unsigned)o)); });
     } else {
       for_pixels_ref([&](auto inside, int x, int y, bool a) {
         if (!a) return;
         // the reference load first: it is in flight during the interpolation
-        body(x, y, ref_value<decltype(inside)::value>(x - ref.ox, y - ref.oy));
+        body((float)(x - cx), (float)(y - cy), ref_value<decltype(inside)::value>(x - ref.ox, y - ref.oy));
       });
     }
   }
@@ -1636,15 +1647,74 @@ struct Solver {
     keys_weights(u - (float)fu, wxu);
     keys_weights(v - (float)fv, wyu);
     float pt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for_strips([&](int x, int y) {
-      const int rx = x - ref.ox, ry = y - ref.oy;
+    // the walk of for_strips; the Keys sum at (x + u, y + v) is separable with
+    // these fixed weights: each lane keeps the row sums of its column for the
+    // six window rows under its current pixel and, stepping down two rows, adds
+    // the sums of two new rows (the same products and order as keys_sum)
+    const int lane = grp.rank();
+    const int x0 = cx - hw, y0 = cy - hw;
+    auto rowsum = [&](const lds_float *p) {
+      float rs = p[0] * wxu[0];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) rs = fmaf(p[k], wxu[k], rs);
+      return rs;
+    };
+    auto pixel = [&](int o, float interp, const lds_float *e) {
       float refv, gx, gy;
-      ref_pixel<true>(rx, ry, refv, gx, gy);
-      const float sub_r =
-          keys_sum(win + (__umul24(ry + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0)), wpitch, wxu, wyu) - refv;
-      const float e = win[__umul24(ry - wy0, wpitch) + (rx - wx0)];
-      unc_terms(u, v, gx, gy, sub_r, e - refv, pt);
-    });
+      ref_at(o, refv, gx, gy);
+      unc_terms(u, v, gx, gy, interp - refv, *e - refv, pt);
+    };
+    auto column = [&](int x, int r0, int step) {
+      const int rx = x - ref.ox, ry0 = y0 + r0 - ref.oy;
+      int o = ry0 * ref.w + rx;
+      const int ostep = step * ref.w;
+      // taps of (rx + u, ry + v): window rows ry + fv - 2 .. + 3, columns rx + fu - 2 .. + 3
+      const lds_float *t = win + (__umul24(ry0 + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0));
+      const lds_float *e = win + (__umul24(ry0 - wy0, wpitch) + (rx - wx0));
+      const int tstep = step * wpitch;
+      if (step == 2) {
+        float h0 = rowsum(t), h1 = rowsum(t + wpitch), h2 = rowsum(t + 2 * wpitch), h3 = rowsum(t + 3 * wpitch),
+              h4 = rowsum(t + 4 * wpitch), h5 = rowsum(t + 5 * wpitch);
+#pragma unroll 1
+        for (int r = r0; r < S; r += 2) {
+          float vv = h0 * wyu[0];
+          vv = fmaf(h1, wyu[1], vv);
+          vv = fmaf(h2, wyu[2], vv);
+          vv = fmaf(h3, wyu[3], vv);
+          vv = fmaf(h4, wyu[4], vv);
+          vv = fmaf(h5, wyu[5], vv);
+          pixel(o, vv, e);
+          o += ostep;
+          e += tstep;
+          t += tstep;
+          h0 = h2;
+          h1 = h3;
+          h2 = h4;
+          h3 = h5;
+          if (r + 2 < S) {
+            h4 = rowsum(t + 4 * wpitch);
+            h5 = rowsum(t + 5 * wpitch);
+          }
+        }
+      } else {
+#pragma unroll 1
+        for (int r = r0; r < S; r += step) {
+          pixel(o, keys_sum(t, wpitch, wxu, wyu), e);
+          o += ostep;
+          e += tstep;
+          t += tstep;
+        }
+      }
+    };
+    const int nfull = S >> 5;
+#pragma unroll 1
+    for (int st = 0; st < nfull; ++st) column(x0 + (st << 5) + (lane & 31), lane >> 5, 2);
+    const int R = S & 31;
+    if (R) {
+      const int rows = kWave / R;
+      const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
+      if (lr < rows) column(x0 + (nfull << 5) + lc, lr, rows);
+    }
     unc_record(pt, u, v, fields, nsub, gid);
     return true;
   }
@@ -2169,6 +2239,7 @@ struct Solver {
 
   // Immerkaer noise estimate (Subset::noise_std_dev, DICe_Subset.cpp:622-703)
   // over the bbox of the kept pixels at p_state, shifted by the final (u, v).
+  template <bool LEAN = false>
   __device__ __forceinline__ float noise_std_dev(int min_x, int max_x, int min_y, int max_y, float u, float v) {
     min_x = (int)((float)min_x + u);
     max_x = (int)((float)max_x + u);
@@ -2184,6 +2255,56 @@ struct Solver {
     if (hh < 3) hh = 3;
     const int bw = max_x - min_x, bh = max_y - min_y;
     double acc[1] = {0.0};
+    if constexpr (LEAN) {
+      // lean kernel, the region and its 1-px ring inside the image and staged:
+      // the mask is separable, conv = c(y - 1) - 2 c(y) + c(y + 1) with the row
+      // filter c = p(x - 1) - 2 p(x) + p(x + 1); each lane walks one column of a
+      // 32-column strip two rows at a time (or a remainder column) keeping the
+      // row-filtered values of the rows around its pixel; FP32 lane partials
+      const int lx0 = min_x - ox, ly0 = min_y - oy;
+      if (bw > 0 && bh > 0 && lx0 >= 1 && ly0 >= 1 && lx0 + bw <= iw - 1 && ly0 + bh <= ih - 1 &&
+          stage(lx0 - 1, lx0 + bw, ly0 - 1, ly0 + bh, false)) {
+        const int lane = grp.rank();
+        float part = 0.0f;
+        auto rowf = [&](const lds_float *p) { return (p[-1] + p[1]) - 2.0f * p[0]; };
+        auto column = [&](int c, int r0, int step) {
+          const lds_float *t = win + (__umul24(ly0 + r0 - wy0, wpitch) + (lx0 + c - wx0));
+          const int tstep = step * wpitch;
+          if (step == 2) {
+            float cm = rowf(t - wpitch), c0 = rowf(t), cp = rowf(t + wpitch);
+#pragma unroll 1
+            for (int r = r0; r < bh; r += 2) {
+              part += fabsf((cm + cp) - 2.0f * c0);
+              t += tstep;
+              cm = cp;
+              if (r + 2 < bh) {
+                c0 = rowf(t);
+                cp = rowf(t + wpitch);
+              }
+            }
+          } else {
+#pragma unroll 1
+            for (int r = r0; r < bh; r += step) {
+              part += fabsf((rowf(t - wpitch) + rowf(t + wpitch)) - 2.0f * rowf(t));
+              t += tstep;
+            }
+          }
+        };
+        const int nfull = bw >> 5;
+#pragma unroll 1
+        for (int st = 0; st < nfull; ++st) column((st << 5) + (lane & 31), lane >> 5, 2);
+        const int R = bw & 31;
+        if (R) {
+          const int rows = kWave / R;
+          const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
+          if (lr < rows) column((nfull << 5) + lc, lr, rows);
+        }
+        acc[0] = part;
+        grp.sum(acc);
+        const float variance = (float)acc[0];
+        return (float)((double)variance * (sqrt(0.5 * 3.14159265358979323846) / (6.0 * (ww - 2) * (hh - 2))));
+      }
+    }
     if (bw > 0 && bh > 0) {
       const int tot = bw * bh;
       // BATCHED: the bbox and its 1-px ring staged in LDS when it fits (the
@@ -2504,7 +2625,7 @@ struct Solver {
                                            int *fb_count, int *ov_list, int *ov_count) {
     const int nsub = ss.n;
     set_subset(ss, gid);
-    if (S <= 0 || !ref_in || mw > 0 || forced) {
+    if (S <= 0 || !ref_in || cx - hw < 0 || cy - hw < 0 || mw > 0 || forced) {
       push(ov_list, ov_count, gid);
       return;
     }
@@ -2555,7 +2676,7 @@ struct Solver {
     // sigma with the last warp keeping every pixel: noise over the subset bbox
     // shifted by (u, v), gradient sums sum(gx^2), sum(gy^2) from S
     // (Objective::sigma, DICe_Objective.cpp:150-184)
-    const float noise = noise_std_dev(bx0, bx1, by0, by1, A.p[0], A.p[1]);
+    const float noise = noise_std_dev<true>(bx0, bx1, by0, by1, A.p[0], A.p[1]);
     float sig = -1.0f;
     if (noise >= 0.0f) {
       const float sgx = (float)uniform(pers[kLeanS00]), sgy = (float)uniform(pers[kLeanS11]);
