@@ -840,6 +840,12 @@ struct Solver {
       y0 = max(y0 - e, 0);
       x1 = min(x1 + e, def.w - 1);
       y1 = min(y1 + e, def.h - 1);
+      // an odd pitch where there is room: the 32 words of a column that a
+      // half-wave reads at 32 consecutive rows then fall in 32 different banks
+      if (((x1 - x0 + 1) & 1) == 0 && (x1 - x0 + 2) * (y1 - y0 + 1) <= win_cap) {
+        if (x1 + 1 < def.w) ++x1;
+        else if (x0 > 0) --x0;
+      }
     }
     const int ww_s = x1 - x0 + 1, hh_s = y1 - y0 + 1;
     wx0 = __builtin_amdgcn_readfirstlane(x0);  // uniform already: keep it scalar
@@ -1653,7 +1659,7 @@ struct Solver {
     // the sums of two new rows (the same products and order as keys_sum)
     const int lane = grp.rank();
     const int x0 = cx - hw, y0 = cy - hw;
-    auto rowsum = [&](const lds_float *p) {
+    auto rowsum = [&](const auto *p) {
       float rs = p[0] * wxu[0];
 #pragma unroll
       for (int k = 1; k < 6; ++k) rs = fmaf(p[k], wxu[k], rs);
@@ -1672,18 +1678,38 @@ struct Solver {
       const lds_float *t = win + (__umul24(ry0 + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0));
       const lds_float *e = win + (__umul24(ry0 - wy0, wpitch) + (rx - wx0));
       const int tstep = step * wpitch;
-      if (step == 2) {
+      {
         float h0 = rowsum(t), h1 = rowsum(t + wpitch), h2 = rowsum(t + 2 * wpitch), h3 = rowsum(t + 3 * wpitch),
               h4 = rowsum(t + 4 * wpitch), h5 = rowsum(t + 5 * wpitch);
+        // this column in the last row of the window: the rows read ahead at the last step
+        // are clamped to it (their sums are not used), so every step issues its
+        // loads without a branch and they overlap the arithmetic
+        const lds_float *tlast = win + (__umul24(wrows - 1, wpitch) + (rx + fu - 2 - wx0));
+        float F0 = 0.0f, gx0 = 0.0f, gy0 = 0.0f;
+        if (r0 < S) ref_at(o, F0, gx0, gy0);
 #pragma unroll 1
         for (int r = r0; r < S; r += 2) {
+          const lds_float *t4 = t + 6 * wpitch, *t5 = t + 7 * wpitch;
+          t4 = t4 > tlast ? tlast : t4;
+          t5 = t5 > tlast ? tlast : t5;
+          float a[6], b[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            a[k] = t4[k];
+            b[k] = t5[k];
+          }
+          float F1 = 0.0f, gx1 = 0.0f, gy1 = 0.0f;
+          if (r + 2 < S) ref_at(o + ostep, F1, gx1, gy1);
           float vv = h0 * wyu[0];
           vv = fmaf(h1, wyu[1], vv);
           vv = fmaf(h2, wyu[2], vv);
           vv = fmaf(h3, wyu[3], vv);
           vv = fmaf(h4, wyu[4], vv);
           vv = fmaf(h5, wyu[5], vv);
-          pixel(o, vv, e);
+          unc_terms(u, v, gx0, gy0, vv - F0, *e - F0, pt);
+          F0 = F1;
+          gx0 = gx1;
+          gy0 = gy1;
           o += ostep;
           e += tstep;
           t += tstep;
@@ -1691,18 +1717,8 @@ struct Solver {
           h1 = h3;
           h2 = h4;
           h3 = h5;
-          if (r + 2 < S) {
-            h4 = rowsum(t + 4 * wpitch);
-            h5 = rowsum(t + 5 * wpitch);
-          }
-        }
-      } else {
-#pragma unroll 1
-        for (int r = r0; r < S; r += step) {
-          pixel(o, keys_sum(t, wpitch, wxu, wyu), e);
-          o += ostep;
-          e += tstep;
-          t += tstep;
+          h4 = rowsum(a);
+          h5 = rowsum(b);
         }
       }
     };
@@ -1711,9 +1727,24 @@ struct Solver {
     for (int st = 0; st < nfull; ++st) column(x0 + (st << 5) + (lane & 31), lane >> 5, 2);
     const int R = S & 31;
     if (R) {
+      // the last columns R at a time, 64 / R rows per step (row-major, so that
+      // the reference loads of a half-wave stay in few cache lines)
       const int rows = kWave / R;
       const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
-      if (lr < rows) column(x0 + (nfull << 5) + lc, lr, rows);
+      if (lr < rows) {
+        const int rx = x0 + (nfull << 5) + lc - ref.ox, ry0 = y0 + lr - ref.oy;
+        int o = ry0 * ref.w + rx;
+        const lds_float *t = win + (__umul24(ry0 + fv - 2 - wy0, wpitch) + (rx + fu - 2 - wx0));
+        const lds_float *e = win + (__umul24(ry0 - wy0, wpitch) + (rx - wx0));
+        const int ostep = rows * ref.w, tstep = rows * wpitch;
+#pragma unroll 1
+        for (int r = lr; r < S; r += rows) {
+          pixel(o, keys_sum(t, wpitch, wxu, wyu), e);
+          o += ostep;
+          e += tstep;
+          t += tstep;
+        }
+      }
     }
     unc_record(pt, u, v, fields, nsub, gid);
     return true;
@@ -2270,7 +2301,7 @@ struct Solver {
         auto column = [&](int c, int r0, int step) {
           const lds_float *t = win + (__umul24(ly0 + r0 - wy0, wpitch) + (lx0 + c - wx0));
           const int tstep = step * wpitch;
-          if (step == 2) {
+          {
             float cm = rowf(t - wpitch), c0 = rowf(t), cp = rowf(t + wpitch);
 #pragma unroll 1
             for (int r = r0; r < bh; r += 2) {
@@ -2282,12 +2313,6 @@ struct Solver {
                 cp = rowf(t + wpitch);
               }
             }
-          } else {
-#pragma unroll 1
-            for (int r = r0; r < bh; r += step) {
-              part += fabsf((rowf(t - wpitch) + rowf(t + wpitch)) - 2.0f * rowf(t));
-              t += tstep;
-            }
           }
         };
         const int nfull = bw >> 5;
@@ -2295,9 +2320,25 @@ struct Solver {
         for (int st = 0; st < nfull; ++st) column((st << 5) + (lane & 31), lane >> 5, 2);
         const int R = bw & 31;
         if (R) {
-          const int rows = kWave / R;
-          const int lr = (int)(((float)lane + 0.5f) / (float)R), lc = lane - lr * R;
-          if (lr < rows) column((nfull << 5) + lc, lr, rows);
+          // the last columns column-major over the lanes (consecutive rows: a
+          // window pitch apart, distinct banks for an odd pitch)
+          int c = (int)(((float)lane + 0.5f) / (float)bh);
+          int r = lane - c * bh;
+          const int dc = kWave / bh, dr = kWave - dc * bh;
+          const lds_float *t = win + (__umul24(ly0 + r - wy0, wpitch) + (lx0 + (nfull << 5) + c - wx0));
+          const int tstep = dc + dr * wpitch, twrap = 1 - bh * wpitch;
+#pragma unroll 1
+          for (int k = lane; k < R * bh; k += kWave) {
+            part += fabsf((rowf(t - wpitch) + rowf(t + wpitch)) - 2.0f * rowf(t));
+            c += dc;
+            r += dr;
+            t += tstep;
+            if (r >= bh) {
+              r -= bh;
+              ++c;
+              t += twrap;
+            }
+          }
         }
         acc[0] = part;
         grp.sum(acc);
