@@ -3193,7 +3193,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
     const int lo = (int)((long long)total * range / kWorkRanges);
     const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
     int k = 0;
-    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    if (lane == 0) k = atomicAdd(pa.counter + range * kCounterStride, 1);
     k = __builtin_amdgcn_readfirstlane(k) + lo;
     if (k >= hi) {
       range = range + 1 == kWorkRanges ? 0 : range + 1;
@@ -3225,7 +3225,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
     const int lo = (int)((long long)total * range / kWorkRanges);
     const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
     int k = 0;
-    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    if (lane == 0) k = atomicAdd(pa.counter + range * kCounterStride, 1);
     k = __builtin_amdgcn_readfirstlane(k) + lo;
     if (k >= hi) {
       range = range + 1 == kWorkRanges ? 0 : range + 1;
@@ -3276,7 +3276,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
     const int lo = (int)((long long)total * range / kWorkRanges);
     const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
     int k = 0;
-    if (lane == 0) k = atomicAdd(pa.counter + range, 1);
+    if (lane == 0) k = atomicAdd(pa.counter + range * kCounterStride, 1);
     k = __builtin_amdgcn_readfirstlane(k) + lo;
     if (k >= hi) {
       range = range + 1 == kWorkRanges ? 0 : range + 1;

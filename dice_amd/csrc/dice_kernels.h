@@ -142,7 +142,7 @@ struct PassArgs {
   const int *item_count;
   int *fb_list, *fb_count;
   int *ov_list, *ov_count;
-  int *counter;                // BATCHED work counters, kWorkRanges ints (zeroed before the launch)
+  int *counter;                // BATCHED work counters, kWorkRanges of them kCounterStride ints apart (zeroed before the launch)
   const float *nm_result;      // Nelder-Mead passes: per-subset results of simplex_lanes_kernel, or null
 };
 
@@ -156,16 +156,22 @@ struct GammaArgs {
   int adjoint;
 };
 
-// BATCHED work ranges (one per XCD of the MI355X).
+// BATCHED work ranges (one per XCD of the MI355X).  Each range counter sits on
+// a 128-byte line of its own: device-scope atomics on one line are served one
+// after another, so eight counters on one line would serialize every work
+// fetch of the chip (one per subset).
 constexpr int kWorkRanges = 8;
-// Header of the work area: the range counters, then the fallback count and the
-// overflow counts of pass 0 and pass 1.
-constexpr int kWorkHeader = 16;
-constexpr int kWorkFbCount = kWorkRanges, kWorkOv0Count = kWorkRanges + 1, kWorkOv1Count = kWorkRanges + 2;
-constexpr int kWorkLaneCount = kWorkRanges + 3;  // work counter of simplex_lanes_kernel
+constexpr int kCounterStride = 32;  // ints
+constexpr int kWorkCounterInts = kWorkRanges * kCounterStride;
+// Header of the work area: the range counters, then (on the next line) the
+// fallback count and the overflow counts of pass 0 and pass 1.
+constexpr int kWorkHeader = kWorkCounterInts + kCounterStride;
+constexpr int kWorkFbCount = kWorkCounterInts, kWorkOv0Count = kWorkCounterInts + 1,
+              kWorkOv1Count = kWorkCounterInts + 2;
+constexpr int kWorkLaneCount = kWorkCounterInts + 3;  // work counter of simplex_lanes_kernel
 // count of the lean pass's hand-over list, which lives in the storage of the pass-1
 // overflow list (pass 1 starts after pass 0 has finished with it)
-constexpr int kWorkLeanCount = kWorkRanges + 4;
+constexpr int kWorkLeanCount = kWorkCounterInts + 4;
 
 // Device ints of the work area of launch_correlate: the header, the fallback
 // and overflow lists (n each), the Nelder-Mead results (kNmStride per subset).

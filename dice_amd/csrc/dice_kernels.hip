@@ -681,7 +681,7 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
   if (!simplex && pa.pass == 0 && subs.S > 0 && !subs.mbox && !kp.def_grads &&
       kp.momentum == 0.0f && kp.init_gamma_thr == -1.0 && !kp.has_forced && kp.en_rot && kp.en_normal &&
       kp.en_shear && kp.lm_alpha <= 0.0f && lean_list) {
-    e = launch_zero_ints(pa.counter, kWorkRanges, stream);
+    e = launch_zero_ints(pa.counter, kWorkCounterInts, stream);
     if (e != hipSuccess) return e;
     PassArgs pl = pa;
     pl.ov_list = lean_list;
@@ -699,7 +699,7 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
     if (e != hipSuccess) return e;
     pa.nm_result = nm;
   }
-  e = launch_zero_ints(pa.counter, kWorkRanges, stream);
+  e = launch_zero_ints(pa.counter, kWorkCounterInts, stream);
   if (e != hipSuccess) return e;
   pa.ov_list = ov_list;
   pa.ov_count = ov_count;
