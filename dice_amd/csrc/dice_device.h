@@ -202,6 +202,15 @@ __device__ __forceinline__ void keys_weights(float t, float w[6]) {
   w[5] = fmaf(t, -1.0f / 12.0f, 1.0f / 12.0f) * t2;
 }
 
+// A wave-uniform double as a scalar value (both halves through readfirstlane),
+// so that it takes SGPRs rather than two VGPRs.
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // d/dt of keys_weights (the slopes of the six cubics).
 __device__ __forceinline__ void keys_weight_slopes(float t, float w[6]) {
   const float t2 = t * t;
@@ -811,7 +820,8 @@ struct Solver {
   // Stage the deformed-image window [x0, x1] x [y0, y1] (local coordinates,
   // clamped to the image) into the wave's LDS slot.  Returns false (and marks
   // the subset as overflowing) when it does not fit.
-  __device__ __forceinline__ bool stage(int x0, int x1, int y0, int y1, bool hand_over = true) {
+  // ring false: no ring (a window read for one evaluation only).
+  __device__ __forceinline__ bool stage(int x0, int x1, int y0, int y1, bool hand_over = true, bool ring = true) {
     x0 = max(x0, 0);
     y0 = max(y0, 0);
     x1 = min(x1, def.w - 1);
@@ -834,7 +844,7 @@ struct Solver {
     // image), so that the windows of the next steps of this subset, which move
     // by a fraction of a pixel, are usually inside this one
     {
-      int e = 2;
+      int e = ring ? 2 : 0;
       while (e > 0 && (ww + 2 * e) * (hh + 2 * e) > win_cap) --e;
       x0 = max(x0 - e, 0);
       y0 = max(y0 - e, 0);
@@ -906,9 +916,10 @@ struct Solver {
 
   // Window covering the warped subset bbox (+ interpolation support and a
   // 1-px rounding margin) at parameters A.
-  __device__ __forceinline__ bool stage_affine(const Affine &A) {
+  __device__ __forceinline__ bool stage_affine(const Affine &A, bool ring = true) {
     if constexpr (!kIsWave) {
       (void)A;
+      (void)ring;
       return true;
     } else {
       // the map is affine, so every pixel's position lies between the corner
@@ -918,7 +929,8 @@ struct Solver {
       const int lead = INTERP == 0 ? 0 : 2, trail = INTERP == 0 ? 1 : 3;
       const float m = 1e-3f;
       return stage(cx - def.ox + (int)floorf(lo_x - m) - lead, cx - def.ox + (int)floorf(hi_x + m) + trail,
-                   cy - def.oy + (int)floorf(lo_y - m) - lead, cy - def.oy + (int)floorf(hi_y + m) + trail);
+                   cy - def.oy + (int)floorf(lo_y - m) - lead, cy - def.oy + (int)floorf(hi_y + m) + trail, true,
+                   ring);
     }
   }
 
@@ -1300,6 +1312,10 @@ struct Solver {
       acc[5] += d * r;
     });
     grp.sum(acc);
+    return zstats_from(acc, z);
+  }
+  // The ZNSSD terms from the sums (n, sd, sdd, sr, srr, sdr) over the kept pixels.
+  __device__ __forceinline__ static bool zstats_from(const double (&acc)[6], ZStats &z) {
     z.n = acc[0];
     if (z.n <= 0.0) return false;
     z.md = acc[1] / z.n;
@@ -1516,6 +1532,237 @@ struct Solver {
       }
     }
     return true;
+  }
+
+  // Square subsets of at most kGammaPixels pixels with no clamped pixel,
+  // inside the reference image and without blocked pixels (BATCHED waves).
+  // The slot holds a table of the subset's reference values (row-major pixel
+  // order; direct global -> LDS loads issued before the window's, so one wait
+  // covers both) and the window after it (no ring: one evaluation), so that
+  // no pixel pass waits on a global load.  Same sums in the same order as
+  // gamma_at / gamma_adjoint.  The adjoint's second pass evaluates each pixel
+  // once more with the slopes and adds its dL/dI_def taps with the same
+  // weights, walking the (x mod 2, y mod 2) classes so that the taps can be
+  // plain read-modify-writes, into an LDS accumulator after the window whose
+  // rows hold the even window columns, then the odd ones (column c at c / 2,
+  // or half + c / 2 when odd): lanes two columns apart then touch consecutive
+  // words.
+  static constexpr int kGammaPixels = 512;
+  __device__ __forceinline__ bool gamma_square_ok() const {
+    return kIsWave && S > 0 && n <= kGammaPixels && mw == 0 && ref_in && cx - hw >= 0 && cy - hw >= 0 &&
+           kGammaPixels + 2 * kWave <= win_cap;
+  }
+
+  template <bool ADJ>
+  __device__ __forceinline__ bool gamma_square(const float (&p)[6], double dl_dgamma, double &gamma_out,
+                                               double (&dp)[6], float *dimg) {
+    const int lane = grp.rank();
+    const int x0 = cx - hw, y0 = cy - hw;
+    // reference table at the slot's start, the window after it
+    lds_float *const rtab = win;
+    const int rpad = (n + kWave - 1) & ~(kWave - 1);
+    {
+      const float inv = 1.0f / (float)S;  // (k + 0.5) / S: the integer part of k / S for k < 2^9, S < 2^6
+      const unsigned o0 = (unsigned)((y0 - ref.oy) * ref.w + (x0 - ref.ox));
+#pragma unroll 1
+      for (int k0 = 0; k0 < n; k0 += kWave) {
+        const int k = k0 + lane;
+        if (k < n) {
+          const int r = (int)(((float)k + 0.5f) * inv), c = k - r * S;
+          const unsigned o = o0 + (unsigned)(r * ref.w + c);
+          if constexpr (std::is_same<T, float>::value)
+            __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(Pix<T>::ptr(ref.pix) + o),
+                                             (__attribute__((address_space(3))) void *)(rtab + k0), 4, 0, 0);
+          else
+            rtab[k] = Pix<T>::at_u(ref.pix, o);
+        }
+      }
+    }
+    win += rpad;
+    win_cap -= rpad;
+    Affine A;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A.p[k] = p[k];
+    A.prepare();
+    const bool staged = stage_affine(A, false);  // its wait covers the table's loads
+    if (staged) {
+      // (k + 0.5) / S in float has the integer part of k / S for k < 2^8 and S < 2^12
+      const float inv = 1.0f / (float)S;
+      const int drow = (int)(((float)kWave + 0.5f) * inv), dcol = kWave - drow * S;
+      const int row0 = (int)(((float)lane + 0.5f) * inv), col0 = lane - row0 * S;
+      double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+      {
+        int row = row0, col = col0;
+#pragma unroll 1
+        for (int i = lane; i < n; i += kWave) {
+          float ox, oy;
+          A.disp((float)(col - hw), (float)(row - hw), ox, oy);
+          if (kept(ox, oy)) {
+            const double d = sample(ox, oy), r = rtab[i];
+            acc[0] += 1.0;
+            acc[1] += d;
+            acc[2] += d * d;
+            acc[3] += r;
+            acc[4] += r * r;
+            acc[5] += d * r;
+          }
+          col += dcol;
+          row += drow;
+          if (col >= S) { col -= S; ++row; }
+        }
+      }
+      grp.sum(acc);
+      ZStats z;
+      const bool ok = zstats_from(acc, z);
+      gamma_out = ok ? 2.0 - 2.0 * z.rho : -1.0;
+      if constexpr (ADJ) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dp[k] = 0.0;
+        if (ok && dl_dgamma != 0.0) {
+          // dgamma/dd_j = (2 / nd) (rho (d_j - dbar) / nd - (r_j - rbar) / nr)
+          const double s = dl_dgamma * 2.0 / z.nd;
+          const double a = wave_uniform(s * z.rho / z.nd), b = wave_uniform(s / z.nr);
+          const double md = wave_uniform(z.md), mr = wave_uniform(z.mr);
+          // the split accumulator after the window
+          const int cells = wpitch * wrows, half = (wpitch + 1) >> 1, apitch = 2 * half;
+          const int acells = apitch * wrows;
+          const bool split = dimg && wrows > 0 && cells + acells <= win_cap;
+          lds_float *const accl = win + cells;
+          if (split) {
+            for (int k = lane; k < acells; k += kWave) accl[k] = 0.0f;
+            grp.sync();
+          }
+          double g6[6] = {0, 0, 0, 0, 0, 0};
+          // one kept pixel (subset offsets x, y; table index i): the
+          // interpolant and its slopes, dgamma/dd, the dL/dp terms, and with
+          // MODE 1 / 2 the taps of dL/dI_def into the split accumulator with
+          // the same weights (1: plain read-modify-writes, 2: LDS atomics), with
+          // MODE 3 into the image (global atomics)
+          auto pixel = [&](int x, int y, int i, auto mode) {
+            constexpr int MODE = decltype(mode)::value;
+            const float dx = (float)(x - hw), dy = (float)(y - hw);
+            float ox, oy;
+            A.disp(dx, dy, ox, oy);
+            if (!kept(ox, oy)) return;
+            const float flx = floorf(ox), fly = floorf(oy);
+            const float tx = ox - flx, ty = oy - fly;
+            constexpr int lead = INTERP == 0 ? 0 : 2, taps = INTERP == 0 ? 2 : 6;
+            const int c0 = cx - def.ox + (int)flx - lead - wx0, r0 = cy - def.oy + (int)fly - lead - wy0;
+            const lds_float *w0 = win + (__umul24(r0, wpitch) + c0);
+            float wx[6], wy[6], d, Ix, Iy;
+            if (INTERP == 0) {
+              const float b00 = w0[0], b10 = w0[1], b01 = w0[wpitch], b11 = w0[wpitch + 1];
+              Ix = (1.0f - ty) * (b10 - b00) + ty * (b11 - b01);
+              Iy = (1.0f - tx) * (b01 - b00) + tx * (b11 - b10);
+              d = b00 * (1.0f - tx) * (1.0f - ty) + b10 * tx * (1.0f - ty) + b11 * tx * ty + b01 * (1.0f - tx) * ty;
+              wx[0] = 1.0f - tx; wx[1] = tx; wy[0] = 1.0f - ty; wy[1] = ty;
+            } else {
+              // the sums of sample_d
+              float dwx[6], dwy[6];
+              keys_weights(tx, wx);
+              keys_weights(ty, wy);
+              keys_weight_slopes(tx, dwx);
+              keys_weight_slopes(ty, dwy);
+              float v = 0.0f, vx = 0.0f, vy = 0.0f;
+#pragma unroll
+              for (int m = 0; m < 6; ++m) {
+                float rs = 0.0f, rd = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                  const float t = w0[m * wpitch + k];
+                  rs = fmaf(t, wx[k], rs);
+                  rd = fmaf(t, dwx[k], rd);
+                }
+                v = fmaf(rs, wy[m], v);
+                vx = fmaf(rd, wy[m], vx);
+                vy = fmaf(rs, dwy[m], vy);
+              }
+              d = v;
+              Ix = vx;
+              Iy = vy;
+            }
+            const double gj = a * ((double)d - md) - b * ((double)rtab[i] - mr);
+            float rrow[6];
+            A.residual(dx, dy, Ix, Iy, rrow);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) g6[k] = fma(gj, (double)rrow[k], g6[k]);
+            const float g = (float)gj;
+            if constexpr (MODE == 3) {
+              scatter_taps(ox, oy, g, nullptr, dimg);
+            } else if constexpr (MODE != 0) {
+              int coff[taps];
+#pragma unroll
+              for (int k = 0; k < taps; ++k) coff[k] = ((c0 + k) >> 1) + ((c0 + k) & 1) * half;
+              lds_float *rowp = accl + __umul24(r0, apitch);
+#pragma unroll
+              for (int m = 0; m < taps; ++m) {
+                const float gm = g * wy[m];
+#pragma unroll
+                for (int k = 0; k < taps; ++k) {
+                  const float c = gm * wx[k];
+                  if constexpr (MODE == 1) {
+                    rowp[coff[k]] += c;
+                    asm volatile("" ::: "memory");
+                  } else {
+                    __hip_atomic_fetch_add(rowp + coff[k], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                  }
+                }
+                rowp += apitch;
+              }
+            }
+          };
+          auto rows = [&](auto mode) {
+            int row = row0, col = col0;
+#pragma unroll 1
+            for (int i = lane; i < n; i += kWave) {
+              pixel(col, row, i, mode);
+              col += dcol;
+              row += drow;
+              if (col >= S) { col -= S; ++row; }
+            }
+          };
+          // (x mod 2, y mod 2) classes as in gamma_adjoint: with the warp within
+          // ||M - I|| <= 0.45, two samples of a class never share a cell
+          auto classes = [&](auto mode) {
+#pragma unroll 1
+            for (int ph = 0; ph < 4; ++ph) {
+              const int pa = ph >> 1, pb = ph & 1;
+              const int nc = (S - pb + 1) >> 1, cnt = ((S - pa + 1) >> 1) * nc;
+              const float invc = 1.0f / (float)nc;
+#pragma unroll 1
+              for (int k = lane; k < cnt; k += kWave) {
+                const int rr = (int)(((float)k + 0.5f) * invc);  // exact: k < 2^20
+                const int y = pa + 2 * rr, x = pb + 2 * (k - rr * nc);
+                pixel(x, y, y * S + x, mode);
+              }
+            }
+          };
+          const float mi = fmaxf(fabsf(A.a11 - 1.0f) + fabsf(A.a12), fabsf(A.a21) + fabsf(A.a22 - 1.0f));
+          if (!dimg) rows(std::integral_constant<int, 0>{});
+          else if (!split) rows(std::integral_constant<int, 3>{});
+          else if (mi <= 0.45f) classes(std::integral_constant<int, 1>{});
+          else classes(std::integral_constant<int, 2>{});
+          grp.sum(g6);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) dp[k] = g6[k];
+          if (split) {
+            grp.sync();
+            const float invw = 1.0f / (float)wpitch;
+            for (int k = lane; k < cells; k += kWave) {
+              const int r = (int)(((float)k + 0.5f) * invw);
+              const int c = k - r * wpitch;
+              const float v = accl[r * apitch + (c >> 1) + (c & 1) * half];
+              if (v != 0.0f) unsafeAtomicAdd(dimg + (size_t)(wy0 + r) * def.w + (wx0 + c), v);
+            }
+          }
+        }
+      }
+      grp.sync();
+    }
+    win -= rpad;
+    win_cap += rpad;
+    win_ok = false;
+    return staged;
   }
 
   // Objective_ZNSSD::computeUpdateFast.  Returns the solver status (or
@@ -2484,9 +2731,22 @@ struct Solver {
     float p[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) p[k] = (float)ga.params[(size_t)gid * 6 + k];
+    // the register-resident path of square subsets (BATCHED waves), else the general one
+    auto square = [&](auto adj, double w, double &g, double (&dp)[6], bool &ok) {
+      if constexpr (kIsWave) {
+        if (gamma_square_ok()) {
+          ok = gamma_square<decltype(adj)::value>(p, w, g, dp, ga.dimage);
+          return true;
+        }
+      }
+      return false;
+    };
+    bool ok;
     if (ga.adjoint) {
-      double dp[6];
-      if (!gamma_adjoint(p, ga.dgamma[gid], dp, ga.dimage)) {
+      double dp[6], g;
+      const double w = ga.dgamma[gid];
+      if (!square(std::true_type{}, w, g, dp, ok)) ok = gamma_adjoint(p, w, dp, ga.dimage);
+      if (!ok) {
         push(ov_list, ov_count, gid);
         return;
       }
@@ -2494,8 +2754,9 @@ struct Solver {
 #pragma unroll
         for (int k = 0; k < 6; ++k) ga.dparams[(size_t)gid * 6 + k] = dp[k];
     } else {
-      double g;
-      if (!gamma_at(p, g)) {
+      double g, dp[6];
+      if (!square(std::false_type{}, 0.0, g, dp, ok)) ok = gamma_at(p, g);
+      if (!ok) {
         push(ov_list, ov_count, gid);
         return;
       }
@@ -3269,21 +3530,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                         lds_per_wave);
   const int total = items.total(ss.n);
+  const int waves = (int)gridDim.x * (int)(blockDim.x / kWave);
   if ((int)blockIdx.x * (int)(blockDim.x / kWave) + wave >= total) return;  // as in correlate_batched_kernel
+  // One evaluation per item, so items are short and alike: a wave claims a
+  // chunk of up to 4 (about a quarter of its share) per atomic, and claims its
+  // next chunk before it works on the current one, so that the atomic's round
+  // trip overlaps the work.
+  const int chunk = max(1, min(4, total / (4 * waves)));
   int range = (int)(blockIdx.x % kWorkRanges);
+  auto claim = [&](int r) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(pa.counter + r * kCounterStride, chunk);
+    return k;
+  };
+  int next = claim(range);
 #pragma unroll 1
   for (int visited = 0; visited < kWorkRanges;) {
     const int lo = (int)((long long)total * range / kWorkRanges);
     const int hi = (int)((long long)total * (range + 1) / kWorkRanges);
-    int k = 0;
-    if (lane == 0) k = atomicAdd(pa.counter + range * kCounterStride, 1);
-    k = __builtin_amdgcn_readfirstlane(k) + lo;
-    if (k >= hi) {
+    const int k0 = __builtin_amdgcn_readfirstlane(next) + lo;
+    if (k0 >= hi) {
       range = range + 1 == kWorkRanges ? 0 : range + 1;
-      ++visited;
+      if (++visited < kWorkRanges) next = claim(range);
       continue;
     }
-    s.run_gamma(ss, items.at(k), ga, pa.ov_list, pa.ov_count);
+    next = claim(range);
+    const int k1 = min(k0 + chunk, hi);
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) s.run_gamma(ss, items.at(k), ga, pa.ov_list, pa.ov_count);
   }
 }
 
