@@ -211,6 +211,10 @@ __device__ __forceinline__ double wave_uniform(double v) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+__device__ __forceinline__ float wave_uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // d/dt of keys_weights (the slopes of the six cubics).
 __device__ __forceinline__ void keys_weight_slopes(float t, float w[6]) {
   const float t2 = t * t;
@@ -1539,18 +1543,18 @@ struct Solver {
   // The slot holds a table of the subset's reference values (row-major pixel
   // order; direct global -> LDS loads issued before the window's, so one wait
   // covers both) and the window after it (no ring: one evaluation), so that
-  // no pixel pass waits on a global load.  Same sums in the same order as
-  // gamma_at / gamma_adjoint.  The adjoint's second pass evaluates each pixel
-  // once more with the slopes and adds its dL/dI_def taps with the same
-  // weights, walking the (x mod 2, y mod 2) classes so that the taps can be
-  // plain read-modify-writes, into an LDS accumulator after the window whose
-  // rows hold the even window columns, then the odd ones (column c at c / 2,
-  // or half + c / 2 when odd): lanes two columns apart then touch consecutive
-  // words.
+  // no pixel pass waits on a global load.  The gamma sums are those of
+  // gamma_at in the same order.  The adjoint evaluates the interpolant with
+  // its slopes in the same single pass and sums (d - c) row, (r - c) row and
+  // row per lane in FP64 (c: a reference value, a shift), from which dL/dp
+  // follows once the means are known; the interpolated values go to a second
+  // table.  dL/dI_def then walks the (x mod 2, y mod 2) classes, so that the
+  // taps can be plain read-modify-writes, into an LDS accumulator over the
+  // window's space (the window's layout: the taps' offsets are immediates).
   static constexpr int kGammaPixels = 512;
   __device__ __forceinline__ bool gamma_square_ok() const {
     return kIsWave && S > 0 && n <= kGammaPixels && mw == 0 && ref_in && cx - hw >= 0 && cy - hw >= 0 &&
-           kGammaPixels + 2 * kWave <= win_cap;
+           2 * kGammaPixels + 2 * kWave <= win_cap;
   }
 
   template <bool ADJ>
@@ -1558,9 +1562,12 @@ struct Solver {
                                                double (&dp)[6], float *dimg) {
     const int lane = grp.rank();
     const int x0 = cx - hw, y0 = cy - hw;
-    // reference table at the slot's start, the window after it
+    // reference table at the slot's start, (adjoint) the table of interpolated
+    // values, then the window
     lds_float *const rtab = win;
     const int rpad = (n + kWave - 1) & ~(kWave - 1);
+    lds_float *const dtab = rtab + rpad;
+    const int tabs = ADJ ? 2 * rpad : rpad;
     {
       const float inv = 1.0f / (float)S;  // (k + 0.5) / S: the integer part of k / S for k < 2^9, S < 2^6
       const unsigned o0 = (unsigned)((y0 - ref.oy) * ref.w + (x0 - ref.ox));
@@ -1578,8 +1585,8 @@ struct Solver {
         }
       }
     }
-    win += rpad;
-    win_cap -= rpad;
+    win += tabs;
+    win_cap -= tabs;
     Affine A;
 #pragma unroll
     for (int k = 0; k < 6; ++k) A.p[k] = p[k];
@@ -1590,15 +1597,43 @@ struct Solver {
       const float inv = 1.0f / (float)S;
       const int drow = (int)(((float)kWave + 0.5f) * inv), dcol = kWave - drow * S;
       const int row0 = (int)(((float)lane + 0.5f) * inv), col0 = lane - row0 * S;
+      // every pixel passes kept()'s guard (no blocked pixels here): skip the test
+      const bool allk = all_kept(A);
       double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+      // adjoint: sum((d - c) row), sum((r - c) row), sum(row) over the kept
+      // pixels, c = the first reference value (a shift that keeps the terms
+      // small); dL/dp = a (sum((d - dbar) row)) - b (sum((r - rbar) row))
+      double sd6[6] = {0, 0, 0, 0, 0, 0}, sr6[6] = {0, 0, 0, 0, 0, 0}, s16[6] = {0, 0, 0, 0, 0, 0};
+      const double c = ADJ ? (double)wave_uniform_f(rtab[0]) : 0.0;
       {
         int row = row0, col = col0;
 #pragma unroll 1
         for (int i = lane; i < n; i += kWave) {
+          const float dx = (float)(col - hw), dy = (float)(row - hw);
           float ox, oy;
-          A.disp((float)(col - hw), (float)(row - hw), ox, oy);
-          if (kept(ox, oy)) {
-            const double d = sample(ox, oy), r = rtab[i];
+          A.disp(dx, dy, ox, oy);
+          if (allk || kept(ox, oy)) {
+            const float rf = rtab[i];
+            const double r = rf;
+            double d;
+            if constexpr (ADJ) {
+              float Ix, Iy;
+              const float df = sample_d(ox, oy, Ix, Iy);
+              dtab[i] = df;
+              d = df;
+              float rrow[6];
+              A.residual(dx, dy, Ix, Iy, rrow);
+              const double ed = d - c, er = r - c;
+#pragma unroll
+              for (int k = 0; k < 6; ++k) {
+                const double rk = rrow[k];
+                sd6[k] = fma(ed, rk, sd6[k]);
+                sr6[k] = fma(er, rk, sr6[k]);
+                s16[k] += rk;
+              }
+            } else {
+              d = sample(ox, oy);
+            }
             acc[0] += 1.0;
             acc[1] += d;
             acc[2] += d * d;
@@ -1623,144 +1658,109 @@ struct Solver {
           const double s = dl_dgamma * 2.0 / z.nd;
           const double a = wave_uniform(s * z.rho / z.nd), b = wave_uniform(s / z.nr);
           const double md = wave_uniform(z.md), mr = wave_uniform(z.mr);
-          // the split accumulator after the window
-          const int cells = wpitch * wrows, half = (wpitch + 1) >> 1, apitch = 2 * half;
-          const int acells = apitch * wrows;
-          const bool split = dimg && wrows > 0 && cells + acells <= win_cap;
-          lds_float *const accl = win + cells;
-          if (split) {
-            for (int k = lane; k < acells; k += kWave) accl[k] = 0.0f;
-            grp.sync();
-          }
-          double g6[6] = {0, 0, 0, 0, 0, 0};
-          // one kept pixel (subset offsets x, y; table index i): the
-          // interpolant and its slopes, dgamma/dd, the dL/dp terms, and with
-          // MODE 1 / 2 the taps of dL/dI_def into the split accumulator with
-          // the same weights (1: plain read-modify-writes, 2: LDS atomics), with
-          // MODE 3 into the image (global atomics)
-          auto pixel = [&](int x, int y, int i, auto mode) {
-            constexpr int MODE = decltype(mode)::value;
-            const float dx = (float)(x - hw), dy = (float)(y - hw);
-            float ox, oy;
-            A.disp(dx, dy, ox, oy);
-            if (!kept(ox, oy)) return;
-            const float flx = floorf(ox), fly = floorf(oy);
-            const float tx = ox - flx, ty = oy - fly;
-            constexpr int lead = INTERP == 0 ? 0 : 2, taps = INTERP == 0 ? 2 : 6;
-            const int c0 = cx - def.ox + (int)flx - lead - wx0, r0 = cy - def.oy + (int)fly - lead - wy0;
-            const lds_float *w0 = win + (__umul24(r0, wpitch) + c0);
-            float wx[6], wy[6], d, Ix, Iy;
-            if (INTERP == 0) {
-              const float b00 = w0[0], b10 = w0[1], b01 = w0[wpitch], b11 = w0[wpitch + 1];
-              Ix = (1.0f - ty) * (b10 - b00) + ty * (b11 - b01);
-              Iy = (1.0f - tx) * (b01 - b00) + tx * (b11 - b10);
-              d = b00 * (1.0f - tx) * (1.0f - ty) + b10 * tx * (1.0f - ty) + b11 * tx * ty + b01 * (1.0f - tx) * ty;
-              wx[0] = 1.0f - tx; wx[1] = tx; wy[0] = 1.0f - ty; wy[1] = ty;
-            } else {
-              // the sums of sample_d
-              float dwx[6], dwy[6];
-              keys_weights(tx, wx);
-              keys_weights(ty, wy);
-              keys_weight_slopes(tx, dwx);
-              keys_weight_slopes(ty, dwy);
-              float v = 0.0f, vx = 0.0f, vy = 0.0f;
+          double g6[6];
 #pragma unroll
-              for (int m = 0; m < 6; ++m) {
-                float rs = 0.0f, rd = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                  const float t = w0[m * wpitch + k];
-                  rs = fmaf(t, wx[k], rs);
-                  rd = fmaf(t, dwx[k], rd);
-                }
-                v = fmaf(rs, wy[m], v);
-                vx = fmaf(rd, wy[m], vx);
-                vy = fmaf(rs, dwy[m], vy);
-              }
-              d = v;
-              Ix = vx;
-              Iy = vy;
-            }
-            const double gj = a * ((double)d - md) - b * ((double)rtab[i] - mr);
-            float rrow[6];
-            A.residual(dx, dy, Ix, Iy, rrow);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) g6[k] = fma(gj, (double)rrow[k], g6[k]);
-            const float g = (float)gj;
-            if constexpr (MODE == 3) {
-              scatter_taps(ox, oy, g, nullptr, dimg);
-            } else if constexpr (MODE != 0) {
-              int coff[taps];
-#pragma unroll
-              for (int k = 0; k < taps; ++k) coff[k] = ((c0 + k) >> 1) + ((c0 + k) & 1) * half;
-              lds_float *rowp = accl + __umul24(r0, apitch);
-#pragma unroll
-              for (int m = 0; m < taps; ++m) {
-                const float gm = g * wy[m];
-#pragma unroll
-                for (int k = 0; k < taps; ++k) {
-                  const float c = gm * wx[k];
-                  if constexpr (MODE == 1) {
-                    rowp[coff[k]] += c;
-                    asm volatile("" ::: "memory");
-                  } else {
-                    __hip_atomic_fetch_add(rowp + coff[k], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                  }
-                }
-                rowp += apitch;
-              }
-            }
-          };
-          auto rows = [&](auto mode) {
-            int row = row0, col = col0;
-#pragma unroll 1
-            for (int i = lane; i < n; i += kWave) {
-              pixel(col, row, i, mode);
-              col += dcol;
-              row += drow;
-              if (col >= S) { col -= S; ++row; }
-            }
-          };
-          // (x mod 2, y mod 2) classes as in gamma_adjoint: with the warp within
-          // ||M - I|| <= 0.45, two samples of a class never share a cell
-          auto classes = [&](auto mode) {
-#pragma unroll 1
-            for (int ph = 0; ph < 4; ++ph) {
-              const int pa = ph >> 1, pb = ph & 1;
-              const int nc = (S - pb + 1) >> 1, cnt = ((S - pa + 1) >> 1) * nc;
-              const float invc = 1.0f / (float)nc;
-#pragma unroll 1
-              for (int k = lane; k < cnt; k += kWave) {
-                const int rr = (int)(((float)k + 0.5f) * invc);  // exact: k < 2^20
-                const int y = pa + 2 * rr, x = pb + 2 * (k - rr * nc);
-                pixel(x, y, y * S + x, mode);
-              }
-            }
-          };
-          const float mi = fmaxf(fabsf(A.a11 - 1.0f) + fabsf(A.a12), fabsf(A.a21) + fabsf(A.a22 - 1.0f));
-          if (!dimg) rows(std::integral_constant<int, 0>{});
-          else if (!split) rows(std::integral_constant<int, 3>{});
-          else if (mi <= 0.45f) classes(std::integral_constant<int, 1>{});
-          else classes(std::integral_constant<int, 2>{});
+          for (int k = 0; k < 6; ++k) g6[k] = a * (sd6[k] - (md - c) * s16[k]) - b * (sr6[k] - (mr - c) * s16[k]);
           grp.sum(g6);
 #pragma unroll
           for (int k = 0; k < 6; ++k) dp[k] = g6[k];
-          if (split) {
-            grp.sync();
-            const float invw = 1.0f / (float)wpitch;
-            for (int k = lane; k < cells; k += kWave) {
-              const int r = (int)(((float)k + 0.5f) * invw);
-              const int c = k - r * wpitch;
-              const float v = accl[r * apitch + (c >> 1) + (c & 1) * half];
-              if (v != 0.0f) unsafeAtomicAdd(dimg + (size_t)(wy0 + r) * def.w + (wx0 + c), v);
+          if (dimg) {
+            // dL/dI_def: the accumulator over the window's space (dead now)
+            const int cells = wpitch * wrows, apitch = wpitch;
+            const int acells = cells;
+            const bool in_lds = wrows > 0 && acells <= win_cap;
+            lds_float *const accl = win;
+            if (in_lds) {
+              grp.sync();
+              for (int k = lane; k < acells; k += kWave) accl[k] = 0.0f;
+              grp.sync();
+            }
+            // the taps of one kept pixel (subset offsets x, y; table index i):
+            // MODE 1 / 2 into the accumulator (1: plain read-modify-writes,
+            // 2: LDS atomics), MODE 3 into the image (global atomics)
+            auto pixel = [&](int x, int y, int i, auto mode) {
+              constexpr int MODE = decltype(mode)::value;
+              float ox, oy;
+              A.disp((float)(x - hw), (float)(y - hw), ox, oy);
+              if (!allk && !kept(ox, oy)) return;
+              const float g = (float)(a * ((double)dtab[i] - md) - b * ((double)rtab[i] - mr));
+              if constexpr (MODE == 3) {
+                scatter_taps(ox, oy, g, nullptr, dimg);
+              } else {
+                const float flx = floorf(ox), fly = floorf(oy);
+                const float tx = ox - flx, ty = oy - fly;
+                constexpr int lead = INTERP == 0 ? 0 : 2, taps = INTERP == 0 ? 2 : 6;
+                const int c0 = cx - def.ox + (int)flx - lead - wx0, r0 = cy - def.oy + (int)fly - lead - wy0;
+                float wx[6], wy[6];
+                if (INTERP == 0) {
+                  wx[0] = 1.0f - tx; wx[1] = tx; wy[0] = 1.0f - ty; wy[1] = ty;
+                } else {
+                  keys_weights(tx, wx);
+                  keys_weights(ty, wy);
+                }
+                lds_float *rowp = accl + (__umul24(r0, apitch) + c0);
+#pragma unroll
+                for (int m = 0; m < taps; ++m) {
+                  const float gm = g * wy[m];
+#pragma unroll
+                  for (int k = 0; k < taps; ++k) {
+                    const float cv = gm * wx[k];
+                    if constexpr (MODE == 1) {
+                      rowp[k] += cv;
+                      asm volatile("" ::: "memory");
+                    } else {
+                      __hip_atomic_fetch_add(rowp + k, cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                  }
+                  rowp += apitch;
+                }
+              }
+            };
+            if (!in_lds) {
+              int row = row0, col = col0;
+#pragma unroll 1
+              for (int i = lane; i < n; i += kWave) {
+                pixel(col, row, i, std::integral_constant<int, 3>{});
+                col += dcol;
+                row += drow;
+                if (col >= S) { col -= S; ++row; }
+              }
+            } else {
+              // (x mod 2, y mod 2) classes as in gamma_adjoint: with the warp
+              // within ||M - I|| <= 0.45, two samples of a class never share a cell
+              auto classes = [&](auto mode) {
+#pragma unroll 1
+                for (int ph = 0; ph < 4; ++ph) {
+                  const int pa = ph >> 1, pb = ph & 1;
+                  const int nc = (S - pb + 1) >> 1, cnt = ((S - pa + 1) >> 1) * nc;
+                  const float invc = 1.0f / (float)nc;
+#pragma unroll 1
+                  for (int k = lane; k < cnt; k += kWave) {
+                    const int rr = (int)(((float)k + 0.5f) * invc);  // exact: k < 2^20
+                    const int y = pa + 2 * rr, x = pb + 2 * (k - rr * nc);
+                    pixel(x, y, y * S + x, mode);
+                  }
+                }
+              };
+              const float mi = fmaxf(fabsf(A.a11 - 1.0f) + fabsf(A.a12), fabsf(A.a21) + fabsf(A.a22 - 1.0f));
+              if (mi <= 0.45f) classes(std::integral_constant<int, 1>{});
+              else classes(std::integral_constant<int, 2>{});
+              grp.sync();
+              const float invw = 1.0f / (float)wpitch;
+              for (int k = lane; k < cells; k += kWave) {
+                const int r = (int)(((float)k + 0.5f) * invw);
+                const int cc = k - r * wpitch;
+                const float v = accl[k];
+                if (v != 0.0f) unsafeAtomicAdd(dimg + (size_t)(wy0 + r) * def.w + (wx0 + cc), v);
+              }
             }
           }
         }
       }
       grp.sync();
     }
-    win -= rpad;
-    win_cap += rpad;
+    win -= tabs;
+    win_cap += tabs;
     win_ok = false;
     return staged;
   }
