@@ -286,7 +286,7 @@ Launch current_launch(dice_hip_context *ctx, int mode) {
   }
   const int max_groups = (ctx->n + L.waves_per_group - 1) / L.waves_per_group;
   if (L.groups > max_groups) L.groups = max_groups > 0 ? max_groups : 1;
-  L.lane_simplex = ctx->max_count <= dice_hip::kLaneSimplexMaxPixels ? 1 : 0;
+  L.lane_simplex = ctx->max_count <= dice_hip::kLaneSimplexMaxPixels ? std::max(ctx->max_count, 1) : 0;
   return L;
 }
 

@@ -72,13 +72,20 @@ __device__ __forceinline__ double uniform(double v) {
   const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
   return __hiloint2double(hi, lo);
 }
-// Sum over the wave.  Each step adds a value and its partner's, so all lanes of
-// a row hold bit-identical row sums; the result is uniform.
-__device__ __forceinline__ double wave_sum(double v) {
+// Sum over a row of 16 lanes (every lane of the row active).  Each step adds a
+// value and its partner's, so all lanes of the row hold bit-identical sums: the
+// pairwise tree (((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7))) + (same
+// for v8..v15).
+__device__ __forceinline__ double row_sum16(double v) {
   v += dpp<kDppXor1>(v);
   v += dpp<kDppXor2>(v);
   v += dpp<kDppHalfMirror>(v);
   v += dpp<kDppMirror>(v);
+  return v;
+}
+// Sum over the wave: the four row sums added by v_readlane; the result is uniform.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = row_sum16(v);
   return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ __forceinline__ int wave_min(int v) {
@@ -119,6 +126,12 @@ struct WaveGroup {
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
   }
+  // rank 0's values to every rank
+  template <int K>
+  __device__ __forceinline__ void bcast0(double (&v)[K]) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = lane_value(v[k], 0);
+  }
   __device__ __forceinline__ int imin(int v) const { return wave_min(v); }
   __device__ __forceinline__ int imax(int v) const { return wave_max(v); }
 };
@@ -143,6 +156,17 @@ struct BlockGroup {
 #pragma unroll
     for (int k = 0; k < K; ++k)
       v[k] = (red[k] + red[kMaxRed + k]) + (red[2 * kMaxRed + k] + red[3 * kMaxRed + k]);
+    __syncthreads();
+  }
+  template <int K>
+  __device__ __forceinline__ void bcast0(double (&v)[K]) const {
+    static_assert(K <= kMaxRed, "broadcast too wide");
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = red[k];
     __syncthreads();
   }
   __device__ __forceinline__ int ired(int v, bool take_min) const {
@@ -1212,6 +1236,63 @@ struct Solver {
     grp.sync();
   }
 
+  // ------------------------------- objective sums of small subsets ---
+  // Subsets of at most kLaneSimplexMaxPixels pixels sum the objective in one
+  // fixed order in every execution mode, so that the Nelder-Mead paths (which
+  // fork on rounding) agree between them: lane l < 16 of a row adds its
+  // pixels l, l + 16, l + 32, ... in that order (FP64), then the 16 partials
+  // are added by row_sum16's pairwise tree.
+  static constexpr int kSmallPerLane = kLaneSimplexMaxPixels / 16;
+  // f(k, x, y, active) for the pixels i = l + 16 k of the subset (coordinates
+  // as for_pixels gives them)
+  template <int KPER = kSmallPerLane, class F>
+  __device__ __forceinline__ void small_walk(int l, F &&f) const {
+    int row = 0, col = 0, drow = 0, dcol = 0;
+    if (S > 0) {
+      row = l / S;
+      col = l - row * S;
+      drow = 16 / S;
+      dcol = 16 - drow * S;
+    }
+#pragma unroll
+    for (int k = 0; k < KPER; ++k) {
+      const int i = l + 16 * k;
+      if (i >= n) break;
+      if (S > 0) {
+        f(k, max(cx - hw + col, 0), max(cy - hw + row, 0), true);
+        col += dcol;
+        row += drow;
+        if (col >= S) { col -= S; ++row; }
+      } else if (pk) {
+        const uint32_t w = pk[i];
+        f(k, (int)(w & 0x7fffu), (int)((w >> 15) & 0x7fffu), (w >> 30) == 0u);
+      } else {
+        f(k, max(px[i], 0), max(py[i], 0), pact ? (pact[i] != 0) : true);
+      }
+    }
+  }
+  // Lane l's partial gamma sums (n, sd, sdd, sr, srr, sdr) of the kept pixels at
+  // A; smp(ox, oy): the interpolant, refv(x, y, k): the reference value.  KPER:
+  // a bound on the pixels per lane (the subset has at most 16 KPER pixels).
+  template <int KPER = kSmallPerLane, class Smp, class Ref>
+  __device__ __forceinline__ void small_partial(const Affine &A, int l, double (&acc)[6], Smp &&smp,
+                                                Ref &&refv) const {
+    small_walk<KPER>(l, [&](int k, int x, int y, bool a) {
+      if (!a) return;
+      const double r = refv(x, y, k);
+      float ox, oy;
+      A.disp((float)(x - cx), (float)(y - cy), ox, oy);
+      if (!kept(ox, oy)) return;
+      const double d = smp(ox, oy);
+      acc[0] += 1.0;
+      acc[1] += d;
+      acc[2] += d * d;
+      acc[3] += r;
+      acc[4] += r * r;
+      acc[5] += d * r;
+    });
+  }
+
   // Objective::gamma (DICe_Objective.cpp:58-76) at parameters A: warp, then
   // the ZNSSD of Subset::gamma from FP64 sums; also active count and contrast.
   __device__ __forceinline__ float gamma_eval(const Affine &A, int &active, float &contrast) {
@@ -1219,6 +1300,19 @@ struct Solver {
     contrast = 0.0f;
     if (!stage_affine(A)) return -1.0f;
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
+    if constexpr (G::kSize >= 16) {
+      if (n <= kLaneSimplexMaxPixels) {
+        // the order of simplex_groups_kernel (small_partial), in ranks 0..15
+        if (grp.rank() < 16) {
+          small_partial(A, grp.rank(), acc, [&](float ox, float oy) { return sample(ox, oy); },
+                        [&](int x, int y, int) { return ref_value(x - ref.ox, y - ref.oy); });
+#pragma unroll
+          for (int k = 0; k < 6; ++k) acc[k] = row_sum16(acc[k]);
+        }
+        grp.bcast0(acc);
+        return gamma_of(acc, active, contrast);
+      }
+    }
     bool every = false;  // BATCHED: every active pixel is kept (no per-pixel guard test)
     if constexpr (kIsWave) every = all_kept(A);
     if (every) {
@@ -2842,8 +2936,8 @@ struct Solver {
     float p_state[6];
     int corr;
     if (SIMPLEX_OPT) {
-      if (nm) {
-        const float *r = nm + (size_t)gid * kNmStride;
+      const float *r = nm ? nm + (size_t)gid * kNmStride : nullptr;
+      if (r && __float_as_int(r[7]) != kNmNoResult) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) A.p[k] = r[k];
         iters = __float_as_int(r[6]);
@@ -3040,12 +3134,9 @@ struct Items {
   __device__ __forceinline__ int at(int k) const { return list ? list[k] : k; }
 };
 
-// ------------------------------------------- one subset per lane: simplex ---
-// Pixels per gather group of the bilinear lane objective, and the register
-// budget of simplex_lanes_kernel as waves per SIMD (0: the compiler's choice).
-#ifndef DICE_LANE_PIXEL_GROUP
-#define DICE_LANE_PIXEL_GROUP 4
-#endif
+// ------------------------------------- 16 lanes per subset: simplex ---
+// The register budget of simplex_groups_kernel as waves per SIMD (0: the
+// compiler's choice).
 #ifndef DICE_LANE_WAVES_PER_EU
 #define DICE_LANE_WAVES_PER_EU 4
 #endif
@@ -3067,117 +3158,69 @@ struct LaneGroup {
   __device__ __forceinline__ int imax(int v) const { return v; }
 };
 
-// Simplex::minimize (DICe_Simplex.cpp:60-277) of one subset per lane, written
-// as a state machine whose every step ends in exactly one objective
-// evaluation.  The lanes of a wave are each at their own stage of their own
-// subset (vertex set-up, reflection, expansion, contraction, shrink), and all
-// of them run the evaluation -- the expensive part -- together; only the cheap
-// bookkeeping between evaluations diverges.  Comparisons, update order and
-// arithmetic follow Solver::simplex.
+// Simplex::minimize (DICe_Simplex.cpp:60-277) of one subset per row of 16
+// lanes, written as a state machine whose every step ends in exactly one
+// objective evaluation.  The rows of a wave are each at their own stage of
+// their own subset (vertex set-up, reflection, expansion, contraction,
+// shrink), and all of them run the evaluation -- the expensive part --
+// together; only the cheap bookkeeping between evaluations diverges.
+// Comparisons, update order and arithmetic follow Solver::simplex.  Every lane
+// of a row holds the same scalars and computes the same values; the vertex
+// table, the vertices' objective values and the initial point live in the
+// row's LDS (nm, kNmFloats floats), written by the row's lane 0.
+constexpr int kNmFloats = 56;  // pts 7 x 6, gv 7, init 6, one spare
 template <typename T, int INTERP, bool SQUARE>
 struct LaneSimplex {
   enum : int { kVertex = 0, kReflect = 1, kExpand = 2, kContract = 3, kShrink = 4 };
   Solver<T, LaneGroup, INTERP, false> s;
   Affine A;
-  float pts[7][6], gv[7], sums[6], init[6], phi[6];  // phi: the high vertex; the low one while shrinking
+  lds_float *nm;
+  bool writer;
+  float sums[6], phi[6];  // phi: the high vertex; the low one while shrinking
   float g_lo, g_hi, g_nhi, ysave, old_rtol, gamma_new, gamma_old;
   int ilo, ihi, inhi, vk, iteration, repeats, phase;
   int nd, mpts;
 
-  __device__ __forceinline__ LaneSimplex(const KParams &kp, const DImage &r, const DImage &d)
-      : s(kp, r, d, LaneGroup{}, nullptr, 0) {
+  __device__ __forceinline__ LaneSimplex(const KParams &kp, const DImage &r, const DImage &d, lds_float *table,
+                                         bool w)
+      : s(kp, r, d, LaneGroup{}, nullptr, 0), nm(table), writer(w) {
     nd = 2 + (kp.en_rot ? 1 : 0) + (kp.en_normal ? 2 : 0) + (kp.en_shear ? 1 : 0);
     mpts = nd + 1;
   }
+  __device__ __forceinline__ float pt(int i, int j) const { return nm[i * 6 + j]; }
+  __device__ __forceinline__ void set_pt(int i, int j, float v) const {
+    if (writer) nm[i * 6 + j] = v;
+  }
+  __device__ __forceinline__ float gval(int i) const { return nm[42 + i]; }
+  __device__ __forceinline__ void set_gval(int i, float v) const {
+    if (writer) nm[42 + i] = v;
+  }
+  __device__ __forceinline__ float init(int j) const { return nm[49 + j]; }
 
   // Start subset gid: vertices p0 and p0 + delta_i e_i; the first step
   // evaluates vertex 0.
   __device__ __forceinline__ void start(const DSubsets &ss, int gid, const double *fields) {
     s.set_subset(ss, gid);
     s.init_params(A, fields, ss.n, gid);
+    if (writer) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j) init[j] = A.p[j];
+      for (int j = 0; j < 6; ++j) nm[49 + j] = A.p[j];
 #pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      gv[i] = 0.0f;
+      for (int i = 0; i < 7; ++i) nm[42 + i] = 0.0f;
+      // vertex i moves the (i-1)-th enabled parameter (reference order u, v,
+      // theta, ex, ey, gxy) by its delta; rank: enabled parameters before slot j
+      int rank = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) pts[i][j] = init[j];
-    }
-    // vertex i moves the (i-1)-th enabled parameter (reference order u, v,
-    // theta, ex, ey, gxy) by its delta; rank: enabled parameters before slot j
-    int rank = 0;
+      for (int j = 0; j < 6; ++j) {
+        const bool on = s.en[j] != 0.0f;
+        const float delta = (j < 2) ? s.kp.delta_disp : s.kp.delta_theta;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      if (s.en[j] != 0.0f) {
-#pragma unroll
-        for (int i = 1; i < 7; ++i)
-          if (i - 1 == rank) pts[i][j] += (j < 2) ? s.kp.delta_disp : s.kp.delta_theta;
-        ++rank;
+        for (int i = 0; i < 7; ++i) nm[i * 6 + j] = (on && i - 1 == rank) ? A.p[j] + delta : A.p[j];
+        if (on) ++rank;
       }
     }
     phase = kVertex;
     vk = 0;
-  }
-
-  // Objective::gamma at A.p (the sums of Solver::gamma_eval, pixels in subset
-  // order).  Pixels go in groups (four for bilinear) and every load of a group is issued
-  // before any is used: a pixel that is not kept samples a fixed interior
-  // position (4 px inside the deformed image, so every tap is in bounds) and
-  // adds nothing, so the loads carry no branches.
-  __device__ __forceinline__ float lane_gamma() {
-    // Keys: 36 taps a pixel already fill the register budget
-    constexpr int kU = INTERP == 0 ? DICE_LANE_PIXEL_GROUP : 1;
-    const int n = s.n, S = s.S;
-    const float safe_x = (float)(s.def.ox + 4 - s.cx), safe_y = (float)(s.def.oy + 4 - s.cy);
-    double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
-    int cnt = 0;
-    int row = 0, col = 0;
-    const int x0 = s.cx - s.hw, y0 = s.cy - s.hw;
-#pragma unroll 1
-    for (int i0 = 0; i0 < n; i0 += kU) {
-      float dv[kU], rv[kU];
-      bool keep[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int i = i0 + u;
-        const bool valid = i < n;
-        int x, y;
-        bool a = true;
-        if constexpr (SQUARE) {
-          x = max(x0 + col, 0);
-          y = max(y0 + row, 0);
-          if (++col == S) { col = 0; ++row; }
-        } else {
-          const int ii = valid ? i : n - 1;
-          x = max(s.px[ii], 0);
-          y = max(s.py[ii], 0);
-          if (s.pact) a = s.pact[ii] != 0;
-        }
-        float ox, oy;
-        A.disp((float)(x - s.cx), (float)(y - s.cy), ox, oy);
-        keep[u] = valid && a && s.kept(ox, oy);
-        dv[u] = s.sample(keep[u] ? ox : safe_x, keep[u] ? oy : safe_y);
-        const int rx = x - s.ref.ox, ry = y - s.ref.oy;
-        const bool rin = rx >= 0 && ry >= 0 && rx < s.ref.w && ry < s.ref.h;
-        rv[u] = Pix<T>::at(s.ref.pix, rin ? ry * s.ref.w + rx : 0);
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (keep[u]) {
-          const double d = dv[u], r = rv[u];
-          ++cnt;
-          acc[1] += d;
-          acc[2] += d * d;
-          acc[3] += r;
-          acc[4] += r * r;
-          acc[5] += d * r;
-        }
-      }
-    }
-    acc[0] = (double)cnt;
-    int active;
-    float contrast;
-    return s.gamma_of(acc, active, contrast);
   }
 
   __device__ __forceinline__ void column_sums() {
@@ -3185,7 +3228,7 @@ struct LaneSimplex {
     for (int j = 0; j < 6; ++j) {
       float acc = 0.0f;
 #pragma unroll
-      for (int i = 0; i < 7; ++i) if (i < mpts) acc += pts[i][j];
+      for (int i = 0; i < 7; ++i) if (i < mpts) acc += pt(i, j);
       sums[j] = acc;
     }
   }
@@ -3197,16 +3240,18 @@ struct LaneSimplex {
     if (iteration >= s.kp.max_robust) { iters = iteration; return CORRELATION_SUCCESSFUL; }
     if (iteration >= s.kp.max_robust - 1) { iters = iteration; return MAX_ITERATIONS_REACHED; }
     ilo = 0;
-    if (gv[0] > gv[1]) { inhi = 1; ihi = 0; } else { inhi = 0; ihi = 1; }
-    g_lo = gv[0];
-    g_hi = ihi == 0 ? gv[0] : gv[1];
-    g_nhi = inhi == 0 ? gv[0] : gv[1];
+    const float g0 = gval(0), g1 = gval(1);
+    if (g0 > g1) { inhi = 1; ihi = 0; } else { inhi = 0; ihi = 1; }
+    g_lo = g0;
+    g_hi = ihi == 0 ? g0 : g1;
+    g_nhi = inhi == 0 ? g0 : g1;
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
       if (i < mpts) {
-        if (gv[i] <= g_lo) { ilo = i; g_lo = gv[i]; }
-        if (gv[i] > g_hi) { inhi = ihi; g_nhi = g_hi; ihi = i; g_hi = gv[i]; }
-        else if (gv[i] > g_nhi && i != ihi) { inhi = i; g_nhi = gv[i]; }
+        const float gi = gval(i);
+        if (gi <= g_lo) { ilo = i; g_lo = gi; }
+        if (gi > g_hi) { inhi = ihi; g_nhi = g_hi; ihi = i; g_hi = gi; }
+        else if (gi > g_nhi && i != ihi) { inhi = i; g_nhi = gi; }
       }
     }
     const float tiny = 1.0e-10f;
@@ -3220,12 +3265,7 @@ struct LaneSimplex {
       return CORRELATION_SUCCESSFUL;
     }
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      float v = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 7; ++k) if (k == ihi) v = pts[k][j];
-      phi[j] = v;
-    }
+    for (int j = 0; j < 6; ++j) phi[j] = pt(ihi, j);
     phase = kReflect;
     return -1;
   }
@@ -3233,7 +3273,7 @@ struct LaneSimplex {
   // Consume the objective value g at A.p; set A.p to the next point to
   // evaluate (return -1) or finish (return the status; iters, and A.p = the
   // last evaluated point, are the result).  Each piece of bookkeeping has one
-  // site, so lanes in different phases pay for each once.  The extra
+  // site, so rows in different phases pay for each once.  The extra
   // evaluation at the end of an iteration (DICe_Simplex.cpp:264) is at the
   // last evaluated point, so its value is g.
   __device__ __forceinline__ int step(float g, int &iters) {
@@ -3242,21 +3282,16 @@ struct LaneSimplex {
     bool next_iter = false; // the iteration ended with evaluation g
     bool first = false;     // the vertices are done: the first iteration head
     if (phase == kVertex) {
-#pragma unroll
-      for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
+      set_gval(vk, g);
       if (vk == 0 && g < s.kp.skip_thr && g >= 0.0f) { iters = 0; return CORRELATION_SUCCESSFUL; }
-#pragma unroll
-      for (int j = 0; j < 6; ++j) A.p[j] = init[j];
       ++vk;
       if (vk < mpts) {
 #pragma unroll
-        for (int i = 0; i < 7; ++i)
-          if (i == vk) {
-#pragma unroll
-            for (int j = 0; j < 6; ++j) A.p[j] = pts[i][j];
-          }
+        for (int j = 0; j < 6; ++j) A.p[j] = pt(vk, j);
         return -1;
       }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) A.p[j] = init(j);
       column_sums();
       old_rtol = 0.0f;
       gamma_new = 0.0f;
@@ -3265,21 +3300,18 @@ struct LaneSimplex {
       iteration = 0;
       first = true;
     } else if (phase == kShrink) {
-#pragma unroll
-      for (int i = 0; i < 7; ++i) if (i == vk) gv[i] = g;
+      set_gval(vk, g);
       shrink = true;
     } else {
       // a trial point: it replaces the high vertex if it is better
       if (g < g_hi) {
+        set_gval(ihi, g);
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
-          if (k == ihi) {
-            gv[k] = g;
-#pragma unroll
-            for (int j = 0; j < 6; ++j) { sums[j] += A.p[j] - pts[k][j]; pts[k][j] = A.p[j]; }
-          }
-#pragma unroll
-        for (int j = 0; j < 6; ++j) phi[j] = A.p[j];
+        for (int j = 0; j < 6; ++j) {
+          sums[j] += A.p[j] - pt(ihi, j);
+          set_pt(ihi, j, A.p[j]);
+          phi[j] = A.p[j];
+        }
         g_hi = g;
       }
       if (phase == kReflect && g <= g_lo) {
@@ -3292,12 +3324,7 @@ struct LaneSimplex {
       } else if (phase == kContract && g >= ysave) {
         // shrink every vertex but the low one towards it (held in phi)
 #pragma unroll
-        for (int j = 0; j < 6; ++j) {
-          float v = 0.0f;
-#pragma unroll
-          for (int k = 0; k < 7; ++k) if (k == ilo) v = pts[k][j];
-          phi[j] = v;
-        }
+        for (int j = 0; j < 6; ++j) phi[j] = pt(ilo, j);
         phase = kShrink;
         vk = -1;
         shrink = true;
@@ -3313,14 +3340,11 @@ struct LaneSimplex {
       if (next < 7) {
         vk = next;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          if (i == vk) {
-#pragma unroll
-            for (int j = 0; j < 6; ++j) {
-              pts[i][j] = sums[j] = (float)(0.5 * ((double)pts[i][j] + (double)phi[j]));
-              A.p[j] = sums[j];
-            }
-          }
+        for (int j = 0; j < 6; ++j) {
+          const float v = (float)(0.5 * ((double)pt(vk, j) + (double)phi[j]));
+          set_pt(vk, j, v);
+          sums[j] = v;
+          A.p[j] = v;
         }
         return -1;
       }
@@ -3342,41 +3366,65 @@ struct LaneSimplex {
 #pragma unroll
     for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
 #pragma unroll
-    for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init[j];
+    for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init(j);
     return -1;
   }
 };
 
-// BATCHED Nelder-Mead iterations, one subset per lane, for small subsets.
-// Lanes that finish a subset write its result to nm_out and take the next
-// items: one atomic per wave and refill, the lanes numbered by their rank in
-// the ballot.  A lane whose item index is past the end stays idle, and the
-// wave exits when all of its lanes are idle: every simplex ends within
-// max_robust iterations, so every wave reaches that exit.
-template <typename T, int INTERP, bool SQUARE>
-__global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_lanes_kernel(KParams kp, DImage ref, DImage def,
-                                                                             DSubsets ss, const double *fields,
-                                                                             PassArgs pa, float *nm_out, int *counter) {
+// BATCHED Nelder-Mead iterations for small subsets, four per wave: each row
+// of 16 lanes owns one subset, runs LaneSimplex's state machine in all of its
+// lanes (the same values in each) and evaluates the objective together --
+// lane l takes the pixels l, l + 16, ... (Solver::small_partial), the row adds
+// the partials (row_sum16).  The row keeps its lanes' reference values in
+// registers and stages the deformed window of the current point, with a
+// margin of up to 3 px, into its LDS slot of `slot` floats (the last kNmFloats
+// of them hold LaneSimplex's vertex table); a point whose footprint leaves the
+// window restages it.  A point whose window does not fit the slot ends the
+// row's work on the subset with status kNmNoResult, and the pass that follows
+// runs that subset's simplex itself (in the same summation order, so with the
+// same result).  Rows that finish a subset write its result to
+// nm_out and take the next items: one atomic per wave and refill, the rows
+// numbered by their rank in the ballot.  A row whose item index is past the
+// end stays idle, and the wave exits when all of its rows are idle: every
+// simplex ends within max_robust iterations, so every wave reaches that exit.
+template <typename T, int INTERP, bool SQUARE, int KPER>
+__global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KParams kp, DImage ref, DImage def,
+                                                                              DSubsets ss, const double *fields,
+                                                                              PassArgs pa, float *nm_out,
+                                                                              int *counter, int slot) {
+  extern __shared__ float lds[];
   const Items items{pa.items, pa.item_count};
   const int total = items.total(ss.n);
-  const int lane = (int)(threadIdx.x % kWave);
-  LaneSimplex<T, INTERP, SQUARE> m(kp, ref, def);
+  const int lane = (int)(threadIdx.x % kWave), l = lane & 15;
+  // the row's slot: the window, then the vertex table
+  lds_float *const win = (lds_float *)lds + (size_t)(threadIdx.x >> 4) * slot;
+  const int cap = slot - kNmFloats;
+  LaneSimplex<T, INTERP, SQUARE> m(kp, ref, def, win + cap, l == 0);
+  auto &s = m.s;
+  constexpr int kPer = KPER;
+  constexpr int lead = INTERP == 0 ? 0 : 2, trail = INTERP == 0 ? 1 : 3;
+  float rv[kPer];  // reference values of the lane's pixels
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) rv[k] = 0.0f;
+  int wx0 = 0, wy0 = 0, wp = 1, wr = 0;  // the row's window (local deformed coordinates); wr = 0: none
   int gid = 0;
   bool need = true, live = false;
 #pragma unroll 1
   while (true) {
-    const unsigned long long want = __ballot(need);
+    const unsigned long long want = __ballot(need && l == 0);
     if (want) {
       const int leader = __ffsll((unsigned long long)want) - 1;
       int base = 0;
       if (lane == leader) base = atomicAdd(counter, __popcll(want));
       base = __builtin_amdgcn_readlane(base, leader);
       if (need) {
-        const int k = base + __popcll(want & ((1ull << lane) - 1ull));
+        const int k = base + __popcll(want & ((1ull << (lane & ~15)) - 1ull));
         live = k < total;
         if (live) {
           gid = items.at(k);
           m.start(ss, gid, fields);
+          s.template small_walk<KPER>(l, [&](int q, int x, int y, bool) { rv[q] = s.ref_value(x - ref.ox, y - ref.oy); });
+          wr = 0;
         }
         need = false;
       }
@@ -3384,27 +3432,88 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_lanes_kernel(KPa
     if (__ballot(live) == 0) break;
     if (live) {
       m.A.prepare();
-      const float g = m.lane_gamma();
+      // the deformed pixels under this point's footprint (as Solver::stage_affine)
+      float lo_x, hi_x, lo_y, hi_y;
+      s.warp_bounds(m.A, lo_x, hi_x, lo_y, hi_y);
+      const float mg = 1e-3f;
+      const int nx0 = max(s.cx - def.ox + (int)floorf(lo_x - mg) - lead, 0);
+      const int nx1 = min(s.cx - def.ox + (int)floorf(hi_x + mg) + trail, def.w - 1);
+      const int ny0 = max(s.cy - def.oy + (int)floorf(lo_y - mg) - lead, 0);
+      const int ny1 = min(s.cy - def.oy + (int)floorf(hi_y + mg) + trail, def.h - 1);
+      const bool inwin = wr > 0 && nx0 >= wx0 && nx1 < wx0 + wp && ny0 >= wy0 && ny1 < wy0 + wr;
+      bool fits = true;
+      if (!inwin && nx1 >= nx0 && ny1 >= ny0) {
+        const int bw = nx1 - nx0 + 1, bh = ny1 - ny0 + 1;
+        int e = 3;
+        while (e > 0 && (bw + 2 * e) * (bh + 2 * e) > cap) --e;
+        fits = bw * bh <= cap;
+        if (fits) {
+          const int x0 = max(nx0 - e, 0), y0 = max(ny0 - e, 0);
+          wx0 = x0;
+          wy0 = y0;
+          wp = min(nx1 + e, def.w - 1) - x0 + 1;
+          wr = min(ny1 + e, def.h - 1) - y0 + 1;
+          const auto *P = Pix<T>::ptr(def.pix) + y0 * def.w + x0;
+          const float inv = 1.0f / (float)wp;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+#pragma unroll 4
+          for (int c = l; c < wp * wr; c += 16) {
+            const int r = (int)(((float)c + 0.5f) * inv);  // exact: c < 2^12
+            win[c] = (float)P[r * def.w + (c - r * wp)];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        }
+      }
+      if (!fits) {
+        if (l == 0) nm_out[(size_t)gid * kNmStride + 7] = __int_as_float(kNmNoResult);
+        need = true;
+        continue;
+      }
+      // the interpolant from the window (the arithmetic of Solver::sample)
+      auto smp = [&](float ox, float oy) -> float {
+        const float flx = floorf(ox), fly = floorf(oy);
+        const float tx = ox - flx, ty = oy - fly;
+        const int ix = s.cx - def.ox + (int)flx - wx0, iy = s.cy - def.oy + (int)fly - wy0;
+        if (INTERP == 0) return bilinear_sum(win + (iy * wp + ix), wp, tx, ty);
+        float wx[6], wy[6];
+        keys_weights(tx, wx);
+        keys_weights(ty, wy);
+        return keys_sum(win + ((iy - 2) * wp + ix - 2), wp, wx, wy);
+      };
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      s.template small_partial<KPER>(m.A, l, acc, smp, [&](int, int, int k) { return rv[k]; });
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k] = row_sum16(acc[k]);
+      int active;
+      float contrast;
+      const float g = s.gamma_of(acc, active, contrast);
       int iters = 0;
       const int st = m.step(g, iters);
       if (st >= 0) {
-        float *o = nm_out + (size_t)gid * kNmStride;
+        if (l == 0) {
+          float *o = nm_out + (size_t)gid * kNmStride;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) o[j] = m.A.p[j];
-        o[6] = __int_as_float(iters);
-        o[7] = __int_as_float(st);
+          for (int j = 0; j < 6; ++j) o[j] = m.A.p[j];
+          o[6] = __int_as_float(iters);
+          o[7] = __int_as_float(st);
+        }
         need = true;
       }
     }
   }
 }
 
-template <typename T, int INTERP, bool SQUARE>
+// LDS floats of one row's window slot in simplex_groups_kernel (16 rows per
+// 256-thread workgroup, four workgroups per CU: 150 KB).
+constexpr int kGroupSlotFloats = 600;
+
+template <typename T, int INTERP, bool SQUARE, int KPER>
 hipError_t launch_simplex_lanes_variant(const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &subs,
                                         const double *fields, const PassArgs &pa, float *nm_out, int *counter,
                                         int groups, hipStream_t stream) {
-  hipLaunchKernelGGL((simplex_lanes_kernel<T, INTERP, SQUARE>), dim3(groups), dim3(256), 0, stream, kp, ref, def,
-                     subs, fields, pa, nm_out, counter);
+  hipLaunchKernelGGL((simplex_groups_kernel<T, INTERP, SQUARE, KPER>), dim3(groups), dim3(256),
+                     (size_t)16 * kGroupSlotFloats * sizeof(float), stream, kp, ref, def, subs, fields, pa, nm_out,
+                     counter, kGroupSlotFloats);
   return hipGetLastError();
 }
 

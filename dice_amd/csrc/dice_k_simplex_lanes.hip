@@ -1,36 +1,54 @@
-// simplex_lanes_kernel: BATCHED Nelder-Mead iterations with one subset per lane
-// (template instantiations of dice_device.h per pixel storage / interpolation).
+// simplex_groups_kernel: BATCHED Nelder-Mead iterations with 16 lanes per
+// subset (template instantiations of dice_device.h per pixel storage /
+// interpolation / subset shape, and per bound on the pixels of one lane).
 #include "dice_device.h"
 
 namespace dice_hip {
 
 namespace {
-// Persistent grid: every CU at the occupancy the kernel's registers allow.
-template <typename T, int INTERP, bool SQUARE>
-int lane_groups() {
-  static int groups = 0;
-  if (groups == 0) {
+// Persistent grid: every CU at the occupancy the kernel's registers and LDS allow.
+template <typename T, int INTERP, bool SQUARE, int KPER>
+int group_blocks() {
+  static int blocks = 0;
+  if (blocks == 0) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::simplex_lanes_kernel<T, INTERP, SQUARE>, 256, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::simplex_groups_kernel<T, INTERP, SQUARE, KPER>, 256,
+                                                     (size_t)16 * dev::kGroupSlotFloats * sizeof(float)) != hipSuccess ||
         per_cu <= 0)
       per_cu = 2;
-    groups = cus * per_cu;
+    blocks = cus * per_cu;
   }
-  return groups;
+  return blocks;
+}
+
+template <typename T, int INTERP, bool SQUARE, int KPER>
+hipError_t launch(const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &subs,
+                  const double *fields, const PassArgs &pa, float *nm_out, int *counter, hipStream_t stream) {
+  const int need = (subs.n + 15) / 16;  // at most one row of 16 lanes per subset
+  const int have = group_blocks<T, INTERP, SQUARE, KPER>();
+  return dev::launch_simplex_lanes_variant<T, INTERP, SQUARE, KPER>(kp, ref, def, subs, fields, pa, nm_out, counter,
+                                                                    need < have ? need : have, stream);
+}
+
+// subsets of at most 96 pixels (9 x 9 and smaller): six pixels per lane
+template <typename T, int INTERP, bool SQUARE>
+hipError_t launch_by_size(int max_pixels, const KParams &kp, const DImage &ref, const DImage &def,
+                          const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out, int *counter,
+                          hipStream_t stream) {
+  if (max_pixels <= 96) return launch<T, INTERP, SQUARE, 6>(kp, ref, def, subs, fields, pa, nm_out, counter, stream);
+  return launch<T, INTERP, SQUARE, kLaneSimplexMaxPixels / 16>(kp, ref, def, subs, fields, pa, nm_out, counter,
+                                                               stream);
 }
 }  // namespace
 
-hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, const DImage &ref, const DImage &def,
-                                const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out,
-                                int *counter, hipStream_t stream) {
-  const int need = (subs.n + 255) / 256;  // at most one lane per subset
-#define V(T, I, Q)                                                                                         \
-  dev::launch_simplex_lanes_variant<T, I, Q>(kp, ref, def, subs, fields, pa, nm_out, counter,              \
-                                             need < lane_groups<T, I, Q>() ? need : lane_groups<T, I, Q>(), stream)
+hipError_t launch_simplex_lanes(int storage, int interp, int max_pixels, const KParams &kp, const DImage &ref,
+                                const DImage &def, const DSubsets &subs, const double *fields, const PassArgs &pa,
+                                float *nm_out, int *counter, hipStream_t stream) {
+  if (max_pixels <= 0 || max_pixels > kLaneSimplexMaxPixels) return hipErrorInvalidValue;
+#define V(T, I, Q) launch_by_size<T, I, Q>(max_pixels, kp, ref, def, subs, fields, pa, nm_out, counter, stream)
   if (subs.S > 0) {
     if (storage == 0) return interp == 0 ? V(float, 0, true) : V(float, 2, true);
     return interp == 0 ? V(uint16_t, 0, true) : V(uint16_t, 2, true);

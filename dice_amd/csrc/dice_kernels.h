@@ -121,16 +121,20 @@ struct Launch {
   int waves_per_group;         // batched
   int groups;                  // batched: persistent grid size
   int lds_floats_per_wave;     // batched: LDS window budget
-  int lane_simplex;            // batched: Nelder-Mead passes run one subset per lane first
+  int lane_simplex;            // batched: 0, or the most pixels of a subset (<= kLaneSimplexMaxPixels):
+                               // Nelder-Mead passes run 16 lanes per subset first (simplex_groups_kernel)
 };
 
 // Subsets of at most this many pixels run the Nelder-Mead iterations of the
-// BATCHED mode one subset per lane (simplex_lanes_kernel): a wave then works on
-// 64 subsets instead of spreading a few dozen pixels over 64 lanes.
+// BATCHED mode 16 lanes per subset (simplex_groups_kernel): a wave then works
+// on four subsets instead of spreading a few dozen pixels over 64 lanes.
 constexpr int kLaneSimplexMaxPixels = 256;
-// Per-subset result of simplex_lanes_kernel: the last evaluated point (6
+// Per-subset result of simplex_groups_kernel: the last evaluated point (6
 // floats), then the iteration count and the optimizer status as int bits.
+// Status kNmNoResult: the kernel left the subset (its window did not fit the
+// LDS slot), and the pass runs the simplex itself.
 constexpr int kNmStride = 8;
+constexpr int kNmNoResult = -1;
 
 // Device work lists of one correlation pass.  items == nullptr: every subset;
 // otherwise the *item_count subsets listed in items.  Subsets that ask for the
@@ -143,7 +147,7 @@ struct PassArgs {
   int *fb_list, *fb_count;
   int *ov_list, *ov_count;
   int *counter;                // BATCHED work counters, kWorkRanges of them kCounterStride ints apart (zeroed before the launch)
-  const float *nm_result;      // Nelder-Mead passes: per-subset results of simplex_lanes_kernel, or null
+  const float *nm_result;      // Nelder-Mead passes: per-subset results of simplex_groups_kernel, or null
 };
 
 // Differentiation path (forward gamma / adjoint) of one launch.
@@ -168,7 +172,7 @@ constexpr int kWorkCounterInts = kWorkRanges * kCounterStride;
 constexpr int kWorkHeader = kWorkCounterInts + kCounterStride;
 constexpr int kWorkFbCount = kWorkCounterInts, kWorkOv0Count = kWorkCounterInts + 1,
               kWorkOv1Count = kWorkCounterInts + 2;
-constexpr int kWorkLaneCount = kWorkCounterInts + 3;  // work counter of simplex_lanes_kernel
+constexpr int kWorkLaneCount = kWorkCounterInts + 3;  // work counter of simplex_groups_kernel
 // count of the lean pass's hand-over list, which lives in the storage of the pass-1
 // overflow list (pass 1 starts after pass 0 has finished with it)
 constexpr int kWorkLeanCount = kWorkCounterInts + 4;
@@ -261,12 +265,13 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
 hipError_t launch_gamma(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
                         const DSubsets &subs, const GammaArgs &ga, int *work, hipStream_t stream);
 
-// Enqueue the Nelder-Mead iterations of the items of pa, one subset per lane:
-// results to nm_out (kNmStride floats per subset); counter: one int, zeroed
-// before the launch.
-hipError_t launch_simplex_lanes(int storage, int interp, const KParams &kp, const DImage &ref, const DImage &def,
-                                const DSubsets &subs, const double *fields, const PassArgs &pa, float *nm_out,
-                                int *counter, hipStream_t stream);
+// Enqueue the Nelder-Mead iterations of the items of pa, 16 lanes per subset
+// (subsets of at most max_pixels <= kLaneSimplexMaxPixels pixels): results to
+// nm_out (kNmStride floats per subset); counter: one int, zeroed before the
+// launch.
+hipError_t launch_simplex_lanes(int storage, int interp, int max_pixels, const KParams &kp, const DImage &ref,
+                                const DImage &def, const DSubsets &subs, const double *fields, const PassArgs &pa,
+                                float *nm_out, int *counter, hipStream_t stream);
 
 // Finite-difference gradients (+ optional 5x5 smoothing) of an image.
 hipError_t launch_gradients(int storage, const void *pix, int w, int h, float *gx, float *gy, int smooth,

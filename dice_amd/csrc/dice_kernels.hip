@@ -695,7 +695,8 @@ static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const 
     int *lane_counter = pa.counter + kWorkLaneCount;
     e = launch_zero_ints(lane_counter, 1, stream);
     if (e != hipSuccess) return e;
-    e = launch_simplex_lanes(storage, kp.interp, kp, ref, def, subs, fields, pa, nm, lane_counter, stream);
+    e = launch_simplex_lanes(storage, kp.interp, L.lane_simplex, kp, ref, def, subs, fields, pa, nm, lane_counter,
+                             stream);
     if (e != hipSuccess) return e;
     pa.nm_result = nm;
   }

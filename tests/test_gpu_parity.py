@@ -272,7 +272,7 @@ def test_simplex_speckle_parity(ctx, mode, nparams):
 @pytest.mark.parametrize("nparams", [2, 6])
 def test_simplex_lane_path_speckle(ctx, mode, nparams):
     """SIMPLEX on 15 x 15 subsets (225 px): small enough for the BATCHED mode to run the Nelder-Mead
-    iterations one subset per lane, the persistent kernel doing the checks and the epilogue."""
+    iterations 16 lanes per subset, the persistent kernel doing the checks and the epilogue."""
     d = load("speckle_512.npz")
     cx, cy = dh.regular_grid(512, 512, 15, 24)
     p = dh.default_params()
@@ -290,7 +290,7 @@ def test_simplex_lane_path_speckle(ctx, mode, nparams):
 
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
 def test_simplex_lane_path_conformal(ctx, mode):
-    """SIMPLEX on polygon subsets of under 256 pixels (with excluded pixels) through the one-subset-per-lane
+    """SIMPLEX on polygon subsets of under 256 pixels (with excluded pixels) through the 16-lanes-per-subset
     iterations."""
     ref, de = synth.config1(seed=5)
     cx, cy, conf = conformal_set(512, 512, 120, 4, rmin=7, rmax=8.8)
@@ -307,7 +307,7 @@ def test_simplex_lane_path_conformal(ctx, mode):
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: MODE_NAMES[m])
 def test_fallback_lane_path(ctx, mode):
     """GRADIENT_BASED_THEN_SIMPLEX on 15 x 15 subsets: the simplex fallback pass over the device list runs
-    one subset per lane; FOV and initial-guess failures keep their statuses."""
+    16 lanes per subset; FOV and initial-guess failures keep their statuses."""
     d = load("speckle_512.npz")
     cx = np.array([200.0, 4.0, 300.0, 150.0, 260.0, 330.0])
     cy = np.array([200.0, 200.0, 300.0, 150.0, 120.0, 400.0])
@@ -323,6 +323,27 @@ def test_fallback_lane_path(ctx, mode):
     ok = np.isin(fg[dh.F["STATUS"]], dh.CONVERGED_STATUSES)
     assert ok.sum() == 4
     assert np.abs(fg[dh.F["DISP_X"]] - fo[o.F["DISP_X"]])[ok].max() < 2e-2
+
+
+@pytest.mark.parametrize("size", [9, 15], ids=["9x9", "15x15"])
+@pytest.mark.parametrize("interp", [dh.BILINEAR, dh.KEYS_FOURTH], ids=["bilinear", "keys"])
+def test_simplex_plain_and_batched_paths_agree(ctx, size, interp):
+    """Subsets of at most 256 pixels sum the Nelder-Mead objective in one order in both modes (16 strided
+    lane partials, then a pairwise tree), so the BATCHED 16-lanes-per-subset iterations and the PLAIN
+    workgroup follow the same path: identical statuses, iteration counts and final parameters on every
+    subset (the epilogue's other sums may differ in rounding)."""
+    d = load("speckle_512.npz")
+    cx, cy = dh.regular_grid(512, 512, size, 12)
+    p = dh.default_params()
+    p.optimization_method = dh.SIMPLEX
+    p.interpolation_method = interp
+    p.enable_rotation = p.enable_normal_strain = p.enable_shear_strain = 1
+    fb = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, size, dh.MODE_BATCHED)
+    fp = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, size, dh.MODE_PLAIN)
+    assert np.array_equal(fb[dh.F["STATUS"]], fp[dh.F["STATUS"]])
+    assert np.array_equal(fb[dh.F["ITERATIONS"]], fp[dh.F["ITERATIONS"]])
+    for name in ("DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "SS_XY"):
+        assert np.array_equal(fb[dh.F[name]], fp[dh.F[name]]), name
 
 
 def conformal_set(w, h, n, seed, rmin=10, rmax=20):
