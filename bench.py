@@ -140,13 +140,30 @@ def result_line(args, metric, value, unit, ms_per_step, dtype, config, roofline,
     return r
 
 
-def roofline(bytes_per_launch, launch_ms, kernel, launches, traffic=None, traffic_src=None):
+def profile_figures(name, applies=True):
+    """(traffic_bytes_per_launch, source, valu_busy) of a committed profile summary: HBM-side bytes per
+    launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of the workload, scaled by a copy kernel of
+    known size, and the VALU issue fraction from its SQ counters.  (None, None, None) when the summary is
+    absent or does not describe this configuration (applies False)."""
+    path = os.path.join(ROOT, "profiles", name)
+    traffic = valu_busy = None
+    if applies and os.path.exists(path):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
+            elif line.startswith("valu_busy:"):
+                valu_busy = float(line.split(":", 1)[1])
+    return traffic, ("profiles/" + name if traffic is not None else None), valu_busy
+
+
+def roofline(bytes_per_launch, launch_ms, kernel, launches, traffic=None, traffic_src=None, valu_busy=None):
     gbs = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": round(traffic) if traffic else None,
             "traffic_source": traffic_src if traffic else None, "kernel": kernel,
             "avg_launch_ms": round(launch_ms, 4), "launches_timed": launches,
-            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
+            **({"valu_busy": valu_busy} if valu_busy is not None else {})}
 
 
 def bench_c2(args):
@@ -242,17 +259,8 @@ def bench_c2(args):
     dvv = np.abs(hist[:, dh.F["DISP_Y"]] - vt)[conv]
     du_w = np.abs(hist[:, dh.F["DISP_X"]] - ut_w)[conv]
 
-    # HBM-side bytes per launch of the batched kernel come from rocprofv3 --pmc
-    # FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a copy kernel of
-    # known size; the summary that holds them is committed with the profiles
-    traffic = valu_busy = None
-    traffic_src = os.path.join(ROOT, "profiles", "correlate_lean_c2_hbm_traffic.md")
-    if (W, S, args.step, len(frames)) == (2048, 41, 5, 10) and os.path.exists(traffic_src):
-        for line in open(traffic_src):
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = float(line.split(":", 1)[1])
-            elif line.startswith("valu_busy:"):
-                valu_busy = float(line.split(":", 1)[1])
+    traffic, traffic_src, valu_busy = profile_figures("correlate_lean_c2_hbm_traffic.md",
+                                                      (W, S, args.step, len(frames)) == (2048, 41, 5, 10))
 
     # roofline of the correlation: algorithmic bytes per frame / the time of one frame's correlation,
     # taken from the same timed graph replays as ms_per_step (every kernel of the frame counted: the
@@ -275,12 +283,9 @@ def bench_c2(args):
          "mean_abs_u_err_vs_window_avg_px": round(float(du_w.mean()), 5),
          "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
         roofline(avg_bytes, frame_ms, "correlate_lean_kernel (GN common case) + correlate_batched_kernel (hand-offs)",
-                 args.steps * len(frames), traffic, "profiles/correlate_lean_c2_hbm_traffic.md"),
+                 args.steps * len(frames), traffic, traffic_src, valu_busy),
         eager_ms_per_step=round(1000.0 * el_e / max(1, min(args.steps, 3)), 3),
         eager_launch_ms=round(kern_ms, 4))
-    if valu_busy is not None:
-        # the bound that binds: VALU issue, from the SQ counters of the same profile
-        result["roofline"]["valu_busy"] = valu_busy
 
     if not args.no_plain:
         el_p, _, _ = timed(dh.MODE_PLAIN, max(1, min(args.steps, 2)), 1)
@@ -423,7 +428,8 @@ def bench_c3(args):
          "launch_shape": ctx.launch_shape(), "default_launch_shape": default_shape,
          "input_generation_s": round(gen_s, 1)},
         roofline(float(bytes_per_frame.mean()), kern_ms,
-                 "correlation pass (correlate_batched_kernel + plain overflow), GN, Keys-4, f32", kern_cnt))
+                 "correlation pass (correlate_batched_kernel + plain overflow), GN, Keys-4, f32", kern_cnt,
+                 *profile_figures("correlate_batched_c3_counters.md", n_req == 50000)))
 
     if not args.no_plain:
         ps = max(1, min(args.steps, 1))
@@ -533,7 +539,8 @@ def bench_c4(args):
          "forward_roofline_frac": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
          "mean_gamma": float(g.mean()), "dparams_finite": bool(np.isfinite(dp).all()),
          "dimage_nonzero_px": int((img != 0).sum()), "input_generation_s": round(gen_s, 1)},
-        roofline(adj_bytes, adj_ms, "gamma_batched_kernel (adjoint: dL/dp + dL/dI_def, Keys-4)", len(adj)),
+        roofline(adj_bytes, adj_ms, "gamma_batched_kernel (adjoint: dL/dp + dL/dI_def, Keys-4)", len(adj),
+                 *profile_figures("gamma_batched_c4_counters.md")),
         cpu_baseline={"value": None, "unit": "subsets/s", "cores": 0, "kind": "port",
                       "sample": "none: the reference has no adjoint of its objective (its CPU path differentiates "
                                 "only inside computeUpdateFast's Gauss-Newton step)"})
@@ -616,6 +623,8 @@ def bench_c5(args):
     # Nelder-Mead evaluations: 7 vertices + at least 2 per iteration (the trial point and the
     # end-of-iteration evaluation); uint16 storage, bilinear window (S+3)^2
     evals = np.where(iters >= 0, 7 + 2 * np.maximum(iters, 0), 0)
+    traffic, traffic_src, valu_busy = profile_figures("simplex_groups_c5_kernel_trace.md",
+                                                      (W, H, S, step) == (4096, 3000, 9, 4))
     bytes_launch = float(algorithmic_bytes(iters, status, S * S, (S + 3) ** 2, storage_bytes=2, evals=evals).sum()
                          - 10.0 * S * S * n)  # reference: 2 N (uint16 intensities, no gradients)
     result = result_line(
@@ -629,8 +638,9 @@ def bench_c5(args):
          "mean_abs_u_err_px": round(float(du.mean()), 5), "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
          "mean_abs_v_err_px": round(float(dv.mean()), 5),
          "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
-        roofline(bytes_launch, kern_ms, "correlation pass (correlate_batched_kernel, Nelder-Mead, bilinear, u16)",
-                 kern_cnt))
+        roofline(bytes_launch, kern_ms, "correlation pass (simplex_groups_kernel: Nelder-Mead iterations, 16 lanes "
+                 "per subset; correlate_batched_kernel: epilogue), bilinear, u16", kern_cnt, traffic, traffic_src,
+                 valu_busy))
     if not args.no_plain:
         el_p, _, _ = timed(dh.MODE_PLAIN, 1, 0)
         fp = d_fields.cpu().numpy()
