@@ -2871,7 +2871,7 @@ struct Solver {
   // SIMPLEX_OPT selects which optimizer this instantiation runs.  A BATCHED
   // wave whose window overflows its LDS slot pushes the subset to ov_list and
   // leaves it to the PLAIN kernel (nothing it wrote is an input of the solve).
-  // nm: per-subset Nelder-Mead results of simplex_lanes_kernel for this pass
+  // nm: per-subset Nelder-Mead results of simplex_groups_kernel for this pass
   // (SIMPLEX_OPT only), or null to run the simplex here.
   template <bool SIMPLEX_OPT>
   __device__ __forceinline__ void run(const DSubsets &ss, int gid, double *fields, int pass, int *fb_list,
