@@ -125,6 +125,7 @@ struct dice_hip_context {
   int max_side = 0, p90_side = 0;
   int max_count = 0;  // most pixels in one subset
   DevBuf cx, cy, off, px, py, act, bbox, nact, pxy, abox;
+  DevBuf order;         // conformal: the work order (Morton order of the centres)
   bool packed = false;  // conformal pixels also packed into pxy
   // per-subset options: force simplex flags, blocked-pixel masks (box, word offset, bits)
   DevBuf force, mbox, moff, mbits;
@@ -271,7 +272,8 @@ void fill_views(dice_hip_context *ctx, DImage &ref, DImage &def, DSubsets &ss) {
                 ctx->has_force ? ctx->force.as<uint8_t>() : nullptr,
                 ctx->has_masks ? ctx->mbox.as<int4>() : nullptr,
                 ctx->has_masks ? ctx->moff.as<int>() : nullptr,
-                ctx->has_masks ? ctx->mbits.as<uint32_t>() : nullptr};
+                ctx->has_masks ? ctx->mbits.as<uint32_t>() : nullptr,
+                ctx->S > 0 ? nullptr : ctx->order.as<int>()};
 }
 
 Launch current_launch(dice_hip_context *ctx, int mode) {
@@ -406,7 +408,7 @@ int dice_hip_destroy(dice_hip_context *ctx) {
                     &ctx->nb_guess[1], &ctx->nb_list, &ctx->nb_counts, &ctx->vsg_cells, &ctx->vsg_ids, &ctx->vsg_keys, &ctx->vsg_keys2, &ctx->vsg_ids2,
                     &ctx->vsg_rec, &ctx->vsg_temp, &ctx->vsg_bounds,
                     &ctx->vsg_out, &ctx->obj_io, &ctx->fields, &ctx->work, &ctx->force, &ctx->mbox, &ctx->moff,
-                    &ctx->mbits, &ctx->lean_cache})
+                    &ctx->mbits, &ctx->lean_cache, &ctx->order})
     b->release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -624,6 +626,28 @@ int dice_hip_set_subsets(dice_hip_context *ctx, int n, const double *coord_x, co
     HIP_TRY(ctx, hipMemcpy(ctx->nact.p, na.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
     HIP_TRY(ctx, ctx->abox.ensure(n * sizeof(int4)), "hipMalloc subsets");
     HIP_TRY(ctx, hipMemcpy(ctx->abox.p, ab.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
+    // work order: Morton order of the centres (ties by subset id)
+    {
+      auto spread = [](uint32_t v) {
+        v &= 0xffffu;
+        v = (v | (v << 8)) & 0x00ff00ffu;
+        v = (v | (v << 4)) & 0x0f0f0f0fu;
+        v = (v | (v << 2)) & 0x33333333u;
+        v = (v | (v << 1)) & 0x55555555u;
+        return v;
+      };
+      std::vector<std::pair<uint32_t, int>> key(n);
+      for (int i = 0; i < n; ++i) {
+        const uint32_t x = (uint32_t)std::min(std::max(hx[i], 0), 0xffff);
+        const uint32_t y = (uint32_t)std::min(std::max(hy[i], 0), 0xffff);
+        key[i] = {spread(x) | (spread(y) << 1), i};
+      }
+      std::sort(key.begin(), key.end());
+      std::vector<int> ord(n);
+      for (int i = 0; i < n; ++i) ord[i] = key[i].second;
+      HIP_TRY(ctx, ctx->order.ensure(n * sizeof(int)), "hipMalloc subsets");
+      HIP_TRY(ctx, hipMemcpy(ctx->order.p, ord.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    }
     ctx->packed = packed;
     if (packed) {
       HIP_TRY(ctx, ctx->pxy.ensure((size_t)total * sizeof(uint32_t)), "hipMalloc subsets");

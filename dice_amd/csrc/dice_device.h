@@ -3130,8 +3130,9 @@ constexpr int kListGroups = 2048;
 struct Items {
   const int *list;
   const int *count;
+  const int *order;  // all subsets: the work order (DSubsets::order), or null
   __device__ __forceinline__ int total(int n) const { return list ? *count : n; }
-  __device__ __forceinline__ int at(int k) const { return list ? list[k] : k; }
+  __device__ __forceinline__ int at(int k) const { return list ? list[k] : (order ? order[k] : k); }
 };
 
 // ------------------------------------- 16 lanes per subset: simplex ---
@@ -3393,7 +3394,7 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
                                                                               PassArgs pa, float *nm_out,
                                                                               int *counter, int slot) {
   extern __shared__ float lds[];
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   const int total = items.total(ss.n);
   const int lane = (int)(threadIdx.x % kWave), l = lane & 15;
   // the row's slot: the window, then the vertex table
@@ -3522,7 +3523,7 @@ template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
 __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams kp, DImage ref, DImage def,
                                                                         DSubsets ss, double *fields, PassArgs pa) {
   __shared__ double red[4 * kMaxRed];
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   const int total = items.total(ss.n);
   BlockGroup g{(lds_double *)red};
   Solver<T, BlockGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)red, 0);
@@ -3549,7 +3550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   extern __shared__ float lds[];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   WaveGroup g{lane};
   Solver<T, WaveGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                           lds_per_wave);
@@ -3583,7 +3584,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   extern __shared__ float lds[];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   WaveGroup g{lane};
   Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                         lds_per_wave);
@@ -3620,7 +3621,7 @@ template <typename T, int INTERP>
 __global__ __launch_bounds__(kPlainThreads) void gamma_plain_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
                                                                     GammaArgs ga, PassArgs pa) {
   __shared__ double red[4 * kMaxRed];
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   const int total = items.total(ss.n);
   BlockGroup g{(lds_double *)red};
   Solver<T, BlockGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)red, 0);
@@ -3634,7 +3635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHE
   extern __shared__ float lds[];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const Items items{pa.items, pa.item_count};
+  const Items items{pa.items, pa.item_count, ss.order};
   WaveGroup g{lane};
   Solver<T, WaveGroup, INTERP, false> s(kp, ref, def, g, (lds_float *)(lds + (size_t)wave * lds_per_wave),
                                         lds_per_wave);

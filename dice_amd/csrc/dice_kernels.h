@@ -91,6 +91,11 @@ struct DSubsets {
   const int4 *mbox;
   const int *moff;
   const uint32_t *mbits;
+  // Work order of a pass over all subsets (may be null: subset order): item k is
+  // subset order[k].  Conformal subsets go in Morton order of their centres, so
+  // that the contiguous item range each XCD takes is one region of the images
+  // and its L2 holds that region's pixels.
+  const int *order;
 };
 
 // BATCHED Gauss-Newton reduction: kGnSums per-lane partials (q 6, H upper
