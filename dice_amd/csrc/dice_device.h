@@ -3539,13 +3539,14 @@ __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams 
 // round-robin over the 8 XCDs, so workgroup b starts on range b mod 8: the
 // subsets that one XCD works on at a time are neighbours in the grid, whose
 // windows overlap, and its L2 serves them.  An exhausted range sends the wave
-// on to the next one (work stealing).  The register budget is set for
-// DICE_BATCHED_WAVES_PER_EU waves per SIMD (4: 128 VGPRs).
+// on to the next one (work stealing).  The register budget is set for WPE
+// waves per SIMD: DICE_BATCHED_WAVES_PER_EU (4: 128 VGPRs), or 2 (256 VGPRs,
+// no spills) for slots too large for four waves per SIMD in the LDS.
 #ifndef DICE_BATCHED_WAVES_PER_EU
 #define DICE_BATCHED_WAVES_PER_EU 4
 #endif
-template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DICE_BATCHED_WAVES_PER_EU))) void correlate_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
+template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT, int WPE = DICE_BATCHED_WAVES_PER_EU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void correlate_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
                                                                 double *fields, PassArgs pa, int lds_per_wave) {
   extern __shared__ float lds[];
   const int wave = threadIdx.x / kWave;
@@ -3698,9 +3699,17 @@ hipError_t launch_variant(const Launch &L, const KParams &kp, const DImage &ref,
     return hipGetLastError();
   }
   const size_t shm = (size_t)L.waves_per_group * L.lds_floats_per_wave * sizeof(float);
-  hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX>), dim3(L.groups),
-                     dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
-                     L.lds_floats_per_wave);
+  // slots larger than a sixteenth of the CU's 160 KB leave fewer than four waves
+  // per SIMD: the variant with twice the registers then costs no occupancy at
+  // two waves per SIMD and spills nothing
+  if (L.lds_floats_per_wave > kWideSlotFloats)
+    hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX, 2>), dim3(L.groups),
+                       dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
+                       L.lds_floats_per_wave);
+  else
+    hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX>), dim3(L.groups),
+                       dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
+                       L.lds_floats_per_wave);
   return hipGetLastError();
 }
 

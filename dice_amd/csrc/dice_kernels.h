@@ -120,6 +120,8 @@ constexpr int kLeanL = 0, kLeanDinv = 15, kLeanSphi = 21, kLeanS00 = 27, kLeanS0
 constexpr int kLeanMeanF = 30, kLeanTag = 31, kLeanPass = 32;
 constexpr int kLeanCacheDoubles = 32;
 constexpr int kMinSlotFloats = kGnTotOffset + 2 * 52 + kPersistFloats;  // smallest LDS slot per wave
+// the largest slot with which four waves per SIMD (16 per CU) fit the CU's 160 KB of LDS
+constexpr int kWideSlotFloats = 160 * 1024 / (16 * 4);
 
 struct Launch {
   int mode;                    // 0 plain, 1 batched
