@@ -638,7 +638,7 @@ def bench_c5(args):
          "mean_abs_u_err_px": round(float(du.mean()), 5), "p99_abs_u_err_px": round(float(np.percentile(du, 99)), 5),
          "mean_abs_v_err_px": round(float(dv.mean()), 5),
          "launch_shape": ctx.launch_shape(), "input_generation_s": round(gen_s, 1)},
-        roofline(bytes_launch, kern_ms, "correlation pass (simplex_groups_kernel: Nelder-Mead iterations, 16 lanes "
+        roofline(bytes_launch, kern_ms, "correlation pass (simplex_groups_kernel: Nelder-Mead iterations, 8 lanes "
                  "per subset; correlate_batched_kernel: epilogue), bilinear, u16", kern_cnt, traffic, traffic_src,
                  valu_busy))
     if not args.no_plain:

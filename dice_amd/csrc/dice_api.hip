@@ -233,7 +233,14 @@ KParams make_kparams(const dice_params &p, int first_frame) {
   k.normalize_gamma = p.normalize_gamma_with_active_pixels != 0;
   k.uncertainty = p.compute_uncertainty_fields != 0;
   k.first_frame = first_frame != 0;
+  k.small_w = 16;
   return k;
+}
+
+// The lanes of the small-subset objective order for the current subset set: the
+// width that simplex_groups_kernel runs these subsets at.
+int small_width(const dice_hip_context *ctx) {
+  return ctx->max_count <= dice_hip::kNarrowSimplexMaxPixels ? 8 : 16;
 }
 
 size_t pix_bytes(int storage) { return storage == DICE_STORAGE_U16 ? 2 : 4; }
@@ -308,6 +315,7 @@ int enqueue_correlate(dice_hip_context *ctx, int mode, double *d_fields, int fir
   DSubsets ss;
   fill_views(ctx, ref, def, ss);
   KParams kp = make_kparams(ctx->params, first_frame);
+  kp.small_w = small_width(ctx);
   kp.guess = guess;
   kp.has_forced = ctx->has_force ? 1 : 0;
   // the cache holds one entry per subset of the current set (square subsets only)
@@ -935,7 +943,8 @@ int dice_hip_objective(dice_hip_context *ctx, int subset_id, int call, double *p
   DImage ref, def;
   DSubsets ss;
   fill_views(ctx, ref, def, ss);
-  const KParams kp = make_kparams(ctx->params, 0);
+  KParams kp = make_kparams(ctx->params, 0);
+  kp.small_w = small_width(ctx);
   HIP_TRY(ctx, ctx->obj_io.ensure(8 * sizeof(double)), "hipMalloc objective");
   double io[8] = {params[0], params[1], params[2], params[3], params[4], params[5], 0.0, 0.0};
   HIP_TRY(ctx, hipMemcpyAsync(ctx->obj_io.p, io, sizeof io, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy objective");
@@ -1014,7 +1023,8 @@ static int enqueue_gamma(dice_hip_context *ctx, int mode, const dice_hip::GammaA
   DImage ref, def;
   DSubsets ss;
   fill_views(ctx, ref, def, ss);
-  const KParams kp = make_kparams(ctx->params, 0);
+  KParams kp = make_kparams(ctx->params, 0);
+  kp.small_w = small_width(ctx);
   const Launch L = current_launch(ctx, mode);
   HIP_TRY(ctx, dice_hip::launch_gamma(L, ctx->rstorage, kp, ref, def, ss, ga, ctx->work.as<int>(), ctx->stream),
           "gamma launch");

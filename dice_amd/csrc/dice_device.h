@@ -83,6 +83,19 @@ __device__ __forceinline__ double row_sum16(double v) {
   v += dpp<kDppMirror>(v);
   return v;
 }
+// The same tree over each half-row of 8 lanes (the first three steps).
+__device__ __forceinline__ double row_sum8(double v) {
+  v += dpp<kDppXor1>(v);
+  v += dpp<kDppXor2>(v);
+  v += dpp<kDppHalfMirror>(v);
+  return v;
+}
+template <int W>
+__device__ __forceinline__ double row_sum(double v) {
+  static_assert(W == 8 || W == 16, "rows of 8 or 16 lanes");
+  if constexpr (W == 8) return row_sum8(v);
+  else return row_sum16(v);
+}
 // Sum over the wave: the four row sums added by v_readlane; the result is uniform.
 __device__ __forceinline__ double wave_sum(double v) {
   v = row_sum16(v);
@@ -268,8 +281,13 @@ __device__ __forceinline__ float keys_sum(const P *b, int p, const float wx[6], 
 }
 template <typename P>
 __device__ __forceinline__ float bilinear_sum(const P *b, int p, float tx, float ty) {
-  return (float)b[0] * (1.0f - tx) * (1.0f - ty) + (float)b[1] * tx * (1.0f - ty) + (float)b[p + 1] * tx * ty +
-         (float)b[p] * (1.0f - tx) * ty;
+  // explicit products and fmaf: no multiply-add is left for the compiler to
+  // contract, so every kernel that inlines this rounds it the same way
+  const float w00 = (1.0f - tx) * (1.0f - ty), w10 = tx * (1.0f - ty), w11 = tx * ty, w01 = (1.0f - tx) * ty;
+  float v = (float)b[0] * w00;
+  v = fmaf((float)b[1], w10, v);
+  v = fmaf((float)b[p + 1], w11, v);
+  return fmaf((float)b[p], w01, v);
 }
 
 // Affine map and its Jacobian rows (Affine_Shape_Function::map / residuals,
@@ -283,16 +301,19 @@ struct Affine {
     c = cosf(p[2]);
     s = sinf(p[2]);
     const float sx = 1.0f + p[3], sy = 1.0f + p[4], g = p[5];
-    // x' - cx = c*Dx - s*Dy + u, Dx = sx*dx + g*dy, Dy = sy*dy + g*dx
-    a11 = c * sx - s * g;
-    a12 = c * g - s * sy;
-    a21 = s * sx + c * g;
-    a22 = s * g + c * sy;
+    // x' - cx = c*Dx - s*Dy + u, Dx = sx*dx + g*dy, Dy = sy*dy + g*dx.  Here and in
+    // disp the multiply-adds are explicit fmaf: nothing is left for the compiler
+    // to contract, so every kernel computes the same warp (the Nelder-Mead paths
+    // of the execution modes must agree bit for bit)
+    a11 = fmaf(c, sx, -(s * g));
+    a12 = fmaf(c, g, -(s * sy));
+    a21 = fmaf(s, sx, c * g);
+    a22 = fmaf(s, g, c * sy);
   }
   // displacement of pixel offset (dx, dy) from the centroid: x' - cx, y' - cy
   __device__ __forceinline__ void disp(float dx, float dy, float &ox, float &oy) const {
-    ox = p[0] + (a11 * dx + a12 * dy);
-    oy = p[1] + (a21 * dx + a22 * dy);
+    ox = p[0] + fmaf(a11, dx, a12 * dy);
+    oy = p[1] + fmaf(a21, dx, a22 * dy);
   }
   // residual row; (Gx, Gy) is the gradient to use (rotated ref or interpolated def)
   __device__ __forceinline__ void residual(float dx, float dy, float Gx, float Gy, float r[6]) const {
@@ -1239,25 +1260,25 @@ struct Solver {
   // ------------------------------- objective sums of small subsets ---
   // Subsets of at most kLaneSimplexMaxPixels pixels sum the objective in one
   // fixed order in every execution mode, so that the Nelder-Mead paths (which
-  // fork on rounding) agree between them: lane l < 16 of a row adds its
-  // pixels l, l + 16, l + 32, ... in that order (FP64), then the 16 partials
-  // are added by row_sum16's pairwise tree.
+  // fork on rounding) agree between them: lane l < W of a row adds its pixels
+  // l, l + W, l + 2 W, ... in that order (FP64), then the W partials are added
+  // by row_sum<W>'s pairwise tree.  W = kp.small_w: 8 when no subset of the set
+  // has more than kNarrowSimplexMaxPixels pixels, else 16.
   static constexpr int kSmallPerLane = kLaneSimplexMaxPixels / 16;
-  // f(k, x, y, active) for the pixels i = l + 16 k of the subset (coordinates
+  // f(k, x, y, active) for the pixels i = l + W k of the subset (coordinates
   // as for_pixels gives them)
-  template <int KPER = kSmallPerLane, class F>
+  // UNROLL: the loop unrolled (f may then index registers by k), else rolled.
+  template <int W = 16, int KPER = kSmallPerLane, bool UNROLL = true, class F>
   __device__ __forceinline__ void small_walk(int l, F &&f) const {
     int row = 0, col = 0, drow = 0, dcol = 0;
     if (S > 0) {
       row = l / S;
       col = l - row * S;
-      drow = 16 / S;
-      dcol = 16 - drow * S;
+      drow = W / S;
+      dcol = W - drow * S;
     }
-#pragma unroll
-    for (int k = 0; k < KPER; ++k) {
-      const int i = l + 16 * k;
-      if (i >= n) break;
+    auto pixel = [&](int k) {
+      const int i = l + W * k;
       if (S > 0) {
         f(k, max(cx - hw + col, 0), max(cy - hw + row, 0), true);
         col += dcol;
@@ -1269,15 +1290,25 @@ struct Solver {
       } else {
         f(k, max(px[i], 0), max(py[i], 0), pact ? (pact[i] != 0) : true);
       }
+    };
+    if constexpr (UNROLL) {
+#pragma unroll
+      for (int k = 0; k < KPER; ++k) {
+        if (l + W * k >= n) break;
+        pixel(k);
+      }
+    } else {
+#pragma unroll 1
+      for (int k = 0; l + W * k < n; ++k) pixel(k);
     }
   }
   // Lane l's partial gamma sums (n, sd, sdd, sr, srr, sdr) of the kept pixels at
   // A; smp(ox, oy): the interpolant, refv(x, y, k): the reference value.  KPER:
-  // a bound on the pixels per lane (the subset has at most 16 KPER pixels).
-  template <int KPER = kSmallPerLane, class Smp, class Ref>
+  // a bound on the pixels per lane (the subset has at most W KPER pixels).
+  template <int W = 16, int KPER = kSmallPerLane, bool UNROLL = true, class Smp, class Ref>
   __device__ __forceinline__ void small_partial(const Affine &A, int l, double (&acc)[6], Smp &&smp,
                                                 Ref &&refv) const {
-    small_walk<KPER>(l, [&](int k, int x, int y, bool a) {
+    small_walk<W, KPER, UNROLL>(l, [&](int k, int x, int y, bool a) {
       if (!a) return;
       const double r = refv(x, y, k);
       float ox, oy;
@@ -1302,10 +1333,18 @@ struct Solver {
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
     if constexpr (G::kSize >= 16) {
       if (n <= kLaneSimplexMaxPixels) {
-        // the order of simplex_groups_kernel (small_partial), in ranks 0..15
-        if (grp.rank() < 16) {
-          small_partial(A, grp.rank(), acc, [&](float ox, float oy) { return sample(ox, oy); },
-                        [&](int x, int y, int) { return ref_value(x - ref.ox, y - ref.oy); });
+        // the order of simplex_groups_kernel (small_partial), in ranks 0..W-1
+        auto smp = [&](float ox, float oy) { return sample(ox, oy); };
+        auto refv = [&](int x, int y, int) { return ref_value(x - ref.ox, y - ref.oy); };
+        const int r0 = grp.rank();
+        if (kp.small_w == 8 && n <= kNarrowSimplexMaxPixels) {
+          if (r0 < 8) {
+            small_partial<8, kNarrowSimplexMaxPixels / 8, false>(A, r0, acc, smp, refv);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] = row_sum8(acc[k]);
+          }
+        } else if (r0 < 16) {
+          small_partial<16, kSmallPerLane, false>(A, r0, acc, smp, refv);
 #pragma unroll
           for (int k = 0; k < 6; ++k) acc[k] = row_sum16(acc[k]);
         }
@@ -1368,14 +1407,15 @@ struct Solver {
     const double cnt = acc[0];
     active = (int)cnt;
     if (cnt <= 0.0) return -1.0f;
+    // explicit fma (no contraction left to the compiler, as in Affine)
     const double md = acc[1] / cnt, mr = acc[3] / cnt;
-    const double sdd = fmax(acc[2] - cnt * md * md, 0.0);
-    const double srr = fmax(acc[4] - cnt * mr * mr, 0.0);
-    const double sdr = acc[5] - cnt * md * mr;
+    const double sdd = fmax(fma(-(cnt * md), md, acc[2]), 0.0);
+    const double srr = fmax(fma(-(cnt * mr), mr, acc[4]), 0.0);
+    const double sdr = fma(-(cnt * md), mr, acc[5]);
     contrast = (float)sqrt(sdd / cnt);
     // Subset::gamma returns -1 when either norm is zero (flat subset)
     if (sdd <= 1e-13 * acc[2] || srr <= 1e-13 * acc[4]) return -1.0f;
-    double gam = 2.0 - 2.0 * sdr / (sqrt(sdd) * sqrt(srr));
+    double gam = fma(-2.0, sdr / (sqrt(sdd) * sqrt(srr)), 2.0);
     if (gam < 0.0) gam = 0.0;
     float g = (float)gam;
     if (kp.normalize_gamma && n_def_active > 0) g = (float)((double)g / (double)n_def_active);
@@ -2540,7 +2580,7 @@ struct Solver {
           const float fac2 = fac1 - fac;
           if (st == 2) ysave = g_hi;
 #pragma unroll
-          for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
+          for (int j = 0; j < 6; ++j) A.p[j] = fmaf(sums[j], fac1, -(phi[j] * fac2));
 #pragma unroll
           for (int j = 2; j < 6; ++j) if (en[j] == 0.0f) A.p[j] = init[j];
           ytry = eval_here(A);
@@ -3135,7 +3175,7 @@ struct Items {
   __device__ __forceinline__ int at(int k) const { return list ? list[k] : (order ? order[k] : k); }
 };
 
-// ------------------------------------- 16 lanes per subset: simplex ---
+// --------------------------------- 8 or 16 lanes per subset: simplex ---
 // The register budget of simplex_groups_kernel as waves per SIMD (0: the
 // compiler's choice).
 #ifndef DICE_LANE_WAVES_PER_EU
@@ -3159,7 +3199,7 @@ struct LaneGroup {
   __device__ __forceinline__ int imax(int v) const { return v; }
 };
 
-// Simplex::minimize (DICe_Simplex.cpp:60-277) of one subset per row of 16
+// Simplex::minimize (DICe_Simplex.cpp:60-277) of one subset per row of 8 or 16
 // lanes, written as a state machine whose every step ends in exactly one
 // objective evaluation.  The rows of a wave are each at their own stage of
 // their own subset (vertex set-up, reflection, expansion, contraction,
@@ -3365,18 +3405,21 @@ struct LaneSimplex {
     const float fac1 = (float)((1.0 - fac) / nd);
     const float fac2 = fac1 - fac;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) A.p[j] = sums[j] * fac1 - phi[j] * fac2;
+    for (int j = 0; j < 6; ++j) A.p[j] = fmaf(sums[j], fac1, -(phi[j] * fac2));
 #pragma unroll
     for (int j = 2; j < 6; ++j) if (s.en[j] == 0.0f) A.p[j] = init(j);
     return -1;
   }
 };
 
-// BATCHED Nelder-Mead iterations for small subsets, four per wave: each row
-// of 16 lanes owns one subset, runs LaneSimplex's state machine in all of its
+// BATCHED Nelder-Mead iterations for small subsets, 64 / W per wave: each row
+// of W lanes (16, or 8 when no subset has more than kNarrowSimplexMaxPixels
+// pixels) owns one subset, runs LaneSimplex's state machine in all of its
 // lanes (the same values in each) and evaluates the objective together --
-// lane l takes the pixels l, l + 16, ... (Solver::small_partial), the row adds
-// the partials (row_sum16).  The row keeps its lanes' reference values in
+// lane l takes the pixels l, l + W, ... (Solver::small_partial), the row adds
+// the partials (row_sum<W>).  The wave pays for the state machine's
+// bookkeeping once per step whatever the number of rows, so narrower rows
+// spend less of it per subset.  The row keeps its lanes' reference values in
 // registers and stages the deformed window of the current point, with a
 // margin of up to 3 px, into its LDS slot of `slot` floats (the last kNmFloats
 // of them hold LaneSimplex's vertex table); a point whose footprint leaves the
@@ -3388,7 +3431,7 @@ struct LaneSimplex {
 // numbered by their rank in the ballot.  A row whose item index is past the
 // end stays idle, and the wave exits when all of its rows are idle: every
 // simplex ends within max_robust iterations, so every wave reaches that exit.
-template <typename T, int INTERP, bool SQUARE, int KPER>
+template <typename T, int INTERP, bool SQUARE, int W, int KPER>
 __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KParams kp, DImage ref, DImage def,
                                                                               DSubsets ss, const double *fields,
                                                                               PassArgs pa, float *nm_out,
@@ -3396,9 +3439,9 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
   extern __shared__ float lds[];
   const Items items{pa.items, pa.item_count, ss.order};
   const int total = items.total(ss.n);
-  const int lane = (int)(threadIdx.x % kWave), l = lane & 15;
+  const int lane = (int)(threadIdx.x % kWave), l = lane & (W - 1);
   // the row's slot: the window, then the vertex table
-  lds_float *const win = (lds_float *)lds + (size_t)(threadIdx.x >> 4) * slot;
+  lds_float *const win = (lds_float *)lds + (size_t)(threadIdx.x / W) * slot;
   const int cap = slot - kNmFloats;
   LaneSimplex<T, INTERP, SQUARE> m(kp, ref, def, win + cap, l == 0);
   auto &s = m.s;
@@ -3419,12 +3462,12 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
       if (lane == leader) base = atomicAdd(counter, __popcll(want));
       base = __builtin_amdgcn_readlane(base, leader);
       if (need) {
-        const int k = base + __popcll(want & ((1ull << (lane & ~15)) - 1ull));
+        const int k = base + __popcll(want & ((1ull << (lane & ~(W - 1))) - 1ull));
         live = k < total;
         if (live) {
           gid = items.at(k);
           m.start(ss, gid, fields);
-          s.template small_walk<KPER>(l, [&](int q, int x, int y, bool) { rv[q] = s.ref_value(x - ref.ox, y - ref.oy); });
+          s.template small_walk<W, KPER>(l, [&](int q, int x, int y, bool) { rv[q] = s.ref_value(x - ref.ox, y - ref.oy); });
           wr = 0;
         }
         need = false;
@@ -3458,7 +3501,7 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
           const float inv = 1.0f / (float)wp;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 #pragma unroll 4
-          for (int c = l; c < wp * wr; c += 16) {
+          for (int c = l; c < wp * wr; c += W) {
             const int r = (int)(((float)c + 0.5f) * inv);  // exact: c < 2^12
             win[c] = (float)P[r * def.w + (c - r * wp)];
           }
@@ -3482,9 +3525,9 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
         return keys_sum(win + ((iy - 2) * wp + ix - 2), wp, wx, wy);
       };
       double acc[6] = {0, 0, 0, 0, 0, 0};
-      s.template small_partial<KPER>(m.A, l, acc, smp, [&](int, int, int k) { return rv[k]; });
+      s.template small_partial<W, KPER>(m.A, l, acc, smp, [&](int, int, int k) { return rv[k]; });
 #pragma unroll
-      for (int k = 0; k < 6; ++k) acc[k] = row_sum16(acc[k]);
+      for (int k = 0; k < 6; ++k) acc[k] = row_sum<W>(acc[k]);
       int active;
       float contrast;
       const float g = s.gamma_of(acc, active, contrast);
@@ -3504,17 +3547,22 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
   }
 }
 
-// LDS floats of one row's window slot in simplex_groups_kernel (16 rows per
-// 256-thread workgroup, four workgroups per CU: 150 KB).
-constexpr int kGroupSlotFloats = 600;
+// LDS floats of one row's slot in simplex_groups_kernel, rows of W lanes: 16
+// rows of 16 lanes (600 floats each) or 32 rows of 8 lanes (312 floats: a 16 x 16
+// window and the vertex table) per 256-thread workgroup; four workgroups per
+// CU: 150 KB or 156 KB.
+template <int W>
+constexpr int group_slot_floats() {
+  return W == 8 ? 312 : 600;
+}
 
-template <typename T, int INTERP, bool SQUARE, int KPER>
+template <typename T, int INTERP, bool SQUARE, int W, int KPER>
 hipError_t launch_simplex_lanes_variant(const KParams &kp, const DImage &ref, const DImage &def, const DSubsets &subs,
                                         const double *fields, const PassArgs &pa, float *nm_out, int *counter,
                                         int groups, hipStream_t stream) {
-  hipLaunchKernelGGL((simplex_groups_kernel<T, INTERP, SQUARE, KPER>), dim3(groups), dim3(256),
-                     (size_t)16 * kGroupSlotFloats * sizeof(float), stream, kp, ref, def, subs, fields, pa, nm_out,
-                     counter, kGroupSlotFloats);
+  hipLaunchKernelGGL((simplex_groups_kernel<T, INTERP, SQUARE, W, KPER>), dim3(groups), dim3(256),
+                     (size_t)(256 / W) * group_slot_floats<W>() * sizeof(float), stream, kp, ref, def, subs, fields,
+                     pa, nm_out, counter, group_slot_floats<W>());
   return hipGetLastError();
 }
 
@@ -3701,15 +3749,19 @@ hipError_t launch_variant(const Launch &L, const KParams &kp, const DImage &ref,
   const size_t shm = (size_t)L.waves_per_group * L.lds_floats_per_wave * sizeof(float);
   // slots larger than a sixteenth of the CU's 160 KB leave fewer than four waves
   // per SIMD: the variant with twice the registers then costs no occupancy at
-  // two waves per SIMD and spills nothing
-  if (L.lds_floats_per_wave > kWideSlotFloats)
-    hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX, 2>), dim3(L.groups),
-                       dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
-                       L.lds_floats_per_wave);
-  else
-    hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX>), dim3(L.groups),
-                       dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
-                       L.lds_floats_per_wave);
+  // two waves per SIMD and spills nothing (built for the Gauss-Newton kernel
+  // with reference gradients, the large-conformal-subset case)
+  if constexpr (!SX && !DEFGRAD) {
+    if (L.lds_floats_per_wave > kWideSlotFloats) {
+      hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX, 2>), dim3(L.groups),
+                         dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
+                         L.lds_floats_per_wave);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX>), dim3(L.groups),
+                     dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields, pa,
+                     L.lds_floats_per_wave);
   return hipGetLastError();
 }
 

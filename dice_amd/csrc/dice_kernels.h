@@ -50,6 +50,7 @@ struct KParams {
   int uncertainty;
   int first_frame;
   int has_forced;        // some subset has force[i] set: a GN primary pass hands them to a Nelder-Mead pass
+  int small_w;           // lanes of the fixed summation order of small subsets' objective (8 or 16)
   // lean kernel: per-subset cache of kLeanCacheDoubles doubles (or null), and the
   // generation its entries must carry to be current (entries of S that is not
   // positive definite carry -generation)
@@ -136,6 +137,9 @@ struct Launch {
 // BATCHED mode 16 lanes per subset (simplex_groups_kernel): a wave then works
 // on four subsets instead of spreading a few dozen pixels over 64 lanes.
 constexpr int kLaneSimplexMaxPixels = 256;
+// When no subset has more than this many pixels, rows of 8 lanes (eight subsets
+// per wave) instead of 16.
+constexpr int kNarrowSimplexMaxPixels = 88;
 // Per-subset result of simplex_groups_kernel: the last evaluated point (6
 // floats), then the iteration count and the optimizer status as int bits.
 // Status kNmNoResult: the kernel left the subset (its window did not fit the
