@@ -312,8 +312,10 @@ struct Affine {
   }
   // displacement of pixel offset (dx, dy) from the centroid: x' - cx, y' - cy
   __device__ __forceinline__ void disp(float dx, float dy, float &ox, float &oy) const {
-    ox = p[0] + fmaf(a11, dx, a12 * dy);
-    oy = p[1] + fmaf(a21, dx, a22 * dy);
+    // (the dx products first: a pixel walk that keeps dx and steps dy computes
+    // them once per column)
+    ox = p[0] + fmaf(a12, dy, a11 * dx);
+    oy = p[1] + fmaf(a22, dy, a21 * dx);
   }
   // residual row; (Gx, Gy) is the gradient to use (rotated ref or interpolated def)
   __device__ __forceinline__ void residual(float dx, float dy, float Gx, float Gy, float r[6]) const {
@@ -605,6 +607,9 @@ struct Solver {
   // blocked image pixels of the subset: box (mx0, my0, mw, mh), bits at mb (mw = 0: none)
   int mx0, my0, mw, mh;
   const gu32 *mb;
+  // a workgroup's LDS tables for the objective of subsets of at most
+  // kLaneSimplexMaxPixels pixels (3 kLaneSimplexMaxPixels floats), or null
+  lds_float *small_tab = nullptr;
   bool forced;  // Nelder-Mead only
   float meanF;
   float en[6];  // 1 for enabled parameters
@@ -1317,10 +1322,10 @@ struct Solver {
       const double d = smp(ox, oy);
       acc[0] += 1.0;
       acc[1] += d;
-      acc[2] += d * d;
+      acc[2] = fma(d, d, acc[2]);
       acc[3] += r;
-      acc[4] += r * r;
-      acc[5] += d * r;
+      acc[4] = fma(r, r, acc[4]);
+      acc[5] = fma(d, r, acc[5]);
     });
   }
 
@@ -1333,11 +1338,77 @@ struct Solver {
     double acc[6] = {0, 0, 0, 0, 0, 0};  // n, sd, sdd, sr, srr, sdr
     if constexpr (G::kSize >= 16) {
       if (n <= kLaneSimplexMaxPixels) {
+        const int w16 = kp.small_w == 8 && n <= kNarrowSimplexMaxPixels ? 8 : 16;
+        // every rank evaluates pixels rank, rank + G::kSize, ... into three LDS
+        // tables (kept, d, r): after the window in a wave's slot, or small_tab
+        // in a workgroup; then ranks 0..W-1 add them in small_partial's order --
+        // the same values in the same order, with the interpolation (and, in a
+        // workgroup, its global loads) spread over the whole group
+        lds_float *kt = small_tab;
+        if constexpr (kIsWave) {
+          const int cells = wpitch * wrows;
+          kt = cells + 3 * n <= win_cap ? win + cells : nullptr;
+        }
+        if (kt) {
+          lds_float *const dt = kt + n, *const rt = dt + n;
+          const int lane = grp.rank();
+          const float invs = S > 0 ? 1.0f / (float)S : 0.0f;
+          for (int i = lane; i < n; i += G::kSize) {
+            int x, y;
+            bool a = true;
+            if (S > 0) {
+              const int row = (int)(((float)i + 0.5f) * invs);  // exact: i < 2^8, S <= 16
+              x = max(cx - hw + (i - row * S), 0);
+              y = max(cy - hw + row, 0);
+            } else if (pk) {
+              const uint32_t w = pk[i];
+              x = (int)(w & 0x7fffu);
+              y = (int)((w >> 15) & 0x7fffu);
+              a = (w >> 30) == 0u;
+            } else {
+              x = max(px[i], 0);
+              y = max(py[i], 0);
+              a = pact ? (pact[i] != 0) : true;
+            }
+            float kf = 0.0f, d = 0.0f, r = 0.0f;
+            if (a) {
+              r = ref_value(x - ref.ox, y - ref.oy);
+              float ox, oy;
+              A.disp((float)(x - cx), (float)(y - cy), ox, oy);
+              if (kept(ox, oy)) {
+                kf = 1.0f;
+                d = sample(ox, oy);
+              }
+            }
+            kt[i] = kf;
+            dt[i] = d;
+            rt[i] = r;
+          }
+          grp.sync();
+          if (lane < w16) {
+#pragma unroll 1
+            for (int i = lane; i < n; i += w16) {
+              if (kt[i] == 0.0f) continue;
+              const double d = dt[i], r = rt[i];
+              acc[0] += 1.0;
+              acc[1] += d;
+              acc[2] = fma(d, d, acc[2]);
+              acc[3] += r;
+              acc[4] = fma(r, r, acc[4]);
+              acc[5] = fma(d, r, acc[5]);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] = w16 == 8 ? row_sum8(acc[k]) : row_sum16(acc[k]);
+          }
+          grp.bcast0(acc);  // a workgroup: ends with a barrier
+          if constexpr (kIsWave) grp.sync();
+          return gamma_of(acc, active, contrast);
+        }
         // the order of simplex_groups_kernel (small_partial), in ranks 0..W-1
         auto smp = [&](float ox, float oy) { return sample(ox, oy); };
         auto refv = [&](int x, int y, int) { return ref_value(x - ref.ox, y - ref.oy); };
         const int r0 = grp.rank();
-        if (kp.small_w == 8 && n <= kNarrowSimplexMaxPixels) {
+        if (w16 == 8) {
           if (r0 < 8) {
             small_partial<8, kNarrowSimplexMaxPixels / 8, false>(A, r0, acc, smp, refv);
 #pragma unroll
@@ -2912,8 +2983,11 @@ struct Solver {
   // wave whose window overflows its LDS slot pushes the subset to ov_list and
   // leaves it to the PLAIN kernel (nothing it wrote is an input of the solve).
   // nm: per-subset Nelder-Mead results of simplex_groups_kernel for this pass
-  // (SIMPLEX_OPT only), or null to run the simplex here.
-  template <bool SIMPLEX_OPT>
+  // (SIMPLEX_OPT only), or null to run the simplex here.  NM_ONLY: the
+  // instantiation holds no simplex (and so needs far fewer registers); a
+  // subset without a result goes to ov_list, and the PLAIN kernel runs its
+  // simplex.
+  template <bool SIMPLEX_OPT, bool NM_ONLY = false>
   __device__ __forceinline__ void run(const DSubsets &ss, int gid, double *fields, int pass, int *fb_list,
                                       int *fb_count, int *ov_list, int *ov_count, const float *nm = nullptr) {
     const int nsub = ss.n;
@@ -2982,6 +3056,8 @@ struct Solver {
         for (int k = 0; k < 6; ++k) A.p[k] = r[k];
         iters = __float_as_int(r[6]);
         corr = __float_as_int(r[7]);
+      } else if constexpr (NM_ONLY) {
+        corr = kOverflow;
       } else {
         corr = simplex(A, iters);
       }
@@ -3423,10 +3499,11 @@ struct LaneSimplex {
 // registers and stages the deformed window of the current point, with a
 // margin of up to 3 px, into its LDS slot of `slot` floats (the last kNmFloats
 // of them hold LaneSimplex's vertex table); a point whose footprint leaves the
-// window restages it.  A point whose window does not fit the slot ends the
-// row's work on the subset with status kNmNoResult, and the pass that follows
-// runs that subset's simplex itself (in the same summation order, so with the
-// same result).  Rows that finish a subset write its result to
+// window restages it.  A point whose window does not fit the slot reads its
+// taps from the image (bilinear) or ends the row's work on the subset with
+// status kNmNoResult (Keys): the epilogue kernel hands such a subset to the
+// PLAIN kernel, which runs its simplex (in the same summation order, so with
+// the same result).  Rows that finish a subset write its result to
 // nm_out and take the next items: one atomic per wave and refill, the rows
 // numbered by their rank in the ballot.  A row whose item index is past the
 // end stays idle, and the wave exits when all of its rows are idle: every
@@ -3508,7 +3585,11 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         }
       }
-      if (!fits) {
+      // bilinear: a point whose window does not fit reads its taps from the
+      // image itself (bilinear_sum's arithmetic does not depend on where the
+      // taps come from); Keys: the subset goes back to the epilogue kernel
+      const bool glob = INTERP == 0 && !fits;
+      if (!fits && INTERP != 0) {
         if (l == 0) nm_out[(size_t)gid * kNmStride + 7] = __int_as_float(kNmNoResult);
         need = true;
         continue;
@@ -3517,6 +3598,10 @@ __global__ __launch_bounds__(256) DICE_LANE_BOUNDS void simplex_groups_kernel(KP
       auto smp = [&](float ox, float oy) -> float {
         const float flx = floorf(ox), fly = floorf(oy);
         const float tx = ox - flx, ty = oy - fly;
+        if (INTERP == 0 && glob) {
+          const int gx = s.cx - def.ox + (int)flx, gy = s.cy - def.oy + (int)fly;
+          return bilinear_sum(Pix<T>::ptr(def.pix) + (gy * def.w + gx), def.w, tx, ty);
+        }
         const int ix = s.cx - def.ox + (int)flx - wx0, iy = s.cy - def.oy + (int)fly - wy0;
         if (INTERP == 0) return bilinear_sum(win + (iy * wp + ix), wp, tx, ty);
         float wx[6], wy[6];
@@ -3571,10 +3656,12 @@ template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT>
 __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams kp, DImage ref, DImage def,
                                                                         DSubsets ss, double *fields, PassArgs pa) {
   __shared__ double red[4 * kMaxRed];
+  __shared__ float small_tab[3 * kLaneSimplexMaxPixels];
   const Items items{pa.items, pa.item_count, ss.order};
   const int total = items.total(ss.n);
   BlockGroup g{(lds_double *)red};
   Solver<T, BlockGroup, INTERP, DEFGRAD> s(kp, ref, def, g, (lds_float *)red, 0);
+  s.small_tab = (lds_float *)small_tab;
 #pragma unroll 1
   for (int k = blockIdx.x; k < total; k += gridDim.x)
     s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, nullptr, nullptr,
@@ -3593,7 +3680,8 @@ __global__ __launch_bounds__(kPlainThreads) void correlate_plain_kernel(KParams 
 #ifndef DICE_BATCHED_WAVES_PER_EU
 #define DICE_BATCHED_WAVES_PER_EU 4
 #endif
-template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT, int WPE = DICE_BATCHED_WAVES_PER_EU>
+template <typename T, int INTERP, bool DEFGRAD, bool SIMPLEX_OPT, int WPE = DICE_BATCHED_WAVES_PER_EU,
+          bool NM_ONLY = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void correlate_batched_kernel(KParams kp, DImage ref, DImage def, DSubsets ss,
                                                                 double *fields, PassArgs pa, int lds_per_wave) {
   extern __shared__ float lds[];
@@ -3620,8 +3708,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       ++visited;
       continue;
     }
-    s.template run<SIMPLEX_OPT>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
-                                pa.ov_count, pa.nm_result);
+    s.template run<SIMPLEX_OPT, NM_ONLY>(ss, items.at(k), fields, pa.pass, pa.fb_list, pa.fb_count, pa.ov_list,
+                                         pa.ov_count, pa.nm_result);
   }
 }
 
@@ -3751,6 +3839,16 @@ hipError_t launch_variant(const Launch &L, const KParams &kp, const DImage &ref,
   // per SIMD: the variant with twice the registers then costs no occupancy at
   // two waves per SIMD and spills nothing (built for the Gauss-Newton kernel
   // with reference gradients, the large-conformal-subset case)
+  // the epilogue of simplex_groups_kernel's results: the variant without a
+  // simplex (the subsets it has no result for go to the PLAIN kernel)
+  if constexpr (SX && !DEFGRAD) {
+    if (pa.nm_result && pa.ov_list) {
+      hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX, DICE_BATCHED_WAVES_PER_EU, true>),
+                         dim3(L.groups), dim3(L.waves_per_group * kWave), shm, stream, kp, ref, def, subs, fields,
+                         pa, L.lds_floats_per_wave);
+      return hipGetLastError();
+    }
+  }
   if constexpr (!SX && !DEFGRAD) {
     if (L.lds_floats_per_wave > kWideSlotFloats) {
       hipLaunchKernelGGL((correlate_batched_kernel<T, INTERP, DEFGRAD, SX, 2>), dim3(L.groups),

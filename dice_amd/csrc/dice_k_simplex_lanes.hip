@@ -1,4 +1,4 @@
-// simplex_groups_kernel: BATCHED Nelder-Mead iterations with 16 lanes per
+// simplex_groups_kernel: BATCHED Nelder-Mead iterations with 8 or 16 lanes per
 // subset (template instantiations of dice_device.h per pixel storage /
 // interpolation / subset shape, and per bound on the pixels of one lane).
 #include "dice_device.h"

@@ -660,10 +660,10 @@ hipError_t launch_reset_listed(const int *list, const int *count, const double *
 
 // One optimizer pass over `items` (nullptr: all subsets).  BATCHED mode runs the
 // persistent kernel, then the PLAIN kernel over the subsets it handed over.
-// A Nelder-Mead pass with L.lane_simplex first runs the iterations 16 lanes per
-// subset (results in nm); the persistent kernel then does the checks and the
-// epilogue of each subset around them (and the simplex of any subset that
-// kernel left with kNmNoResult).
+// A Nelder-Mead pass with L.lane_simplex first runs the iterations in rows of 8
+// or 16 lanes per subset (results in nm); the persistent kernel then does the
+// checks and the epilogue of each subset around them, and hands any subset
+// that kernel left with kNmNoResult to the PLAIN kernel, which runs its simplex.
 static hipError_t launch_pass(bool simplex, int storage, const Launch &L, const KParams &kp, const DImage &ref,
                               const DImage &def, const DSubsets &subs, double *fields, PassArgs pa, int *ov_list,
                               int *ov_count, float *nm, hipStream_t stream, int *lean_list = nullptr) {
