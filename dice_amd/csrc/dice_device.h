@@ -1933,17 +1933,81 @@ struct Solver {
             } else {
               // (x mod 2, y mod 2) classes as in gamma_adjoint: with the warp
               // within ||M - I|| <= 0.45, two samples of a class never share a cell
+              // MODE 1 takes two samples of a class per lane at a time when every
+              // lane's first one is kept: the cells of all the class's samples
+              // differ at each tap, so both reads of a tap go out before the one
+              // wait.  A lane without a second sample points it at its first with
+              // a zero contribution and writes it first, so the first's write is
+              // the one that stays; no lane branches.
+              auto pair = [&](int k0, int nc, int cnt, int pa, int pb, float invc) {
+                lds_float *rp[2];
+                float gv[2], wx[2][6], wy[2][6];
+                bool on[2];
+                int xs[2], ys[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                  const int kk = k0 + lane + j * kWave;
+                  on[j] = kk < cnt;
+                  const int kc = on[j] ? kk : 0;
+                  const int rr = (int)(((float)kc + 0.5f) * invc);  // exact: kc < 2^20
+                  ys[j] = pa + 2 * rr;
+                  xs[j] = pb + 2 * (kc - rr * nc);
+                  float ox, oy;
+                  A.disp((float)(xs[j] - hw), (float)(ys[j] - hw), ox, oy);
+                  if (!allk && on[j]) on[j] = kept(ox, oy);
+                  const int i = ys[j] * S + xs[j];
+                  gv[j] = on[j] ? (float)(a * ((double)dtab[i] - md) - b * ((double)rtab[i] - mr)) : 0.0f;
+                  const float flx = floorf(ox), fly = floorf(oy);
+                  const float tx = ox - flx, ty = oy - fly;
+                  constexpr int lead = INTERP == 0 ? 0 : 2;
+                  const int c0 = cx - def.ox + (int)flx - lead - wx0, r0 = cy - def.oy + (int)fly - lead - wy0;
+                  if (INTERP == 0) {
+                    wx[j][0] = 1.0f - tx; wx[j][1] = tx; wy[j][0] = 1.0f - ty; wy[j][1] = ty;
+                  } else {
+                    keys_weights(tx, wx[j]);
+                    keys_weights(ty, wy[j]);
+                  }
+                  rp[j] = accl + (__umul24(r0, apitch) + c0);
+                }
+                if (__ballot(!on[0]) != 0ull) {
+                  // a lane's first sample is missing or not kept: one at a time
+                  if (on[0]) pixel(xs[0], ys[0], ys[0] * S + xs[0], std::integral_constant<int, 1>{});
+                  if (on[1]) pixel(xs[1], ys[1], ys[1] * S + xs[1], std::integral_constant<int, 1>{});
+                  return;
+                }
+                if (!on[1]) rp[1] = rp[0];
+                constexpr int taps = INTERP == 0 ? 2 : 6;
+#pragma unroll
+                for (int m = 0; m < taps; ++m) {
+                  const float gm0 = gv[0] * wy[0][m], gm1 = gv[1] * wy[1][m];
+#pragma unroll
+                  for (int t = 0; t < taps; ++t) {
+                    const float v0 = rp[0][t], v1 = rp[1][t];
+                    rp[1][t] = v1 + gm1 * wx[1][t];
+                    rp[0][t] = v0 + gm0 * wx[0][t];
+                    asm volatile("" ::: "memory");
+                  }
+                  rp[0] += apitch;
+                  rp[1] += apitch;
+                }
+              };
               auto classes = [&](auto mode) {
+                constexpr int MODE = decltype(mode)::value;
 #pragma unroll 1
                 for (int ph = 0; ph < 4; ++ph) {
                   const int pa = ph >> 1, pb = ph & 1;
                   const int nc = (S - pb + 1) >> 1, cnt = ((S - pa + 1) >> 1) * nc;
                   const float invc = 1.0f / (float)nc;
+                  if constexpr (MODE == 1) {
 #pragma unroll 1
-                  for (int k = lane; k < cnt; k += kWave) {
-                    const int rr = (int)(((float)k + 0.5f) * invc);  // exact: k < 2^20
-                    const int y = pa + 2 * rr, x = pb + 2 * (k - rr * nc);
-                    pixel(x, y, y * S + x, mode);
+                    for (int k0 = 0; k0 < cnt; k0 += 2 * kWave) pair(k0, nc, cnt, pa, pb, invc);
+                  } else {
+#pragma unroll 1
+                    for (int k = lane; k < cnt; k += kWave) {
+                      const int rr = (int)(((float)k + 0.5f) * invc);  // exact: k < 2^20
+                      const int y = pa + 2 * rr, x = pb + 2 * (k - rr * nc);
+                      pixel(x, y, y * S + x, mode);
+                    }
                   }
                 }
               };
