@@ -130,12 +130,14 @@ struct Launch {
   int groups;                  // batched: persistent grid size
   int lds_floats_per_wave;     // batched: LDS window budget
   int lane_simplex;            // batched: 0, or the most pixels of a subset (<= kLaneSimplexMaxPixels):
-                               // Nelder-Mead passes run 16 lanes per subset first (simplex_groups_kernel)
+                               // Nelder-Mead passes run rows of 8 or 16 lanes per subset first
+                               // (simplex_groups_kernel)
 };
 
 // Subsets of at most this many pixels run the Nelder-Mead iterations of the
-// BATCHED mode 16 lanes per subset (simplex_groups_kernel): a wave then works
-// on four subsets instead of spreading a few dozen pixels over 64 lanes.
+// BATCHED mode in rows of 16 lanes per subset (simplex_groups_kernel): a wave
+// then works on four subsets instead of spreading a few dozen pixels over 64
+// lanes.
 constexpr int kLaneSimplexMaxPixels = 256;
 // When no subset has more than this many pixels, rows of 8 lanes (eight subsets
 // per wave) instead of 16.
@@ -276,8 +278,9 @@ hipError_t launch_correlate(const Launch &L, int storage, const KParams &kp, con
 hipError_t launch_gamma(const Launch &L, int storage, const KParams &kp, const DImage &ref, const DImage &def,
                         const DSubsets &subs, const GammaArgs &ga, int *work, hipStream_t stream);
 
-// Enqueue the Nelder-Mead iterations of the items of pa, 16 lanes per subset
-// (subsets of at most max_pixels <= kLaneSimplexMaxPixels pixels): results to
+// Enqueue the Nelder-Mead iterations of the items of pa, rows of 8 lanes per
+// subset when max_pixels <= kNarrowSimplexMaxPixels, else 16 (subsets of at
+// most max_pixels <= kLaneSimplexMaxPixels pixels): results to
 // nm_out (kNmStride floats per subset); counter: one int, zeroed before the
 // launch.
 hipError_t launch_simplex_lanes(int storage, int interp, int max_pixels, const KParams &kp, const DImage &ref,
