@@ -325,16 +325,17 @@ def test_fallback_lane_path(ctx, mode):
     assert np.abs(fg[dh.F["DISP_X"]] - fo[o.F["DISP_X"]])[ok].max() < 2e-2
 
 
-@pytest.mark.parametrize("size,delta", [(9, None), (15, None), (9, 1.5)], ids=["9x9", "15x15", "9x9-wide-steps"])
+@pytest.mark.parametrize("size,delta", [(9, None), (15, None), (9, 2.0)], ids=["9x9", "15x15", "9x9-wide-steps"])
 @pytest.mark.parametrize("interp", [dh.BILINEAR, dh.KEYS_FOURTH], ids=["bilinear", "keys"])
 def test_simplex_plain_and_batched_paths_agree(ctx, size, delta, interp):
     """Subsets of at most 256 pixels sum the Nelder-Mead objective in one order in both modes (8 strided
     lane partials for subsets of at most 88 pixels, else 16, then a pairwise tree), so the BATCHED
     iterations in rows of 8 or 16 lanes and the PLAIN workgroup follow the same path: identical statuses,
     iteration counts and final parameters on every subset (the epilogue's other sums may differ in
-    rounding).  The wide-steps case starts the simplex with strain and rotation steps of 1.5, so that
-    trial points stretch a 9 x 9 subset past the 256-cell window of its row: bilinear rows then read
-    those points' taps from the image, Keys rows hand the subset to the PLAIN kernel."""
+    rounding).  The wide-steps case starts the simplex with strain and rotation steps of 2, so that the
+    first strain and shear vertices already stretch a 9 x 9 subset past the 256-cell window of its row
+    (27 x 11 cells bilinear): bilinear rows then read those points' taps from the image, Keys rows hand
+    the subset to the PLAIN kernel through the overflow list."""
     d = load("speckle_512.npz")
     cx, cy = dh.regular_grid(512, 512, size, 12)
     p = dh.default_params()
@@ -344,7 +345,10 @@ def test_simplex_plain_and_batched_paths_agree(ctx, size, delta, interp):
     if delta is not None:
         p.robust_delta_theta = delta
     fb = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, size, dh.MODE_BATCHED)
+    handed_over = ctx.pass_counts()[1]
     fp = run_gpu(ctx, p, d["ref"], d["defm"], cx, cy, size, dh.MODE_PLAIN)
+    if delta is not None and interp == dh.KEYS_FOURTH:
+        assert handed_over > 0
     assert np.array_equal(fb[dh.F["STATUS"]], fp[dh.F["STATUS"]])
     assert np.array_equal(fb[dh.F["ITERATIONS"]], fp[dh.F["ITERATIONS"]])
     for name in ("DISP_X", "DISP_Y", "ROT_Z", "NS_XX", "NS_YY", "SS_XY"):
