@@ -152,6 +152,28 @@ def test_adjoint_image_batched_matches_plain(scene, strain):
     c.close()
 
 
+def test_adjoint_image_dense_overlap_batched_matches_plain():
+    """Windows that overlap as in the 1536^2 differentiation workload (21 x 21 subsets every 6 px, so
+    about 20 subsets add into each pixel, from waves on every XCD): the BATCHED image gradient (LDS
+    accumulator, one device-scope atomic per window cell) against the PLAIN kernel's (one atomic per
+    tap).  A lost update would move a pixel by a whole subset's contribution."""
+    ref, de, field = synth.config4(size=SIZE, peak=1.5, seed=2)
+    cx, cy = dh.regular_grid(SIZE, SIZE, S, 6)
+    rng = np.random.default_rng(11)
+    P = params_from_field(field, cx, cy)
+    P[:, :2] += rng.normal(0, 0.2, (len(P), 2))
+    w = rng.uniform(0.5, 1.5, len(P))
+    c = make_ctx(ref, de, cx, cy)
+    dpb, ib = adjoint(c, P, w, dh.MODE_BATCHED, shape=de.shape)
+    dpp, ip = adjoint(c, P, w, dh.MODE_PLAIN, shape=de.shape)
+    assert len(cx) > 6000
+    assert np.count_nonzero(ip) > 0.8 * ip.size
+    assert np.linalg.norm(ib - ip) / np.linalg.norm(ip) < 1e-5
+    assert np.abs(ib - ip).max() <= 1e-4 * np.abs(ip).max()
+    assert np.linalg.norm(dpb - dpp) / np.linalg.norm(dpp) < 1e-6
+    c.close()
+
+
 def test_adjoint_direction_matches_gauss_newton_gradient(scene):
     """Near the optimum the ZNSSD gradient points along the Gauss-Newton gradient q of computeUpdateFast
     (ZSSD residual, the deformed image's interpolated FD gradients): dgamma/dp ~ (2 / |d - dbar|^2) q.
